@@ -1,0 +1,181 @@
+/*
+ * turbins_gpu.h -- C-ABI of the MI355X (gfx950) implementation of the TURBINS
+ * per-timestep hot path: ENO convection, momentum/concentration Helmholtz
+ * predictors with immersed-boundary rows, pressure Poisson solve and
+ * projection.  Every entry point is extern "C", takes plain pointers and
+ * sizes, and returns an int status:
+ *      0  ok
+ *     >0  finished but not converged / fallback taken (see the stats struct)
+ *     <0  hard error; tb_last_error(ctx) holds the message.
+ * Nothing here calls exit().  One host thread per context; the context owns
+ * one HIP stream and all device buffers.  Host pointers are borrowed for the
+ * duration of a call only.  Calls are synchronous unless named _async.
+ *
+ * Field layout on the host side is the reference's PETSc DA natural order:
+ * (NX+1)*(NY+1)*(NZ+1) doubles, x fastest, where NX,NY,NZ are the numbers of
+ * physical cells (params->NX etc.; the extra "half cell" plane is
+ * Grid.c:59-62).
+ *
+ * Reference interfaces replaced (file:line in mmnasr/TURBINS src/):
+ *   tb_eno_conv          ENO_compute_convective_terms      Include/ENO.h:6
+ *                        (lifted from one grid line to whole fields, with the
+ *                        wall ghosts of Velocity_*_ENO_ghost_cells
+ *                        Include/Velocity.h and Conc_ENO_ghost_cells
+ *                        Include/Conc.h, and the transposes
+ *                        Velocity_transpose_velocities / Velocity_cell_center)
+ *   tb_set_rhs           Velocity_u/v/w_set_RHS            Include/Velocity.h:18-20
+ *                        Conc_set_RHS                      Include/Conc.h:30
+ *   tb_helmholtz_solve   Velocity_solve                    Include/Velocity.h:15
+ *                        Conc_solve                        Include/Conc.h:29
+ *                        (the diagonal update of Velocity_modify_diagonal /
+ *                        Conc_modify_diagonal is applied from dt)
+ *   tb_poisson_rhs       Pressure_set_RHS                  Include/Pressure.h:9
+ *   tb_poisson_solve     Pressure_solve                    Include/Pressure.h:11
+ *   tb_project           Pressure_project_velocity         Include/Pressure.h:10
+ *   tb_div_max           Pressure_compute_velocity_divergence Include/Pressure.h:12
+ *   tb_setup_geometry    Grid_identify_geometry            Include/Grid.h:20
+ *                        + Surface_find_exact_sdf / Surface_find_q_sdf
+ *                        + Immersed_setup_q_immersed_nodes Include/Immersed.h:13
+ *   tb_rk_average        Velocity_rk_average / Conc_rk_average
+ *   tb_store_old         Velocity_store_old_data / Conc_store_old_data
+ */
+#ifndef TURBINS_GPU_H
+#define TURBINS_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* quantities (grids of the MAC layout) */
+#define TB_Q_U 0
+#define TB_Q_V 1
+#define TB_Q_W 2
+#define TB_Q_C 3
+
+/* node status codes: definitions.h:94-99 */
+#define TB_OUTSIDE 1
+#define TB_SOLID 2
+#define TB_FLUID 3
+#define TB_IMMERSED 5
+#define TB_BOUNDARY 7
+#define TB_BUFFER_SOLID 11
+
+/* concentration field types: definitions.h:44-46 */
+#define TB_SALINITY 1
+#define TB_PARTICLE 2
+
+/* field ids for upload / download */
+enum {
+    TB_F_U = 0, TB_F_V, TB_F_W, TB_F_C,
+    TB_F_DPDX, TB_F_DPDY, TB_F_DPDZ, TB_F_PHI,
+    TB_F_CONV_U, TB_F_CONV_V, TB_F_CONV_W, TB_F_CONV_C,
+    TB_F_RHS_U, TB_F_RHS_V, TB_F_RHS_W, TB_F_RHS_C,
+    TB_F_PRHS,              /* Poisson right-hand side  b = -div(u*)/dt      */
+    TB_F_DIV,               /* |div u| per cell (G_divergence)                */
+    TB_F_SDF_U, TB_F_SDF_V, TB_F_SDF_W, TB_F_SDF_C,
+    TB_F_SCRATCH0, TB_F_SCRATCH1,   /* free vectors for operator tests       */
+    TB_F_COUNT
+};
+
+/* Poisson modes */
+#define TB_POIS_FP64_PCG 0      /* FP64 CG preconditioned by an FP64 V-cycle */
+#define TB_POIS_MIXED_IR 1      /* FP32 V-cycles inside FP64 iterative refinement */
+#define TB_POIS_LEARNED_IR 2    /* learned initial guess + mode 1, fallback to mode 0 */
+
+typedef struct tb_config {
+    int32_t nx, ny, nz;         /* physical cells in x, y, z (params->NX..NZ) */
+    double lx, ly, lz;          /* domain lengths; origin at 0 (params->Lx..Lz) */
+    double re;                  /* Reynolds number (params->Re) */
+    double pe;                  /* Peclet number of the concentration field (params->Pe[0]) */
+    int32_t eno_order;          /* 1..3 (params->ENO_scheme_order) */
+    int32_t wall_noslip[6];     /* velocity walls x0,x1,y0,y1,z0,z1: 1 no-slip, 0 free-slip
+                                   (Boundary.h:5-20 selects these at compile time) */
+    double buoyancy;            /* multiplier of the interpolated c in the v right-hand
+                                   side (Velocity.c:1883 ships 0.0) */
+    int32_t conc_type;          /* TB_SALINITY or TB_PARTICLE (params->conc_type[0]) */
+    double v_settle;            /* settling speed as used (Input.c:481 flips the sign:
+                                   negative = downward) */
+    int32_t sedimentation;      /* particle no-flux top wall ghost (Conc.c:1156-1181) */
+    double helm_rtol;           /* Helmholtz relative residual (params->resmax) */
+    int32_t helm_maxit;
+    double pois_rtol;           /* Poisson relative residual target */
+    int32_t pois_maxit;
+    int32_t pois_mode;          /* TB_POIS_* */
+    int32_t bottom_offset;      /* 1: topography is lifted by 2.001*dy_c[0] (Grid.c:913) */
+    int32_t mg_pre, mg_post;    /* red-black Gauss-Seidel sweeps per V-cycle side */
+    int32_t reserved[8];
+} tb_config;
+
+/* One immersed-boundary node (ImmersedNode, DataTypes.h:520-546), flattened. */
+typedef struct tb_ib_node {
+    int32_t i, j, k;            /* im_index */
+    int32_t n_fluid;            /* 8 (trilinear box) or 1 (nearest fluid node) or 0 */
+    int32_t fi[8], fj[8], fk[8];/* fluid_index[] */
+    double coef[8];             /* fluid_coef[] */
+    double boundary_coef;
+    double n[3];                /* unit normal */
+    double im[3];               /* im_point */
+    double control[3];          /* control_point */
+    double image[3];            /* image_point */
+} tb_ib_node;
+
+typedef struct tb_solve_stats {
+    int32_t iterations;         /* Krylov / refinement iterations */
+    int32_t converged;          /* 1 if the target was met */
+    int32_t fallback;           /* 1 if the plain FP64 path had to take over */
+    int32_t vcycles;            /* V-cycles applied (Poisson) */
+    double res0;                /* initial true relative residual */
+    double res;                 /* final true relative residual ||b-Ax||/||b|| */
+    double ms;                  /* device time, milliseconds */
+} tb_solve_stats;
+
+typedef struct tb_step_stats {
+    tb_solve_stats helm[2][4];  /* per RK stage, per quantity u,v,w,c */
+    tb_solve_stats pois[2];     /* per RK stage */
+    double div_max;             /* max |div u| after the step */
+    double ms;                  /* device time of the whole step */
+} tb_step_stats;
+
+typedef struct tb_ctx tb_ctx;
+
+/* lifecycle ------------------------------------------------------------- */
+tb_ctx *tb_create(const tb_config *cfg);
+void tb_destroy(tb_ctx *ctx);
+const char *tb_last_error(tb_ctx *ctx);
+int tb_device_bytes(tb_ctx *ctx, int64_t *bytes);
+
+/* geometry: SDF reinitialisation, node classification and IB tables.
+ * h_xz: bottom height at cell centres, (nz+1)*(nx+1) doubles, x fastest
+ * (grid->interface_position before the offset), or NULL for no solid.  */
+int tb_setup_geometry(tb_ctx *ctx, const double *h_xz);
+int tb_download_flags(tb_ctx *ctx, int q, uint8_t *out);
+int tb_ib_count(tb_ctx *ctx, int q);
+int tb_download_ib_table(tb_ctx *ctx, int q, tb_ib_node *out, int capacity);
+int tb_reinit_iterations(tb_ctx *ctx);
+
+/* fields ------------------------------------------------------------------ */
+int tb_upload_field(tb_ctx *ctx, int f, const double *host);
+int tb_download_field(tb_ctx *ctx, int f, double *host);
+
+/* per-operation mirrors of the reference entry points ---------------------- */
+int tb_eno_conv(tb_ctx *ctx, int q);
+int tb_set_rhs(tb_ctx *ctx, int q, double dt);
+int tb_helmholtz_apply(tb_ctx *ctx, int q, double dt, int f_in, int f_out);
+int tb_helmholtz_solve(tb_ctx *ctx, int q, double dt, tb_solve_stats *st);
+int tb_poisson_rhs(tb_ctx *ctx, double dt);
+int tb_poisson_apply(tb_ctx *ctx, int f_in, int f_out);
+int tb_poisson_solve(tb_ctx *ctx, tb_solve_stats *st);
+int tb_project(tb_ctx *ctx, double dt);
+int tb_div_max(tb_ctx *ctx, double *div_max, int32_t *ijk);
+int tb_store_old(tb_ctx *ctx);
+int tb_rk_average(tb_ctx *ctx, double old_frac, double new_frac);
+
+/* composite SSP-RK2 step (the missing driver's role) */
+int tb_step(tb_ctx *ctx, double dt, tb_step_stats *st);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,201 @@
+"""CPU oracle for the TURBINS hot path -- TEST INFRASTRUCTURE ONLY.
+
+A plain-C FP64 restatement of the reference algorithms (see tb_oracle.h and
+the file:line citations in the C sources).  Only tests/, the smoke check in
+__graft_entry__ and the cpu_baseline leg of bench.py may use it, and only as
+the checker or the timed CPU baseline; the product path never imports it.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from turbins_amd import abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        L = C.CDLL(path)
+        P = C.c_void_p
+        L.ora_create.restype = P
+        L.ora_create.argtypes = [C.POINTER(abi.Config)]
+        L.ora_destroy.argtypes = [P]
+        L.ora_get_grid_array.argtypes = [P, C.c_int, P]
+        L.ora_setup_geometry.argtypes = [P, P]
+        L.ora_reinit_iterations.argtypes = [P]
+        L.ora_get_flags.argtypes = [P, C.c_int, P]
+        L.ora_ib_count.argtypes = [P, C.c_int]
+        L.ora_get_ib.argtypes = [P, C.c_int, P, C.c_int]
+        L.ora_set_field.argtypes = [P, C.c_int, P]
+        L.ora_get_field.argtypes = [P, C.c_int, P]
+        L.ora_eno_line.argtypes = [C.c_int, C.c_int, P, P, P, P]
+        L.ora_gauss_jordan.argtypes = [C.c_int, P, P]
+        L.ora_eno_conv.argtypes = [P, C.c_int]
+        L.ora_set_rhs.argtypes = [P, C.c_int, C.c_double]
+        L.ora_helmholtz_apply.argtypes = [P, C.c_int, C.c_double, P, P]
+        L.ora_helmholtz_solve.argtypes = [P, C.c_int, C.c_double, C.c_double, C.c_int,
+                                          C.POINTER(abi.SolveStats)]
+        L.ora_poisson_rhs.argtypes = [P, C.c_double]
+        L.ora_poisson_apply.argtypes = [P, P, P]
+        L.ora_poisson_solve.argtypes = [P, C.c_double, C.c_int, C.POINTER(abi.SolveStats)]
+        L.ora_project.argtypes = [P, C.c_double]
+        L.ora_div_max.restype = C.c_double
+        L.ora_div_max.argtypes = [P, P]
+        L.ora_store_old.argtypes = [P]
+        L.ora_rk_average.argtypes = [P, C.c_double, C.c_double]
+        L.ora_step.argtypes = [P, C.c_double, C.POINTER(abi.StepStats)]
+        L.ora_gmg_levels.argtypes = [P, P, C.c_int]
+        _LIB = L
+    return _LIB
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+GRID_ARRAYS = ["xc", "yc", "zc", "xu", "yu", "zu", "xv", "yv", "zv", "xw", "yw", "zw",
+               "dx_c", "dy_c", "dz_c", "mxc", "myc", "mzc", "mxu", "myv", "mzw"]
+
+
+class Oracle:
+    """Same method names as turbins_amd.Solver, computed on the CPU."""
+
+    def __init__(self, cfg):
+        self.cfg = cfg
+        self.L = lib()
+        self.h = self.L.ora_create(C.byref(cfg))
+        if not self.h:
+            raise ValueError("bad config")
+        self.shape = (cfg.nz + 1, cfg.ny + 1, cfg.nx + 1)
+        self.NT = int(np.prod(self.shape))
+
+    def close(self):
+        if self.h:
+            self.L.ora_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def grid(self, name):
+        n = max(self.shape) + 1
+        out = np.zeros(n)
+        m = self.L.ora_get_grid_array(self.h, GRID_ARRAYS.index(name), _ptr(out))
+        return out[:m].copy()
+
+    def setup_geometry(self, h_xz):
+        if h_xz is None:
+            return self.L.ora_setup_geometry(self.h, None)
+        h = np.ascontiguousarray(h_xz, dtype=np.float64)
+        return self.L.ora_setup_geometry(self.h, _ptr(h))
+
+    def reinit_iterations(self):
+        return self.L.ora_reinit_iterations(self.h)
+
+    def flags(self, q):
+        out = np.zeros(self.NT, dtype=np.uint8)
+        self.L.ora_get_flags(self.h, q, _ptr(out))
+        return out
+
+    def ib_table(self, q):
+        n = self.L.ora_ib_count(self.h, q)
+        arr = (abi.IBNode * max(n, 1))()
+        self.L.ora_get_ib(self.h, q, C.cast(arr, C.c_void_p), n)
+        return list(arr)[:n]
+
+    def upload(self, name, a):
+        a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+        assert a.size == self.NT
+        self.L.ora_set_field(self.h, abi.F[name], _ptr(a))
+
+    def download(self, name):
+        out = np.zeros(self.NT)
+        self.L.ora_get_field(self.h, abi.F[name], _ptr(out))
+        return out
+
+    def eno_conv(self, q):
+        return self.L.ora_eno_conv(self.h, q)
+
+    def set_rhs(self, q, dt):
+        return self.L.ora_set_rhs(self.h, q, dt)
+
+    def helmholtz_apply(self, q, dt, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros(self.NT)
+        self.L.ora_helmholtz_apply(self.h, q, dt, _ptr(x), _ptr(y))
+        return y
+
+    def helmholtz_solve(self, q, dt, rtol=1e-14, maxit=500):
+        st = abi.SolveStats()
+        self.L.ora_helmholtz_solve(self.h, q, dt, rtol, maxit, C.byref(st))
+        return st
+
+    def poisson_rhs(self, dt):
+        return self.L.ora_poisson_rhs(self.h, dt)
+
+    def poisson_apply(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros(self.NT)
+        self.L.ora_poisson_apply(self.h, _ptr(x), _ptr(y))
+        return y
+
+    def poisson_solve(self, rtol=1e-13, maxit=500):
+        st = abi.SolveStats()
+        self.L.ora_poisson_solve(self.h, rtol, maxit, C.byref(st))
+        return st
+
+    def project(self, dt):
+        return self.L.ora_project(self.h, dt)
+
+    def div_max(self):
+        ijk = np.zeros(3, dtype=np.int32)
+        d = self.L.ora_div_max(self.h, _ptr(ijk))
+        return d, tuple(int(v) for v in ijk)
+
+    def store_old(self):
+        return self.L.ora_store_old(self.h)
+
+    def rk_average(self, a, b):
+        return self.L.ora_rk_average(self.h, a, b)
+
+    def step(self, dt):
+        st = abi.StepStats()
+        self.L.ora_step(self.h, dt, C.byref(st))
+        return st
+
+    def gmg_levels(self):
+        dims = np.zeros(48, dtype=np.int32)
+        n = self.L.ora_gmg_levels(self.h, _ptr(dims), 16)
+        return [tuple(int(v) for v in dims[3 * l:3 * l + 3]) for l in range(n)]
+
+
+def eno_line(order, D0, pos, vel):
+    D0 = np.ascontiguousarray(D0, dtype=np.float64)
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    vel = np.ascontiguousarray(vel, dtype=np.float64)
+    out = np.zeros_like(D0)
+    lib().ora_eno_line(order, D0.size, _ptr(D0), _ptr(pos), _ptr(vel), _ptr(out))
+    return out
+
+
+def gauss_jordan(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    n = a.shape[0]
+    out = np.zeros_like(a)
+    lib().ora_gauss_jordan(n, _ptr(a), _ptr(out))
+    return out

@@ -1,0 +1,69 @@
+/*
+ * tb_oracle.h -- CPU oracle for the TURBINS per-timestep hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain-C FP64 restatement of the
+ * reference algorithms (mmnasr/TURBINS src/ C files, cited per function in the
+ * sources).  Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it, and only as the checker or the timed CPU
+ * baseline -- never as part of the product path.
+ *
+ * Layout: every 3-D field is (NX)*(NY)*(NZ) doubles with NX = nx+1 etc.,
+ * x fastest (PETSc DA natural order, Grid.c:59-62).
+ */
+#ifndef TB_ORACLE_H
+#define TB_ORACLE_H
+
+#include <stdint.h>
+#include "../include/turbins_gpu.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ora_ctx ora_ctx;
+
+ora_ctx *ora_create(const tb_config *cfg);
+void ora_destroy(ora_ctx *o);
+
+/* grid coordinates / metrics (Grid.c:470-708).  which: 0 xc 1 yc 2 zc
+ * 3 xu 4 yu 5 zu 6 xv 7 yv 8 zv 9 xw 10 yw 11 zw 12 dx_c 13 dy_c 14 dz_c
+ * 15 mxc 16 myc 17 mzc 18 mxu 19 myv 20 mzw */
+int ora_get_grid_array(ora_ctx *o, int which, double *out);
+
+int ora_setup_geometry(ora_ctx *o, const double *h_xz);
+int ora_reinit_iterations(ora_ctx *o);
+int ora_get_flags(ora_ctx *o, int q, uint8_t *out);
+int ora_ib_count(ora_ctx *o, int q);
+int ora_get_ib(ora_ctx *o, int q, tb_ib_node *out, int cap);
+
+int ora_set_field(ora_ctx *o, int f, const double *in);
+int ora_get_field(ora_ctx *o, int f, double *out);
+
+/* ENO_compute_convective_terms on one line (ENO.c:103-245) */
+void ora_eno_line(int order, int n, const double *D0, const double *pos,
+                  const double *vel, double *vdqdx);
+/* 8x8 Gauss-Jordan inverse (MyMath.c:117-209), row-major in/out */
+void ora_gauss_jordan(int n, const double *a, double *a_inv);
+
+int ora_eno_conv(ora_ctx *o, int q);
+int ora_set_rhs(ora_ctx *o, int q, double dt);
+int ora_helmholtz_apply(ora_ctx *o, int q, double dt, const double *x, double *y);
+int ora_helmholtz_solve(ora_ctx *o, int q, double dt, double rtol, int maxit,
+                        tb_solve_stats *st);
+int ora_poisson_rhs(ora_ctx *o, double dt);
+int ora_poisson_apply(ora_ctx *o, const double *x, double *y);
+int ora_poisson_solve(ora_ctx *o, double rtol, int maxit, tb_solve_stats *st);
+int ora_project(ora_ctx *o, double dt);
+double ora_div_max(ora_ctx *o, int32_t *ijk);
+int ora_store_old(ora_ctx *o);
+int ora_rk_average(ora_ctx *o, double old_frac, double new_frac);
+int ora_step(ora_ctx *o, double dt, tb_step_stats *st);
+
+/* GMG hierarchy description (for tests): returns number of levels and fills
+ * dims[3*l..3*l+2] */
+int ora_gmg_levels(ora_ctx *o, int32_t *dims, int cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,198 @@
+"""turbins_amd -- MI355X (gfx950) implementation of the TURBINS per-timestep
+hot path behind the C-ABI declared in include/turbins_gpu.h.
+
+`Solver` is a thin ctypes binding over libturbins_gpu.so.  There is no
+fallback: if the library is missing or cannot be loaded, constructing a
+Solver raises.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from . import abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libturbins_gpu.so")
+_LIB = None
+
+
+def build(force=False):
+    """Compile the HIP library for gfx950 in place (make -C csrc)."""
+    args = ["make", "-s", "-C", os.path.join(_HERE, "csrc"), "-j8"]
+    if force:
+        subprocess.run(["make", "-s", "-C", os.path.join(_HERE, "csrc"), "clean"], check=True)
+    subprocess.run(args, check=True)
+    return LIB_PATH
+
+
+def lib():
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: run turbins_amd.build() first")
+    L = C.CDLL(LIB_PATH)
+    P = C.c_void_p
+    L.tb_create.restype = P
+    L.tb_create.argtypes = [C.POINTER(abi.Config)]
+    L.tb_destroy.argtypes = [P]
+    L.tb_last_error.restype = C.c_char_p
+    L.tb_last_error.argtypes = [P]
+    L.tb_device_bytes.argtypes = [P, C.POINTER(C.c_int64)]
+    L.tb_setup_geometry.argtypes = [P, P]
+    L.tb_download_flags.argtypes = [P, C.c_int, P]
+    L.tb_ib_count.argtypes = [P, C.c_int]
+    L.tb_download_ib_table.argtypes = [P, C.c_int, P, C.c_int]
+    L.tb_reinit_iterations.argtypes = [P]
+    L.tb_upload_field.argtypes = [P, C.c_int, P]
+    L.tb_download_field.argtypes = [P, C.c_int, P]
+    L.tb_eno_conv.argtypes = [P, C.c_int]
+    L.tb_set_rhs.argtypes = [P, C.c_int, C.c_double]
+    L.tb_helmholtz_apply.argtypes = [P, C.c_int, C.c_double, C.c_int, C.c_int]
+    L.tb_helmholtz_solve.argtypes = [P, C.c_int, C.c_double, C.POINTER(abi.SolveStats)]
+    L.tb_poisson_rhs.argtypes = [P, C.c_double]
+    L.tb_poisson_apply.argtypes = [P, C.c_int, C.c_int]
+    L.tb_poisson_solve.argtypes = [P, C.POINTER(abi.SolveStats)]
+    L.tb_project.argtypes = [P, C.c_double]
+    L.tb_div_max.argtypes = [P, C.POINTER(C.c_double), P]
+    L.tb_store_old.argtypes = [P]
+    L.tb_rk_average.argtypes = [P, C.c_double, C.c_double]
+    L.tb_step.argtypes = [P, C.c_double, C.POINTER(abi.StepStats)]
+    _LIB = L
+    return L
+
+
+def exported_symbols():
+    """Entry points declared in include/turbins_gpu.h."""
+    hdr = os.path.join(os.path.dirname(_HERE), "include", "turbins_gpu.h")
+    import re
+    names = []
+    for line in open(hdr):
+        m = re.match(r"^(?:int|void|const char \*|tb_ctx \*)\s*\*?(tb_\w+)\(", line.strip())
+        if m:
+            names.append(m.group(1))
+    return names
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class TBError(RuntimeError):
+    pass
+
+
+class Solver:
+    """One device context (one HIP stream, all fields resident in HBM)."""
+
+    def __init__(self, cfg):
+        self.cfg = cfg
+        self.L = lib()
+        self.h = self.L.tb_create(C.byref(cfg))
+        if not self.h:
+            raise TBError("tb_create failed (bad config or no usable GPU)")
+        self.shape = (cfg.nz + 1, cfg.ny + 1, cfg.nx + 1)
+        self.NT = int(np.prod(self.shape))
+
+    def _ok(self, rc, what):
+        if rc < 0:
+            raise TBError(f"{what}: {self.L.tb_last_error(self.h).decode()}")
+        return rc
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.tb_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def device_bytes(self):
+        v = C.c_int64()
+        self._ok(self.L.tb_device_bytes(self.h, C.byref(v)), "tb_device_bytes")
+        return v.value
+
+    def setup_geometry(self, h_xz):
+        if h_xz is None:
+            return self._ok(self.L.tb_setup_geometry(self.h, None), "tb_setup_geometry")
+        h = np.ascontiguousarray(h_xz, dtype=np.float64)
+        assert h.size == (self.cfg.nz + 1) * (self.cfg.nx + 1)
+        return self._ok(self.L.tb_setup_geometry(self.h, _ptr(h)), "tb_setup_geometry")
+
+    def reinit_iterations(self):
+        return self.L.tb_reinit_iterations(self.h)
+
+    def flags(self, q):
+        out = np.zeros(self.NT, dtype=np.uint8)
+        self._ok(self.L.tb_download_flags(self.h, q, _ptr(out)), "tb_download_flags")
+        return out
+
+    def ib_table(self, q):
+        n = self.L.tb_ib_count(self.h, q)
+        arr = (abi.IBNode * max(n, 1))()
+        self._ok(self.L.tb_download_ib_table(self.h, q, C.cast(arr, C.c_void_p), n), "tb_download_ib_table")
+        return list(arr)[:n]
+
+    def upload(self, name, a):
+        a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+        assert a.size == self.NT
+        self._ok(self.L.tb_upload_field(self.h, abi.F[name], _ptr(a)), "tb_upload_field")
+
+    def download(self, name):
+        out = np.zeros(self.NT)
+        self._ok(self.L.tb_download_field(self.h, abi.F[name], _ptr(out)), "tb_download_field")
+        return out
+
+    def eno_conv(self, q):
+        return self._ok(self.L.tb_eno_conv(self.h, q), "tb_eno_conv")
+
+    def set_rhs(self, q, dt):
+        return self._ok(self.L.tb_set_rhs(self.h, q, dt), "tb_set_rhs")
+
+    def helmholtz_apply(self, q, dt, x):
+        self.upload("SCRATCH0", x)
+        self._ok(self.L.tb_helmholtz_apply(self.h, q, dt, abi.F["SCRATCH0"], abi.F["SCRATCH1"]), "tb_helmholtz_apply")
+        return self.download("SCRATCH1")
+
+    def helmholtz_solve(self, q, dt):
+        st = abi.SolveStats()
+        self._ok(self.L.tb_helmholtz_solve(self.h, q, dt, C.byref(st)), "tb_helmholtz_solve")
+        return st
+
+    def poisson_rhs(self, dt):
+        return self._ok(self.L.tb_poisson_rhs(self.h, dt), "tb_poisson_rhs")
+
+    def poisson_apply(self, x):
+        self.upload("SCRATCH0", x)
+        self._ok(self.L.tb_poisson_apply(self.h, abi.F["SCRATCH0"], abi.F["SCRATCH1"]), "tb_poisson_apply")
+        return self.download("SCRATCH1")
+
+    def poisson_solve(self):
+        st = abi.SolveStats()
+        self._ok(self.L.tb_poisson_solve(self.h, C.byref(st)), "tb_poisson_solve")
+        return st
+
+    def project(self, dt):
+        return self._ok(self.L.tb_project(self.h, dt), "tb_project")
+
+    def div_max(self):
+        v = C.c_double()
+        ijk = np.zeros(3, dtype=np.int32)
+        self._ok(self.L.tb_div_max(self.h, C.byref(v), _ptr(ijk)), "tb_div_max")
+        return v.value, tuple(int(a) for a in ijk)
+
+    def store_old(self):
+        return self._ok(self.L.tb_store_old(self.h), "tb_store_old")
+
+    def rk_average(self, a, b):
+        return self._ok(self.L.tb_rk_average(self.h, a, b), "tb_rk_average")
+
+    def step(self, dt):
+        st = abi.StepStats()
+        self._ok(self.L.tb_step(self.h, dt, C.byref(st)), "tb_step")
+        return st

@@ -1,0 +1,91 @@
+"""ctypes mirror of include/turbins_gpu.h (structs, constants, field ids)."""
+import ctypes as C
+
+Q_U, Q_V, Q_W, Q_C = 0, 1, 2, 3
+OUTSIDE, SOLID, FLUID, IMMERSED, BOUNDARY, BUFFER_SOLID = 1, 2, 3, 5, 7, 11
+SALINITY, PARTICLE = 1, 2
+POIS_FP64_PCG, POIS_MIXED_IR, POIS_LEARNED_IR = 0, 1, 2
+
+FIELDS = [
+    "U", "V", "W", "C", "DPDX", "DPDY", "DPDZ", "PHI",
+    "CONV_U", "CONV_V", "CONV_W", "CONV_C",
+    "RHS_U", "RHS_V", "RHS_W", "RHS_C",
+    "PRHS", "DIV", "SDF_U", "SDF_V", "SDF_W", "SDF_C",
+    "SCRATCH0", "SCRATCH1",
+]
+F = {name: i for i, name in enumerate(FIELDS)}
+F_COUNT = len(FIELDS)
+
+
+class Config(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+        ("lx", C.c_double), ("ly", C.c_double), ("lz", C.c_double),
+        ("re", C.c_double), ("pe", C.c_double),
+        ("eno_order", C.c_int32),
+        ("wall_noslip", C.c_int32 * 6),
+        ("buoyancy", C.c_double),
+        ("conc_type", C.c_int32),
+        ("v_settle", C.c_double),
+        ("sedimentation", C.c_int32),
+        ("helm_rtol", C.c_double), ("helm_maxit", C.c_int32),
+        ("pois_rtol", C.c_double), ("pois_maxit", C.c_int32),
+        ("pois_mode", C.c_int32),
+        ("bottom_offset", C.c_int32),
+        ("mg_pre", C.c_int32), ("mg_post", C.c_int32),
+        ("reserved", C.c_int32 * 8),
+    ]
+
+
+class IBNode(C.Structure):
+    _fields_ = [
+        ("i", C.c_int32), ("j", C.c_int32), ("k", C.c_int32),
+        ("n_fluid", C.c_int32),
+        ("fi", C.c_int32 * 8), ("fj", C.c_int32 * 8), ("fk", C.c_int32 * 8),
+        ("coef", C.c_double * 8),
+        ("boundary_coef", C.c_double),
+        ("n", C.c_double * 3), ("im", C.c_double * 3),
+        ("control", C.c_double * 3), ("image", C.c_double * 3),
+    ]
+
+
+class SolveStats(C.Structure):
+    _fields_ = [
+        ("iterations", C.c_int32), ("converged", C.c_int32),
+        ("fallback", C.c_int32), ("vcycles", C.c_int32),
+        ("res0", C.c_double), ("res", C.c_double), ("ms", C.c_double),
+    ]
+
+
+class StepStats(C.Structure):
+    _fields_ = [
+        ("helm", (SolveStats * 4) * 2),
+        ("pois", SolveStats * 2),
+        ("div_max", C.c_double),
+        ("ms", C.c_double),
+    ]
+
+
+IB_DTYPE_FIELDS = [f[0] for f in IBNode._fields_]
+
+
+def make_config(**kw):
+    """Build a Config from keyword arguments (defaults follow the reference's
+    hard-coded internals, Input.c:405-509, except where the workload says
+    otherwise)."""
+    cfg = Config()
+    defaults = dict(
+        nx=64, ny=16, nz=16, lx=8.0, ly=1.0, lz=1.0, re=1000.0, pe=1000.0,
+        eno_order=2, wall_noslip=(1, 1, 1, 1, 1, 1), buoyancy=0.0,
+        conc_type=SALINITY, v_settle=0.0, sedimentation=0,
+        helm_rtol=1e-12, helm_maxit=200, pois_rtol=1e-10, pois_maxit=200,
+        pois_mode=POIS_FP64_PCG, bottom_offset=1, mg_pre=2, mg_post=2,
+    )
+    defaults.update(kw)
+    for k, v in defaults.items():
+        if k == "wall_noslip":
+            for a in range(6):
+                cfg.wall_noslip[a] = int(v[a])
+        else:
+            setattr(cfg, k, v)
+    return cfg

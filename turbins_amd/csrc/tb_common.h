@@ -1,0 +1,214 @@
+// tb_common.h -- shared definitions of the gfx950 library: context, field
+// layout, launch geometry and small device helpers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/turbins_gpu.h"
+
+// ---------------------------------------------------------------------------
+// Field layout in HBM: the reference's natural order (x fastest, then y, z)
+// over (nx+1)(ny+1)(nz+1) points, with each x row padded to a pitch that is
+// a multiple of 16 doubles so every row starts on a 128-byte boundary.
+struct Dims {
+    int NX, NY, NZ;     // points per axis (cells + the half cell)
+    int px;             // row pitch in elements
+    long plane;         // px * NY
+    long ntot;          // plane * NZ
+};
+
+__host__ __device__ inline long gidx(const Dims &d, int i, int j, int k) {
+    return ((long)k * d.NY + j) * d.px + i;
+}
+
+// 1-D coordinate and metric arrays (Grid.c:470-708), device pointers.
+struct GridDev {
+    const double *xc, *yc, *zc, *xu, *yu, *zu, *xv, *yv, *zv, *xw, *yw, *zw;
+    const double *dxc, *dyc, *dzc;
+    const double *mxc, *myc, *mzc, *mxu, *myv, *mzw;
+};
+
+// ---------------------------------------------------------------------------
+// Launch geometry for point-wise sweeps: tiles of 64 x 4 points in one k
+// plane.  Workgroups are handed to the 8 XCDs round robin (block b runs on
+// XCD b % 8), so the linear block id is remapped to give each XCD one
+// contiguous slab of tiles: neighbouring rows and planes of a stencil then
+// hit the same XCD's L2.
+constexpr int TILE_X = 64;
+constexpr int TILE_Y = 4;
+constexpr int NXCD = 8;
+
+struct Tiles {
+    int nx, ny, nz;     // extent of the swept box (points)
+    int tx, ty;         // tiles per axis
+    long n;             // tiles in total
+    long per_xcd;       // ceil(n / 8)
+};
+
+inline Tiles make_tiles(int nx, int ny, int nz) {
+    Tiles t;
+    t.nx = nx; t.ny = ny; t.nz = nz;
+    t.tx = (nx + TILE_X - 1) / TILE_X;
+    t.ty = (ny + TILE_Y - 1) / TILE_Y;
+    t.n = (long)t.tx * t.ty * nz;
+    t.per_xcd = (t.n + NXCD - 1) / NXCD;
+    return t;
+}
+
+inline dim3 tiles_grid(const Tiles &t) { return dim3((unsigned)(t.per_xcd * NXCD)); }
+inline dim3 tiles_block() { return dim3(TILE_X, TILE_Y); }
+
+// Returns false for threads outside the box (or remapped past the last tile).
+__device__ inline bool tile_point(const Tiles &t, int &i, int &j, int &k) {
+    long b = blockIdx.x;
+    long lt = (b % NXCD) * t.per_xcd + b / NXCD;
+    if (lt >= t.n) return false;
+    int tx = (int)(lt % t.tx);
+    long r = lt / t.tx;
+    int ty = (int)(r % t.ty);
+    k = (int)(r / t.ty);
+    i = tx * TILE_X + threadIdx.x;
+    j = ty * TILE_Y + threadIdx.y;
+    return i < t.nx && j < t.ny;
+}
+
+// ---------------------------------------------------------------------------
+// Reductions: each block writes partial sums; a one-block kernel adds the
+// partials in a fixed order, so results do not depend on scheduling.
+constexpr int RED_MAX_BLOCKS = 1 << 18;   // per slot; one partial per workgroup
+constexpr int RED_SLOTS = 4;        // up to 4 simultaneous sums per launch
+
+template <typename T>
+__device__ inline T warp_sum(T v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    return v;
+}
+
+// Sums NV values over the block; thread 0 of the block gets the totals.
+template <int NV>
+__device__ inline void block_sum(double (&v)[NV], double *sh /* [NV*16] */) {
+    int lane = (threadIdx.y * blockDim.x + threadIdx.x) & 63;
+    int wid = (threadIdx.y * blockDim.x + threadIdx.x) >> 6;
+    int nw = (blockDim.x * blockDim.y + 63) >> 6;
+#pragma unroll
+    for (int a = 0; a < NV; a++) v[a] = warp_sum(v[a]);
+    if (lane == 0)
+#pragma unroll
+        for (int a = 0; a < NV; a++) sh[a * 16 + wid] = v[a];
+    __syncthreads();
+    if (wid == 0) {
+#pragma unroll
+        for (int a = 0; a < NV; a++) {
+            double s = (lane < nw) ? sh[a * 16 + lane] : 0.0;
+            v[a] = warp_sum(s);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// GMG level description (cell-centred arrays)
+struct Level {
+    int nx, ny, nz;       // cells
+    int px;               // row pitch (elements)
+    long plane, ntot;
+    double hx, hy, hz;
+    int cx, cy, cz;       // halved towards the next level
+    double *x64, *b64, *r64;
+    float *x32, *b32, *r32;
+};
+
+// ---------------------------------------------------------------------------
+struct NetState;    // learned initial guess (tb_net.hip)
+
+struct tb_ctx {
+    tb_config cfg;
+    std::string err;
+    int ncx, ncy, ncz;
+    Dims d;
+    hipStream_t s = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int64_t bytes = 0;
+
+    // host copies of the 1-D grid arrays, and their device copies
+    std::vector<double> hgrid[21];
+    double *dgrid = nullptr;
+    GridDev g;
+    double dEta, dXi, dPhi, dx, dy, dz;
+
+    double *f[TB_F_COUNT] = {};
+    double *old[4] = {};
+    uint8_t *st[4] = {};
+
+    // immersed boundary: node tables and the scaled matrix rows
+    int nib[4] = {0, 0, 0, 0};
+    tb_ib_node *ib[4] = {};
+    long *ib_row[4] = {};
+    int *ib_cnt[4] = {};
+    long *ib_col[4] = {};        // 9 per row, sorted
+    double *ib_val[4] = {};      // 9 per row
+    int reinit_its = 0;
+    bool has_geometry = false;
+
+    // Krylov work vectors (full padded layout)
+    double *kv[8] = {};
+    double *red = nullptr;       // partial sums [RED_SLOTS][RED_MAX_BLOCKS]
+    double *scal = nullptr;      // device scalars
+    double *hscal = nullptr;     // pinned host mirror of scal
+
+    // multigrid
+    int nlev = 0;
+    Level lev[16];
+    double *coarse_inv64 = nullptr;
+    float *coarse_inv32 = nullptr;
+    int coarse_n = 0;
+    bool mg_built = false;
+
+    // Poisson history for the learned guess
+    double *phi_hist[2] = {};
+    int n_hist = 0;
+    NetState *net = nullptr;
+
+    // timing of named regions
+    double t_accum[32] = {};
+    int t_count[32] = {};
+};
+
+// device scalar slots (ctx->scal)
+enum {
+    S_RHO = 0, S_ALPHA, S_OMEGA, S_BETA, S_RZ, S_PQ, S_RR, S_TS, S_TT, S_RHV,
+    S_BB, S_MEAN, S_CORNER, S_TMP0, S_TMP1, S_TMP2, S_COUNT = 32
+};
+
+// error helpers -------------------------------------------------------------
+#define TB_CHECK(ctx, call)                                                      \
+    do {                                                                         \
+        hipError_t e_ = (call);                                                  \
+        if (e_ != hipSuccess) {                                                  \
+            (ctx)->err = std::string(#call) + ": " + hipGetErrorString(e_);      \
+            return -1;                                                           \
+        }                                                                        \
+    } while (0)
+
+#define TB_LAUNCH_CHECK(ctx)                                                     \
+    do {                                                                         \
+        hipError_t e_ = hipGetLastError();                                       \
+        if (e_ != hipSuccess) {                                                  \
+            (ctx)->err = std::string("launch: ") + hipGetErrorString(e_);       \
+            return -1;                                                           \
+        }                                                                        \
+    } while (0)
+
+// internal entry points shared between translation units
+int tb_geometry_run(tb_ctx *c, const double *h_xz);
+int tb_reduce_finish(tb_ctx *c, int nblocks, int nslots, int slot0);
+int tb_dot(tb_ctx *c, const double *a, const double *b, int slot);
+int tb_mg_build(tb_ctx *c);
+void tb_mg_free(tb_ctx *c);
+int tb_vcycle64(tb_ctx *c, const double *r_pad, double *z_pad);
+int tb_vcycle32(tb_ctx *c);
+int tb_net_predict(tb_ctx *c, double *phi_out);
+void tb_net_free(tb_ctx *c);
+double tb_elapsed_ms(tb_ctx *c);

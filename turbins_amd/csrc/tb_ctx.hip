@@ -1,0 +1,282 @@
+// tb_ctx.hip -- context lifecycle, grid arrays, field transfers, reductions
+// and the vector sweeps of the RK step (store_old / rk_average).
+#include <cmath>
+#include <cstring>
+
+#include "tb_common.h"
+
+// ---------------------------------------------------------------------------
+// Grid_generate_mesh_coordinates (Grid.c:470-591, coordinates accumulated by
+// repeated addition) and Grid_compute_metric_coefficients (Grid.c:595-708).
+static void build_grid(tb_ctx *c) {
+    int NX = c->d.NX, NY = c->d.NY, NZ = c->d.NZ;
+    int len[21] = {NX, NY, NZ, NX, NY, NZ, NX, NY, NZ, NX, NY, NZ, NX, NY, NZ, NX, NY, NZ, NX, NY, NZ};
+    for (int a = 0; a < 21; a++) c->hgrid[a].assign(len[a], 0.0);
+    auto &xc = c->hgrid[0], &yc = c->hgrid[1], &zc = c->hgrid[2];
+    auto &xu = c->hgrid[3], &yu = c->hgrid[4], &zu = c->hgrid[5];
+    auto &xv = c->hgrid[6], &yv = c->hgrid[7], &zv = c->hgrid[8];
+    auto &xw = c->hgrid[9], &yw = c->hgrid[10], &zw = c->hgrid[11];
+    auto &dxc = c->hgrid[12], &dyc = c->hgrid[13], &dzc = c->hgrid[14];
+    auto &mxc = c->hgrid[15], &myc = c->hgrid[16], &mzc = c->hgrid[17];
+    auto &mxu = c->hgrid[18], &myv = c->hgrid[19], &mzw = c->hgrid[20];
+    c->dEta = 1.0 / (double)c->ncx;
+    c->dXi = 1.0 / (double)c->ncy;
+    c->dPhi = 1.0 / (double)c->ncz;
+    c->dx = c->cfg.lx * c->dEta;
+    c->dy = c->cfg.ly * c->dXi;
+    c->dz = c->cfg.lz * c->dPhi;
+    const double dx = c->dx, dy = c->dy, dz = c->dz;
+    auto fill = [](std::vector<double> &a, double start, double step) {
+        double g = start;
+        for (size_t i = 0; i < a.size(); i++) { a[i] = g; g += step; }
+    };
+    fill(xu, 0.0, dx); fill(yu, 0.5 * dy, dy); fill(zu, 0.5 * dz, dz);
+    fill(xv, 0.5 * dx, dx); fill(yv, 0.0, dy); fill(zv, 0.5 * dz, dz);
+    fill(xw, 0.5 * dx, dx); fill(yw, 0.5 * dy, dy); fill(zw, 0.0, dz);
+    fill(xc, 0.5 * dx, dx); fill(yc, 0.5 * dy, dy); fill(zc, 0.5 * dz, dz);
+    const double dEta = c->dEta, dXi = c->dXi, dPhi = c->dPhi;
+    for (int i = 1; i < NX - 1; i++) { mxu[i] = dEta / (xc[i] - xc[i - 1]); dxc[i] = xu[i + 1] - xu[i]; }
+    dxc[0] = xu[1] - xu[0];
+    dxc[NX - 1] = xu[NX - 1] - xu[NX - 2];
+    mxu[0] = 0.5 * dEta / (xc[0] - xu[0]);
+    mxu[NX - 1] = 0.5 * dEta / (xu[NX - 1] - xc[NX - 2]);
+    for (int j = 1; j < NY - 1; j++) { myv[j] = dXi / (yc[j] - yc[j - 1]); dyc[j] = yv[j + 1] - yv[j]; }
+    dyc[0] = yv[1] - yv[0];
+    dyc[NY - 1] = yv[NY - 1] - yv[NY - 2];
+    myv[0] = 0.5 * dXi / (yc[0] - yv[0]);
+    myv[NY - 1] = 0.5 * dXi / (yv[NY - 1] - yc[NY - 2]);
+    for (int k = 1; k < NZ - 1; k++) { mzw[k] = dPhi / (zc[k] - zc[k - 1]); dzc[k] = zw[k + 1] - zw[k]; }
+    dzc[0] = zw[1] - zw[0];
+    dzc[NZ - 1] = zw[NZ - 1] - zw[NZ - 2];
+    mzw[0] = 0.5 * dPhi / (zc[0] - zw[0]);
+    mzw[NZ - 1] = 0.5 * dPhi / (zw[NZ - 1] - zc[NZ - 2]);
+    for (int i = 0; i < NX - 1; i++) mxc[i] = dEta / (xu[i + 1] - xu[i]);
+    mxc[NX - 1] = dEta / (xc[NX - 1] - xc[NX - 2]);
+    for (int j = 0; j < NY - 1; j++) myc[j] = dXi / (yv[j + 1] - yv[j]);
+    myc[NY - 1] = dXi / (yc[NY - 1] - yc[NY - 2]);
+    for (int k = 0; k < NZ - 1; k++) mzc[k] = dPhi / (zc[k + 1] - zc[k]);   // zc, as Grid.c:697
+    mzc[NZ - 1] = dPhi / (zc[NZ - 1] - zc[NZ - 2]);
+}
+
+static int dev_alloc(tb_ctx *c, void **p, size_t bytes) {
+    TB_CHECK(c, hipMalloc(p, bytes));
+    TB_CHECK(c, hipMemsetAsync(*p, 0, bytes, c->s));
+    c->bytes += (int64_t)bytes;
+    return 0;
+}
+
+double tb_elapsed_ms(tb_ctx *c) {
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    return (double)ms;
+}
+
+extern "C" tb_ctx *tb_create(const tb_config *cfg) {
+    if (!cfg || cfg->nx < 4 || cfg->ny < 4 || cfg->nz < 4 || cfg->eno_order < 1 || cfg->eno_order > 3)
+        return nullptr;
+    tb_ctx *c = new tb_ctx();
+    c->cfg = *cfg;
+    if (c->cfg.helm_maxit <= 0) c->cfg.helm_maxit = 200;
+    if (c->cfg.pois_maxit <= 0) c->cfg.pois_maxit = 200;
+    if (c->cfg.mg_pre <= 0) c->cfg.mg_pre = 2;
+    if (c->cfg.mg_post <= 0) c->cfg.mg_post = 2;
+    c->ncx = cfg->nx; c->ncy = cfg->ny; c->ncz = cfg->nz;
+    c->d.NX = cfg->nx + 1; c->d.NY = cfg->ny + 1; c->d.NZ = cfg->nz + 1;
+    c->d.px = (c->d.NX + 15) / 16 * 16;
+    c->d.plane = (long)c->d.px * c->d.NY;
+    c->d.ntot = c->d.plane * c->d.NZ;
+    if (hipStreamCreate(&c->s) != hipSuccess || hipEventCreate(&c->ev0) != hipSuccess ||
+        hipEventCreate(&c->ev1) != hipSuccess) {
+        delete c;
+        return nullptr;
+    }
+    build_grid(c);
+    int maxn = c->d.NX > c->d.NY ? c->d.NX : c->d.NY;
+    if (c->d.NZ > maxn) maxn = c->d.NZ;
+    bool ok = dev_alloc(c, (void **)&c->dgrid, sizeof(double) * 21 * (size_t)maxn) == 0;
+    if (ok) {
+        const double **gp[21] = {&c->g.xc, &c->g.yc, &c->g.zc, &c->g.xu, &c->g.yu, &c->g.zu,
+                                 &c->g.xv, &c->g.yv, &c->g.zv, &c->g.xw, &c->g.yw, &c->g.zw,
+                                 &c->g.dxc, &c->g.dyc, &c->g.dzc, &c->g.mxc, &c->g.myc, &c->g.mzc,
+                                 &c->g.mxu, &c->g.myv, &c->g.mzw};
+        for (int a = 0; a < 21; a++) {
+            double *dst = c->dgrid + (size_t)a * maxn;
+            *gp[a] = dst;
+            ok = ok && hipMemcpyAsync(dst, c->hgrid[a].data(), sizeof(double) * c->hgrid[a].size(),
+                                      hipMemcpyHostToDevice, c->s) == hipSuccess;
+        }
+    }
+    size_t fb = sizeof(double) * (size_t)c->d.ntot;
+    for (int f = 0; ok && f < TB_F_COUNT; f++) ok = dev_alloc(c, (void **)&c->f[f], fb) == 0;
+    for (int q = 0; ok && q < 4; q++) ok = dev_alloc(c, (void **)&c->old[q], fb) == 0;
+    for (int q = 0; ok && q < 4; q++) ok = dev_alloc(c, (void **)&c->st[q], (size_t)c->d.ntot) == 0;
+    for (int a = 0; ok && a < 8; a++) ok = dev_alloc(c, (void **)&c->kv[a], fb) == 0;
+    for (int a = 0; ok && a < 2; a++) ok = dev_alloc(c, (void **)&c->phi_hist[a], fb) == 0;
+    ok = ok && dev_alloc(c, (void **)&c->red, sizeof(double) * RED_SLOTS * RED_MAX_BLOCKS) == 0;
+    ok = ok && dev_alloc(c, (void **)&c->scal, sizeof(double) * S_COUNT) == 0;
+    ok = ok && hipHostMalloc((void **)&c->hscal, sizeof(double) * S_COUNT, hipHostMallocDefault) == hipSuccess;
+    ok = ok && hipStreamSynchronize(c->s) == hipSuccess;
+    if (!ok) {
+        tb_destroy(c);
+        return nullptr;
+    }
+    if (tb_setup_geometry(c, nullptr) != 0) {
+        tb_destroy(c);
+        return nullptr;
+    }
+    return c;
+}
+
+static void free_ib(tb_ctx *c, int q) {
+    hipFree(c->ib[q]); hipFree(c->ib_row[q]); hipFree(c->ib_cnt[q]); hipFree(c->ib_col[q]); hipFree(c->ib_val[q]);
+    c->ib[q] = nullptr; c->ib_row[q] = nullptr; c->ib_cnt[q] = nullptr; c->ib_col[q] = nullptr; c->ib_val[q] = nullptr;
+    c->nib[q] = 0;
+}
+
+extern "C" void tb_destroy(tb_ctx *c) {
+    if (!c) return;
+    if (c->s) hipStreamSynchronize(c->s);
+    tb_net_free(c);
+    tb_mg_free(c);
+    hipFree(c->dgrid);
+    for (int f = 0; f < TB_F_COUNT; f++) hipFree(c->f[f]);
+    for (int q = 0; q < 4; q++) { hipFree(c->old[q]); hipFree(c->st[q]); free_ib(c, q); }
+    for (int a = 0; a < 8; a++) hipFree(c->kv[a]);
+    for (int a = 0; a < 2; a++) hipFree(c->phi_hist[a]);
+    hipFree(c->red);
+    hipFree(c->scal);
+    if (c->hscal) hipHostFree(c->hscal);
+    if (c->ev0) hipEventDestroy(c->ev0);
+    if (c->ev1) hipEventDestroy(c->ev1);
+    if (c->s) hipStreamDestroy(c->s);
+    delete c;
+}
+
+extern "C" const char *tb_last_error(tb_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+extern "C" int tb_device_bytes(tb_ctx *c, int64_t *bytes) {
+    if (!c || !bytes) return -1;
+    *bytes = c->bytes;
+    return 0;
+}
+
+void tb_free_ib(tb_ctx *c, int q) { free_ib(c, q); }
+
+// ---------------------------------------------------------------------------
+extern "C" int tb_upload_field(tb_ctx *c, int f, const double *host) {
+    if (!c || f < 0 || f >= TB_F_COUNT || !host) { if (c) c->err = "tb_upload_field: bad argument"; return -1; }
+    const Dims &d = c->d;
+    TB_CHECK(c, hipMemcpy2DAsync(c->f[f], sizeof(double) * d.px, host, sizeof(double) * d.NX,
+                                 sizeof(double) * d.NX, (size_t)d.NY * d.NZ, hipMemcpyHostToDevice, c->s));
+    TB_CHECK(c, hipStreamSynchronize(c->s));
+    if (f == TB_F_PHI) c->n_hist = 0;
+    return 0;
+}
+
+extern "C" int tb_download_field(tb_ctx *c, int f, double *host) {
+    if (!c || f < 0 || f >= TB_F_COUNT || !host) { if (c) c->err = "tb_download_field: bad argument"; return -1; }
+    const Dims &d = c->d;
+    TB_CHECK(c, hipMemcpy2DAsync(host, sizeof(double) * d.NX, c->f[f], sizeof(double) * d.px,
+                                 sizeof(double) * d.NX, (size_t)d.NY * d.NZ, hipMemcpyDeviceToHost, c->s));
+    TB_CHECK(c, hipStreamSynchronize(c->s));
+    return 0;
+}
+
+extern "C" int tb_download_flags(tb_ctx *c, int q, uint8_t *out) {
+    if (!c || q < 0 || q > 3 || !out) { if (c) c->err = "tb_download_flags: bad argument"; return -1; }
+    const Dims &d = c->d;
+    TB_CHECK(c, hipMemcpy2DAsync(out, d.NX, c->st[q], d.px, d.NX, (size_t)d.NY * d.NZ, hipMemcpyDeviceToHost, c->s));
+    TB_CHECK(c, hipStreamSynchronize(c->s));
+    return 0;
+}
+
+extern "C" int tb_ib_count(tb_ctx *c, int q) {
+    if (!c || q < 0 || q > 3) return -1;
+    return c->nib[q];
+}
+
+extern "C" int tb_download_ib_table(tb_ctx *c, int q, tb_ib_node *out, int capacity) {
+    if (!c || q < 0 || q > 3 || (!out && capacity > 0)) { if (c) c->err = "tb_download_ib_table: bad argument"; return -1; }
+    int n = capacity < c->nib[q] ? capacity : c->nib[q];
+    if (n > 0) {
+        TB_CHECK(c, hipMemcpyAsync(out, c->ib[q], sizeof(tb_ib_node) * (size_t)n, hipMemcpyDeviceToHost, c->s));
+        TB_CHECK(c, hipStreamSynchronize(c->s));
+    }
+    return n;
+}
+
+extern "C" int tb_reinit_iterations(tb_ctx *c) { return c ? c->reinit_its : -1; }
+
+// ---------------------------------------------------------------------------
+// reductions
+
+__global__ void k_reduce_finish(const double *__restrict__ part, double *__restrict__ out, int nblocks,
+                                int nslots, int slot0) {
+    __shared__ double sh[RED_SLOTS * 16];
+    double v[RED_SLOTS];
+#pragma unroll
+    for (int a = 0; a < RED_SLOTS; a++) {
+        double s = 0.0;
+        if (a < nslots)
+            for (int b = threadIdx.x; b < nblocks; b += blockDim.x) s += part[a * RED_MAX_BLOCKS + b];
+        v[a] = s;
+    }
+    block_sum<RED_SLOTS>(v, sh);
+    if (threadIdx.x == 0)
+        for (int a = 0; a < nslots; a++) out[slot0 + a] = v[a];
+}
+
+int tb_reduce_finish(tb_ctx *c, int nblocks, int nslots, int slot0) {
+    hipLaunchKernelGGL(k_reduce_finish, dim3(1), dim3(1024), 0, c->s, c->red, c->scal, nblocks, nslots, slot0);
+    TB_LAUNCH_CHECK(c);
+    return 0;
+}
+
+constexpr int DOT_BLOCKS = 1024;
+
+__global__ void k_dot(const double *__restrict__ a, const double *__restrict__ b, long n, double *part) {
+    __shared__ double sh[16];
+    double v[1] = {0.0};
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        v[0] += a[i] * b[i];
+    block_sum<1>(v, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = v[0];
+}
+
+int tb_dot(tb_ctx *c, const double *a, const double *b, int slot) {
+    hipLaunchKernelGGL(k_dot, dim3(DOT_BLOCKS), dim3(256), 0, c->s, a, b, c->d.ntot, c->red);
+    TB_LAUNCH_CHECK(c);
+    return tb_reduce_finish(c, DOT_BLOCKS, 1, slot);
+}
+
+// ---------------------------------------------------------------------------
+// store_old (VecCopy, Velocity.c:297-302 / Conc.c:997-1002) and the RK
+// average y = a*x + b*y (VecAXPBY, Velocity.c:2047-2055 / Conc.c:1007-1017).
+
+__global__ void k_axpby4(double *y0, double *y1, double *y2, double *y3, const double *x0, const double *x1,
+                         const double *x2, const double *x3, double a, double b, long n) {
+    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    y0[i] = a * x0[i] + b * y0[i];
+    y1[i] = a * x1[i] + b * y1[i];
+    y2[i] = a * x2[i] + b * y2[i];
+    y3[i] = a * x3[i] + b * y3[i];
+}
+
+extern "C" int tb_store_old(tb_ctx *c) {
+    if (!c) return -1;
+    for (int q = 0; q < 4; q++)
+        TB_CHECK(c, hipMemcpyAsync(c->old[q], c->f[TB_F_U + q], sizeof(double) * c->d.ntot,
+                                   hipMemcpyDeviceToDevice, c->s));
+    return 0;
+}
+
+extern "C" int tb_rk_average(tb_ctx *c, double old_frac, double new_frac) {
+    if (!c) return -1;
+    long n = c->d.ntot;
+    hipLaunchKernelGGL(k_axpby4, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->s,
+                       c->f[TB_F_U], c->f[TB_F_V], c->f[TB_F_W], c->f[TB_F_C],
+                       c->old[0], c->old[1], c->old[2], c->old[3], old_frac, new_frac, n);
+    TB_LAUNCH_CHECK(c);
+    return 0;
+}

@@ -1,0 +1,377 @@
+// tb_eno.hip -- ENO convective terms for u, v, w and c, and the
+// right-hand sides of the four Helmholtz systems.
+//
+// The reference evaluates ENO_compute_convective_terms (ENO.c:103-245) one
+// grid line at a time after gathering the line and its wall ghosts into 1-D
+// buffers.  Here every thread evaluates one node directly from the field in
+// HBM: it reads the +-order window along each axis, substitutes the wall
+// ghost values of Velocity_*_ENO_ghost_cells / Conc_ENO_ghost_cells where the
+// window leaves the box, and applies the same divided differences and
+// stencil choice.  The advecting velocities of the transposes
+// (Velocity_transpose_velocities, Velocity.c:3341-3663) and of the cell
+// centres (Velocity_cell_center, Velocity.c:1476-1602) are interpolated in
+// registers.  All arithmetic keeps the reference's order (no contraction),
+// so results are bit-identical to the line-by-line evaluation.
+#include <cmath>
+
+#include "tb_common.h"
+
+__device__ inline double interp2d(double q0, double x0, double q1, double x1, double xi) {
+    double m = (q1 - q0) / (x1 - x0);
+    if (fabs(x1 - x0) < 1.0e-8) return q0;
+    return m * (xi - x0) + q0;
+}
+
+// MyMath_interpolate_quantity, four points (MyMath.c:52-73)
+__device__ inline double interp4d(double q0, double q1, double q2, double q3, double X10, double X11, double X20,
+                                  double X22, double x1, double x2) {
+    double a_Dx = 1.0 / (X11 - X10);
+    double Wp0 = fabs((x1 - X10) * a_Dx);
+    double Wp1 = 1.0 - Wp0;
+    double q_top = Wp0 * q2 + Wp1 * q3;
+    double q_bottom = Wp0 * q0 + Wp1 * q1;
+    double a_Dy = 1.0 / (X22 - X20);
+    Wp0 = fabs((x2 - X20) * a_Dy);
+    double Wp2 = 1.0 - Wp0;
+    return Wp0 * q_bottom + Wp2 * q_top;
+}
+
+// One line direction of one quantity: interior span and wall ghost rules.
+struct LineSpec {
+    int lo, hi;            // interior (non-BOUNDARY) range along the axis
+    int kind;              // 0: normal component on the wall; 1: mirrored line
+    int ux_quirk;          // u along x: left mirrors take -D0[order] (Velocity.c:2191)
+    int hi_mode;           // 1: top value scaled by hi_scale (particle no-flux, Conc.c:1164-1179)
+    double wall_lo, wall_hi;
+    double sgn_lo, sgn_hi; // kind 1: -1 negate (no-slip), +1 copy (free-slip / Neumann)
+    double hi_scale;
+    const double *pos;     // coordinates of the quantity's nodes along the axis
+    long stride;           // element stride of the axis in the padded layout
+};
+
+// Window of 2R+1 values and positions around axis position m0 (buffer
+// index R + m0 - lo of the reference's line).
+template <int R>
+__device__ inline void load_window(const LineSpec &L, const double *__restrict__ f, long base, int m0,
+                                   double (&w)[7], double (&p)[7]) {
+#pragma unroll
+    for (int o = -R; o <= R; o++) {
+        int m = m0 + o;
+        double wv, pv;
+        if (m >= L.lo && m <= L.hi) {
+            wv = f[base + (long)o * L.stride];
+            pv = L.pos[m];
+        } else if (m < L.lo) {
+            int ex = R + m - L.lo;                  // ghost_index
+            double first = f[base + (long)(L.lo - m0) * L.stride];
+            if (L.kind == 0) {
+                if (ex == R - 1) { pv = L.wall_lo; wv = 0.0; }
+                else { pv = 2.0 * L.wall_lo - L.pos[L.lo + R - ex - 2]; wv = L.ux_quirk ? -first : 0.0; }
+            } else {
+                pv = 2.0 * L.wall_lo - L.pos[L.lo + R - ex - 1];
+                wv = (L.sgn_lo < 0.0) ? -first : +first;
+            }
+        } else {
+            int n = L.hi - L.lo + 1;
+            int b = R + m - L.lo;
+            int ex = n + 2 * R - 1 - b;
+            double last = f[base + (long)(L.hi - m0) * L.stride];
+            if (L.kind == 0) {
+                if (ex == R - 1) { pv = L.wall_hi; wv = 0.0; }
+                else { pv = 2.0 * L.wall_hi - L.pos[L.hi + ex + 2 - R]; wv = 0.0; }
+            } else {
+                pv = 2.0 * L.wall_hi - L.pos[L.hi + 1 + ex - R];
+                if (L.hi_mode) wv = L.hi_scale * last;
+                else wv = (L.sgn_hi < 0.0) ? -last : +last;
+            }
+        }
+        w[3 + o] = wv;
+        p[3 + o] = pv;
+    }
+}
+
+// ENO_compute_convective_terms for the node at window centre 3.
+template <int R>
+__device__ inline double eno_eval(const double (&w)[7], const double (&p)[7], double vel) {
+    // first divided differences D1[m] = (D0[m+1]-D0[m])/(x[m+1]-x[m]), m = -R..R-1
+    double D1[6];
+#pragma unroll
+    for (int m = -R; m < R; m++) D1[m + 3] = (w[3 + m + 1] - w[3 + m]) / (p[3 + m + 1] - p[3 + m]);
+    bool left = vel > 0.0;                     // k = i-1 if V > 0 else i
+    double dQ = left ? D1[2] : D1[3];
+    if (R > 1) {
+        // D2[m] = (D1[m]-D1[m-1])/(x[m+1]-x[m-1]) for m = -R+1..R-1
+        double D2[5];
+#pragma unroll
+        for (int m = -R + 1; m <= R - 1; m++)
+            D2[m + 2] = (D1[m + 3] - D1[m + 2]) / (p[3 + m + 1] - p[3 + m - 1]);
+        // k = -1 or 0
+        double d2k = left ? D2[1] : D2[2];
+        double d2k1 = left ? D2[2] : D2[3];
+        double c;
+        int kstar;
+        int k = left ? -1 : 0;
+        if (fabs(d2k) <= fabs(d2k1)) { c = d2k; kstar = k - 1; }
+        else { c = d2k1; kstar = k; }
+        double x_i = p[3];
+        double x_k = left ? p[2] : p[3];
+        double x_k1 = left ? p[3] : p[4];
+        dQ += c * ((x_i - x_k) + (x_i - x_k1));
+        if (R > 2) {
+            // D3[m] = (D2[m+1]-D2[m])/(x[m+2]-x[m-1]) for m = -2..1
+            double D3[4];
+#pragma unroll
+            for (int m = -2; m <= 1; m++)
+                D3[m + 2] = (D2[m + 3] - D2[m + 2]) / (p[3 + m + 2] - p[3 + m - 1]);
+            // kstar in {-2,-1,0}
+            double d3a = (kstar == -2) ? D3[0] : (kstar == -1) ? D3[1] : D3[2];
+            double d3b = (kstar == -2) ? D3[1] : (kstar == -1) ? D3[2] : D3[3];
+            double cstar = (fabs(d3a) <= fabs(d3b)) ? d3a : d3b;
+            double x_ks = (kstar == -2) ? p[1] : (kstar == -1) ? p[2] : p[3];
+            double x_ks1 = (kstar == -2) ? p[2] : (kstar == -1) ? p[3] : p[4];
+            double x_ks2 = (kstar == -2) ? p[3] : (kstar == -1) ? p[4] : p[5];
+            dQ += cstar * ((x_i - x_ks) * (x_i - x_ks1) + (x_i - x_ks1) * (x_i - x_ks2) + (x_i - x_ks) * (x_i - x_ks2));
+        }
+    }
+    return vel * dQ;
+}
+
+template <int R>
+__device__ inline double eno_line_point(const LineSpec &L, const double *__restrict__ f, long base, int m0,
+                                        double vel) {
+    double w[7], p[7];
+    load_window<R>(L, f, base, m0, w, p);
+    return eno_eval<R>(w, p, vel);
+}
+
+// Per-quantity line specs for the three axes.
+struct QLines {
+    LineSpec ax[3];
+};
+
+struct MomArgs {
+    Dims d;
+    GridDev g;
+    const double *u, *v, *w;
+    const uint8_t *su, *sv, *sw;
+    double *cu, *cv, *cw;
+    QLines L[3];
+};
+
+template <int R>
+__global__ void __launch_bounds__(256) k_eno_momentum(MomArgs a, Tiles t) {
+    int i, j, k;
+    if (!tile_point(t, i, j, k)) return;
+    const Dims &d = a.d;
+    const GridDev &g = a.g;
+    long id = gidx(d, i, j, k);
+    const double *u = a.u, *v = a.v, *w = a.w;
+    // u node
+    {
+        double r = 0.0;
+        if (a.su[id] != TB_BOUNDARY) {
+            double vu = interp4d(v[id - 1], v[id], v[id + d.px - 1], v[id + d.px], g.xv[i - 1], g.xv[i], g.yv[j],
+                                 g.yv[j + 1], g.xu[i], g.yu[j]);
+            double wu = interp4d(w[id - 1], w[id], w[id + d.plane - 1], w[id + d.plane], g.xw[i - 1], g.xw[i],
+                                 g.zw[k], g.zw[k + 1], g.xu[i], g.zu[k]);
+            r = eno_line_point<R>(a.L[0].ax[0], u, id, i, u[id]);
+            r += eno_line_point<R>(a.L[0].ax[1], u, id, j, vu);
+            r += eno_line_point<R>(a.L[0].ax[2], u, id, k, wu);
+        }
+        a.cu[id] = r;
+    }
+    // v node
+    {
+        double r = 0.0;
+        if (a.sv[id] != TB_BOUNDARY) {
+            double uv = interp4d(u[id - d.px], u[id - d.px + 1], u[id], u[id + 1], g.xu[i], g.xu[i + 1], g.yu[j - 1],
+                                 g.yu[j], g.xv[i], g.yv[j]);
+            double wv = interp4d(w[id - d.px], w[id + d.plane - d.px], w[id], w[id + d.plane], g.zw[k], g.zw[k + 1],
+                                 g.yw[j - 1], g.yw[j], g.zv[k], g.yv[j]);
+            r = eno_line_point<R>(a.L[1].ax[0], v, id, i, uv);
+            r += eno_line_point<R>(a.L[1].ax[1], v, id, j, v[id]);
+            r += eno_line_point<R>(a.L[1].ax[2], v, id, k, wv);
+        }
+        a.cv[id] = r;
+    }
+    // w node
+    {
+        double r = 0.0;
+        if (a.sw[id] != TB_BOUNDARY) {
+            double uw = interp4d(u[id - d.plane], u[id - d.plane + 1], u[id], u[id + 1], g.xu[i], g.xu[i + 1],
+                                 g.zu[k - 1], g.zu[k], g.xw[i], g.zw[k]);
+            double vw = interp4d(v[id - d.plane], v[id], v[id - d.plane + d.px], v[id + d.px], g.zv[k - 1], g.zv[k],
+                                 g.yv[j], g.yv[j + 1], g.zw[k], g.yw[j]);
+            r = eno_line_point<R>(a.L[2].ax[0], w, id, i, uw);
+            r += eno_line_point<R>(a.L[2].ax[1], w, id, j, vw);
+            r += eno_line_point<R>(a.L[2].ax[2], w, id, k, w[id]);
+        }
+        a.cw[id] = r;
+    }
+}
+
+struct ConcArgs {
+    Dims d;
+    GridDev g;
+    const double *u, *v, *w, *c;
+    const uint8_t *sc;
+    double *cc;
+    double v_settle;
+    int particle;
+    QLines L;
+};
+
+template <int R>
+__global__ void __launch_bounds__(256) k_eno_conc(ConcArgs a, Tiles t) {
+    int i, j, k;
+    if (!tile_point(t, i, j, k)) return;
+    const Dims &d = a.d;
+    const GridDev &g = a.g;
+    long id = gidx(d, i, j, k);
+    double r = 0.0;
+    if (a.sc[id] != TB_BOUNDARY) {
+        // Velocity_cell_center (Velocity.c:1476-1602) at this cell
+        double ubc = interp2d(a.u[id], g.xu[i], a.u[id + 1], g.xu[i + 1], g.xc[i]);
+        double vbc = interp2d(a.v[id], g.yv[j], a.v[id + d.px], g.yv[j + 1], g.yc[j]);
+        double wbc = interp2d(a.w[id], g.zw[k], a.w[id + d.plane], g.zw[k + 1], g.zc[k]);
+        if (a.particle) vbc = vbc + a.v_settle;   // Conc_set_particle_v_velocity (Conc.c:734-761)
+        r = eno_line_point<R>(a.L.ax[0], a.c, id, i, ubc);
+        r += eno_line_point<R>(a.L.ax[1], a.c, id, j, vbc);
+        r += eno_line_point<R>(a.L.ax[2], a.c, id, k, wbc);
+    }
+    a.cc[id] = r;
+}
+
+// Line specs per quantity (the ghost rules of Velocity.c:2124-2801 and
+// Conc.c:1021-1220; wall order x0,x1,y0,y1,z0,z1).
+static QLines make_lines(tb_ctx *c, int q) {
+    QLines Q;
+    const Dims &d = c->d;
+    const double *posq[4][3] = {{c->g.xu, c->g.yu, c->g.zu},
+                                {c->g.xv, c->g.yv, c->g.zv},
+                                {c->g.xw, c->g.yw, c->g.zw},
+                                {c->g.xc, c->g.yc, c->g.zc}};
+    int hi[3] = {d.NX - 2, d.NY - 2, d.NZ - 2};
+    double wlo[3] = {c->hgrid[3][0], c->hgrid[7][0], c->hgrid[11][0]};                  // xu[0], yv[0], zw[0]
+    double whi[3] = {c->hgrid[3][d.NX - 1], c->hgrid[7][d.NY - 1], c->hgrid[11][d.NZ - 1]};
+    long stride[3] = {1, d.px, d.plane};
+    for (int ax = 0; ax < 3; ax++) {
+        LineSpec &L = Q.ax[ax];
+        L.lo = (q == ax) ? 1 : 0;
+        L.hi = hi[ax];
+        L.kind = (q == ax) ? 0 : 1;
+        L.ux_quirk = (q == TB_Q_U && ax == 0);
+        L.hi_mode = 0;
+        L.hi_scale = 1.0;
+        L.wall_lo = wlo[ax];
+        L.wall_hi = whi[ax];
+        L.pos = posq[q][ax];
+        L.stride = stride[ax];
+        if (q != TB_Q_C) {
+            L.sgn_lo = c->cfg.wall_noslip[2 * ax] ? -1.0 : 1.0;
+            L.sgn_hi = c->cfg.wall_noslip[2 * ax + 1] ? -1.0 : 1.0;
+        } else {
+            L.sgn_lo = 1.0;
+            L.sgn_hi = 1.0;
+            if (ax == 1 && c->cfg.conc_type == TB_PARTICLE && c->cfg.sedimentation) {
+                double coef = 2.0 / (c->cfg.pe * c->dXi) * c->hgrid[19][d.NY - 1];
+                double Vs = fabs(c->cfg.v_settle);
+                L.hi_mode = 1;
+                L.hi_scale = (coef - Vs) / (coef + Vs);
+            }
+        }
+    }
+    return Q;
+}
+
+int tb_eno_momentum(tb_ctx *c) {
+    MomArgs a;
+    a.d = c->d; a.g = c->g;
+    a.u = c->f[TB_F_U]; a.v = c->f[TB_F_V]; a.w = c->f[TB_F_W];
+    a.su = c->st[0]; a.sv = c->st[1]; a.sw = c->st[2];
+    a.cu = c->f[TB_F_CONV_U]; a.cv = c->f[TB_F_CONV_V]; a.cw = c->f[TB_F_CONV_W];
+    for (int q = 0; q < 3; q++) a.L[q] = make_lines(c, q);
+    Tiles t = make_tiles(c->d.NX, c->d.NY, c->d.NZ);
+    switch (c->cfg.eno_order) {
+    case 1: hipLaunchKernelGGL(k_eno_momentum<1>, tiles_grid(t), tiles_block(), 0, c->s, a, t); break;
+    case 2: hipLaunchKernelGGL(k_eno_momentum<2>, tiles_grid(t), tiles_block(), 0, c->s, a, t); break;
+    default: hipLaunchKernelGGL(k_eno_momentum<3>, tiles_grid(t), tiles_block(), 0, c->s, a, t); break;
+    }
+    TB_LAUNCH_CHECK(c);
+    return 0;
+}
+
+int tb_eno_conc(tb_ctx *c) {
+    ConcArgs a;
+    a.d = c->d; a.g = c->g;
+    a.u = c->f[TB_F_U]; a.v = c->f[TB_F_V]; a.w = c->f[TB_F_W]; a.c = c->f[TB_F_C];
+    a.sc = c->st[3];
+    a.cc = c->f[TB_F_CONV_C];
+    a.v_settle = c->cfg.v_settle;
+    a.particle = (c->cfg.conc_type == TB_PARTICLE);
+    a.L = make_lines(c, TB_Q_C);
+    Tiles t = make_tiles(c->d.NX, c->d.NY, c->d.NZ);
+    switch (c->cfg.eno_order) {
+    case 1: hipLaunchKernelGGL(k_eno_conc<1>, tiles_grid(t), tiles_block(), 0, c->s, a, t); break;
+    case 2: hipLaunchKernelGGL(k_eno_conc<2>, tiles_grid(t), tiles_block(), 0, c->s, a, t); break;
+    default: hipLaunchKernelGGL(k_eno_conc<3>, tiles_grid(t), tiles_block(), 0, c->s, a, t); break;
+    }
+    TB_LAUNCH_CHECK(c);
+    return 0;
+}
+
+extern "C" int tb_eno_conv(tb_ctx *c, int q) {
+    if (!c || q < -1 || q > 3) { if (c) c->err = "tb_eno_conv: bad quantity"; return -1; }
+    int rc = 0;
+    if (q != TB_Q_C) rc = tb_eno_momentum(c);   // u, v, w are produced together
+    if (rc == 0 && (q == TB_Q_C || q == -1)) rc = tb_eno_conc(c);
+    if (rc == 0) TB_CHECK(c, hipStreamSynchronize(c->s));
+    return rc;
+}
+
+// ---------------------------------------------------------------------------
+// Right-hand sides: Velocity_u/v/w_set_RHS (Velocity.c:1607-2043),
+// Conc_set_RHS (Conc.c:864-992).
+
+__global__ void k_rhs_vel(Dims d, GridDev g, int q, const double *__restrict__ data, const double *__restrict__ conv,
+                          const double *__restrict__ dp, const double *__restrict__ cfld,
+                          const uint8_t *__restrict__ st, double *__restrict__ rhs, double a_dt, double buoy,
+                          Tiles t) {
+    int i, j, k;
+    if (!tile_point(t, i, j, k)) return;
+    long id = gidx(d, i, j, k);
+    int s = st[id];
+    double r = 0.0;
+    if (s == TB_FLUID || s == TB_BUFFER_SOLID) {
+        if (q == TB_Q_V) {
+            double c_term = interp2d(cfld[id - d.px], g.yc[j - 1], cfld[id], g.yc[j], g.yv[j]);
+            r = data[id] * a_dt - conv[id] - dp[id] - buoy * c_term;
+        } else {
+            r = data[id] * a_dt - conv[id] - dp[id];
+        }
+    }
+    rhs[id] = r;
+}
+
+__global__ void k_rhs_conc(Dims d, const double *__restrict__ data, const double *__restrict__ conv,
+                           const uint8_t *__restrict__ st, double *__restrict__ rhs, double a_dt, Tiles t) {
+    int i, j, k;
+    if (!tile_point(t, i, j, k)) return;
+    long id = gidx(d, i, j, k);
+    rhs[id] = (st[id] == TB_FLUID) ? data[id] * a_dt - conv[id] : 0.0;
+}
+
+extern "C" int tb_set_rhs(tb_ctx *c, int q, double dt) {
+    if (!c || q < 0 || q > 3) { if (c) c->err = "tb_set_rhs: bad quantity"; return -1; }
+    Tiles t = make_tiles(c->d.NX, c->d.NY, c->d.NZ);
+    double a_dt = 1.0 / dt;
+    if (q == TB_Q_C)
+        hipLaunchKernelGGL(k_rhs_conc, tiles_grid(t), tiles_block(), 0, c->s, c->d, c->f[TB_F_C], c->f[TB_F_CONV_C],
+                           c->st[3], c->f[TB_F_RHS_C], a_dt, t);
+    else
+        hipLaunchKernelGGL(k_rhs_vel, tiles_grid(t), tiles_block(), 0, c->s, c->d, c->g, q, c->f[TB_F_U + q],
+                           c->f[TB_F_CONV_U + q], c->f[TB_F_DPDX + q], c->f[TB_F_C], c->st[q], c->f[TB_F_RHS_U + q],
+                           a_dt, c->cfg.buoyancy, t);
+    TB_LAUNCH_CHECK(c);
+    return 0;
+}

@@ -1,0 +1,893 @@
+// tb_poisson.hip -- pressure Poisson right-hand side, operator, geometric
+// multigrid (FP64 and FP32), GMG-preconditioned CG, mixed-precision
+// iterative refinement, projection and the divergence check.
+//
+// The reference assembles a 7-point -Laplacian over the whole box with
+// Neumann walls and one pinned corner cell (Pressure_form_LHS_matrix,
+// Pressure.c:691-936) and hands it to GMRES + BoomerAMG.  Here the operator
+// is applied matrix-free with the same row values; the solve runs CG on the
+// Neumann operator (the same matrix without the pin) with a symmetric V-cycle
+// preconditioner, then shifts the result so the pinned cell holds 0.  For a
+// compatible right-hand side (sum b = 0, which the divergence RHS satisfies
+// up to rounding) that is the unique solution of the pinned system.
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "tb_common.h"
+
+// ---------------------------------------------------------------------------
+// Pressure_set_RHS (Pressure.c:171-276)
+__global__ void k_prhs(Dims d, GridDev g, const double *__restrict__ u, const double *__restrict__ v,
+                       const double *__restrict__ w, const uint8_t *__restrict__ sc, double *__restrict__ rhs,
+                       double a_dEta, double a_dXi, double a_dPhi, double a_dt, Tiles t) {
+    int i, j, k;
+    if (!tile_point(t, i, j, k)) return;
+    long id = gidx(d, i, j, k);
+    double r = 0.0;
+    if (sc[id] != TB_BOUNDARY) {
+        double C_z = g.mzc[k] * a_dPhi;
+        double C_y = g.myc[j] * a_dXi;
+        double C_x = g.mxc[i] * a_dEta;
+        double dudx = (u[id + 1] - u[id]) * C_x;
+        double dvdy = (v[id + d.px] - v[id]) * C_y;
+        double dwdz = (w[id + d.plane] - w[id]) * C_z;
+        r = -(dudx + dvdy + dwdz) * a_dt;
+    }
+    rhs[id] = r;
+}
+
+extern "C" int tb_poisson_rhs(tb_ctx *c, double dt) {
+    if (!c) return -1;
+    Tiles t = make_tiles(c->d.NX, c->d.NY, c->d.NZ);
+    hipLaunchKernelGGL(k_prhs, tiles_grid(t), tiles_block(), 0, c->s, c->d, c->g, c->f[TB_F_U], c->f[TB_F_V],
+                       c->f[TB_F_W], c->st[3], c->f[TB_F_PRHS], 1.0 / c->dEta, 1.0 / c->dXi, 1.0 / c->dPhi, 1.0 / dt, t);
+    TB_LAUNCH_CHECK(c);
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Operator rows (Pressure.c:740-924); neumann = 1 drops the corner pin.
+struct PoisOp {
+    Dims d;
+    GridDev g;
+    double Cx, Cy, Cz, ap_solid;
+    int neumann;
+};
+
+__device__ inline double pois_row(const PoisOp &P, int i, int j, int k, long id, const double *__restrict__ x) {
+    const Dims &d = P.d;
+    const GridDev &g = P.g;
+    int NI = d.NX - 1, NJ = d.NY - 1, NK = d.NZ - 1;
+    if (i == NI || j == NJ || k == NK) return P.ap_solid * x[id];   // BOUNDARY half-cell rows
+    const double Cx = P.Cx, Cy = P.Cy, Cz = P.Cz;
+    double mpx = g.mxc[i], mpy = g.myc[j], mpz = g.mzc[k];
+    double mex = g.mxu[i + 1], mwx = g.mxu[i], mny = g.myv[j + 1], msy = g.myv[j];
+    double mfz = g.mzw[k + 1], mbz = g.mzw[k];
+    double ap = -Cx * mpx * (mex + mwx) - Cy * mpy * (mny + msy) - Cz * mpz * (mbz + mfz);
+    double aw = Cx * mpx * mwx, ae = Cx * mpx * mex, an = Cy * mpy * mny;
+    double as = Cy * mpy * msy, ab = Cz * mpz * mbz, af = Cz * mpz * mfz;
+    bool hb = k > 0, hs = j > 0, hw = i > 0, he = i < NI - 1, hn = j < NJ - 1, hf = k < NK - 1;
+    if (!hb) ap += ab;
+    if (!hs) ap += as;
+    if (!hw) ap += aw;
+    if (!he && (P.neumann || !((i == NI - 1) && (j == NJ - 1) && (k == NK - 1)))) ap += ae;
+    if (!hn) ap += an;
+    if (!hf) ap += af;
+    double r = 0.0;
+    if (hb) r += ab * x[id - d.plane];
+    if (hs) r += as * x[id - d.px];
+    if (hw) r += aw * x[id - 1];
+    r += ap * x[id];
+    if (he) r += ae * x[id + 1];
+    if (hn) r += an * x[id + d.px];
+    if (hf) r += af * x[id + d.plane];
+    return r;
+}
+
+// y = A x, partial sum of x.y
+__global__ void __launch_bounds__(256) k_pois_apply(PoisOp P, const double *__restrict__ x, double *__restrict__ y,
+                                                    double *part, Tiles t) {
+    __shared__ double sh[16];
+    double acc[1] = {0.0};
+    int i, j, k;
+    if (tile_point(t, i, j, k)) {
+        long id = gidx(P.d, i, j, k);
+        double v = pois_row(P, i, j, k, id, x);
+        y[id] = v;
+        acc[0] = x[id] * v;
+    }
+    block_sum<1>(acc, sh);
+    if (threadIdx.x == 0 && threadIdx.y == 0) part[blockIdx.x] = acc[0];
+}
+
+// r = b - A x ; partials r.r and b.b ; optionally r32 = (float) r
+__global__ void __launch_bounds__(256) k_pois_resid(PoisOp P, const double *__restrict__ x,
+                                                    const double *__restrict__ b, double *__restrict__ r,
+                                                    float *__restrict__ r32, double *part, Tiles t) {
+    __shared__ double sh[32];
+    double acc[2] = {0.0, 0.0};
+    int i, j, k;
+    if (tile_point(t, i, j, k)) {
+        long id = gidx(P.d, i, j, k);
+        double bi = b[id];
+        double ri = bi - pois_row(P, i, j, k, id, x);
+        if (r) r[id] = ri;
+        if (r32) r32[id] = (float)ri;
+        acc[0] = ri * ri;
+        acc[1] = bi * bi;
+    }
+    block_sum<2>(acc, sh);
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        part[blockIdx.x] = acc[0];
+        part[RED_MAX_BLOCKS + blockIdx.x] = acc[1];
+    }
+}
+
+static PoisOp make_pois(tb_ctx *c, int neumann) {
+    PoisOp P;
+    P.d = c->d; P.g = c->g;
+    P.Cx = -1.0 / (c->dEta * c->dEta);
+    P.Cy = -1.0 / (c->dXi * c->dXi);
+    P.Cz = -1.0 / (c->dPhi * c->dPhi);
+    P.ap_solid = -P.Cx * c->hgrid[15][0] - P.Cy * c->hgrid[16][0] - P.Cz * c->hgrid[17][0];
+    P.neumann = neumann;
+    return P;
+}
+
+extern "C" int tb_poisson_apply(tb_ctx *c, int f_in, int f_out) {
+    if (!c || f_in < 0 || f_in >= TB_F_COUNT || f_out < 0 || f_out >= TB_F_COUNT || f_in == f_out) {
+        if (c) c->err = "tb_poisson_apply: bad argument";
+        return -1;
+    }
+    Tiles t = make_tiles(c->d.NX, c->d.NY, c->d.NZ);
+    if (t.per_xcd * NXCD > RED_MAX_BLOCKS * 64) { c->err = "grid too large for reduction buffer"; return -1; }
+    double *part = nullptr;
+    TB_CHECK(c, hipMalloc((void **)&part, sizeof(double) * t.per_xcd * NXCD));
+    hipLaunchKernelGGL(k_pois_apply, tiles_grid(t), tiles_block(), 0, c->s, make_pois(c, 0), c->f[f_in], c->f[f_out],
+                       part, t);
+    TB_LAUNCH_CHECK(c);
+    TB_CHECK(c, hipStreamSynchronize(c->s));
+    hipFree(part);
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Geometric multigrid on cell-centred levels.  Level 0 is the interior of
+// the padded field layout; coarser levels are compact.  Coarsening halves a
+// direction when its spacing is at most half the largest one (semi-coarsening
+// the strongly coupled directions first), stops at <= 512 cells, and the
+// coarsest level is solved with a dense pseudo-inverse (A + J/n)^-1.
+
+template <typename T>
+struct LV {
+    int nx, ny, nz;
+    int px, py;          // row pitch, rows per plane
+    T cx, cy, cz;        // 1/h^2
+};
+
+template <typename T>
+__device__ inline long lidx(const LV<T> &L, int i, int j, int k) {
+    return ((long)k * L.py + j) * L.px + i;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_zero(LV<T> L, T *__restrict__ x, Tiles t) {
+    int i, j, k;
+    if (!tile_point(t, i, j, k)) return;
+    x[lidx(L, i, j, k)] = (T)0;
+}
+
+// red-black Gauss-Seidel half sweep; threads tile (ceil(nx/2), ny, nz)
+template <typename T>
+__global__ void __launch_bounds__(256) k_rbgs(LV<T> L, T *__restrict__ x, const T *__restrict__ b, int color,
+                                              Tiles t) {
+    int ih, j, k;
+    if (!tile_point(t, ih, j, k)) return;
+    int i = 2 * ih + ((j + k + color) & 1);
+    if (i >= L.nx) return;
+    long id = lidx(L, i, j, k);
+    long pl = (long)L.px * L.py;
+    T s = b[id], dg = (T)0;
+    if (i > 0) { s += L.cx * x[id - 1]; dg += L.cx; }
+    if (i < L.nx - 1) { s += L.cx * x[id + 1]; dg += L.cx; }
+    if (j > 0) { s += L.cy * x[id - L.px]; dg += L.cy; }
+    if (j < L.ny - 1) { s += L.cy * x[id + L.px]; dg += L.cy; }
+    if (k > 0) { s += L.cz * x[id - pl]; dg += L.cz; }
+    if (k < L.nz - 1) { s += L.cz * x[id + pl]; dg += L.cz; }
+    x[id] = s / dg;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_mg_resid(LV<T> L, const T *__restrict__ x, const T *__restrict__ b,
+                                                  T *__restrict__ r, Tiles t) {
+    int i, j, k;
+    if (!tile_point(t, i, j, k)) return;
+    long id = lidx(L, i, j, k);
+    long pl = (long)L.px * L.py;
+    T xc = x[id], s = (T)0;
+    if (i > 0) s += L.cx * (xc - x[id - 1]);
+    if (i < L.nx - 1) s += L.cx * (xc - x[id + 1]);
+    if (j > 0) s += L.cy * (xc - x[id - L.px]);
+    if (j < L.ny - 1) s += L.cy * (xc - x[id + L.px]);
+    if (k > 0) s += L.cz * (xc - x[id - pl]);
+    if (k < L.nz - 1) s += L.cz * (xc - x[id + pl]);
+    r[id] = b[id] - s;
+}
+
+__device__ inline int clampi(int a, int lo, int hi) { return a < lo ? lo : (a > hi ? hi : a); }
+
+// restriction: adjoint of the prolongation / 2 per halved axis,
+// 1-D weights (1/8, 3/8, 3/8, 1/8) with reflection at the walls
+template <typename T>
+__global__ void __launch_bounds__(256) k_restrict(LV<T> F, LV<T> C, int cx, int cy, int cz,
+                                                  const T *__restrict__ rf, T *__restrict__ bc, Tiles t) {
+    int I, J, K;
+    if (!tile_point(t, I, J, K)) return;
+    const T W[4] = {(T)0.125, (T)0.375, (T)0.375, (T)0.125};
+    T s = (T)0;
+    int tk0 = cz ? -1 : 0, tk1 = cz ? 2 : 0;
+    int tj0 = cy ? -1 : 0, tj1 = cy ? 2 : 0;
+    int ti0 = cx ? -1 : 0, ti1 = cx ? 2 : 0;
+    for (int tk = tk0; tk <= tk1; tk++) {
+        int kf = cz ? clampi(2 * K + tk, 0, F.nz - 1) : K;
+        T wk = cz ? W[tk + 1] : (T)1;
+        for (int tj = tj0; tj <= tj1; tj++) {
+            int jf = cy ? clampi(2 * J + tj, 0, F.ny - 1) : J;
+            T wj = cy ? W[tj + 1] : (T)1;
+            T sj = (T)0;
+            for (int ti = ti0; ti <= ti1; ti++) {
+                int ifn = cx ? clampi(2 * I + ti, 0, F.nx - 1) : I;
+                T wi = cx ? W[ti + 1] : (T)1;
+                sj += wi * rf[lidx(F, ifn, jf, kf)];
+            }
+            s += wk * wj * sj;
+        }
+    }
+    bc[lidx(C, I, J, K)] = s;
+}
+
+// prolongation: (3/4, 1/4) per halved axis, reflecting; x_f += P x_c
+template <typename T>
+__global__ void __launch_bounds__(256) k_prolong(LV<T> F, LV<T> C, int cx, int cy, int cz, const T *__restrict__ xc,
+                                                 T *__restrict__ xf, Tiles t) {
+    int i, j, k;
+    if (!tile_point(t, i, j, k)) return;
+    int K0, K1, J0, J1, I0, I1;
+    T wk1, wj1, wi1;
+    if (cz) { K0 = k >> 1; K1 = clampi((k & 1) ? K0 + 1 : K0 - 1, 0, C.nz - 1); wk1 = (T)0.25; } else { K0 = K1 = k; wk1 = (T)0; }
+    if (cy) { J0 = j >> 1; J1 = clampi((j & 1) ? J0 + 1 : J0 - 1, 0, C.ny - 1); wj1 = (T)0.25; } else { J0 = J1 = j; wj1 = (T)0; }
+    if (cx) { I0 = i >> 1; I1 = clampi((i & 1) ? I0 + 1 : I0 - 1, 0, C.nx - 1); wi1 = (T)0.25; } else { I0 = I1 = i; wi1 = (T)0; }
+    T wk0 = (T)1 - wk1, wj0 = (T)1 - wj1, wi0 = (T)1 - wi1;
+    T s = wk0 * (wj0 * (wi0 * xc[lidx(C, I0, J0, K0)] + wi1 * xc[lidx(C, I1, J0, K0)]) +
+                 wj1 * (wi0 * xc[lidx(C, I0, J1, K0)] + wi1 * xc[lidx(C, I1, J1, K0)])) +
+          wk1 * (wj0 * (wi0 * xc[lidx(C, I0, J0, K1)] + wi1 * xc[lidx(C, I1, J0, K1)]) +
+                 wj1 * (wi0 * xc[lidx(C, I0, J1, K1)] + wi1 * xc[lidx(C, I1, J1, K1)]));
+    xf[lidx(F, i, j, k)] += s;
+}
+
+// coarsest level: x = Minv (b - mean b), then x -= mean x  (one block)
+template <typename T>
+__global__ void k_coarse(const T *__restrict__ Minv, int n, const T *__restrict__ b, T *__restrict__ x) {
+    __shared__ double sh[16];
+    __shared__ double bb[1024];
+    __shared__ double mean_s;
+    double v[1] = {0.0};
+    for (int a = threadIdx.x; a < n; a += blockDim.x) v[0] += (double)b[a];
+    block_sum<1>(v, sh);
+    if (threadIdx.x == 0) mean_s = v[0] / n;
+    __syncthreads();
+    double mb = mean_s;
+    for (int a = threadIdx.x; a < n; a += blockDim.x) bb[a] = (double)b[a] - mb;
+    __syncthreads();
+    double xr[1] = {0.0};
+    double mine = 0.0;
+    if ((int)threadIdx.x < n) {
+        double s = 0.0;
+        const T *row = Minv + (long)threadIdx.x * n;
+        for (int cc = 0; cc < n; cc++) s += (double)row[cc] * bb[cc];
+        mine = s;
+        xr[0] = s;
+    }
+    __syncthreads();
+    block_sum<1>(xr, sh);
+    if (threadIdx.x == 0) mean_s = xr[0] / n;
+    __syncthreads();
+    if ((int)threadIdx.x < n) x[threadIdx.x] = (T)(mine - mean_s);
+}
+
+template <typename T>
+static LV<T> lv_of(const Level &L) {
+    LV<T> v;
+    v.nx = L.nx; v.ny = L.ny; v.nz = L.nz;
+    v.px = L.px; v.py = (int)(L.plane / L.px);
+    v.cx = (T)(1.0 / (L.hx * L.hx));
+    v.cy = (T)(1.0 / (L.hy * L.hy));
+    v.cz = (T)(1.0 / (L.hz * L.hz));
+    return v;
+}
+
+static int choose(const Level &L, int &cx, int &cy, int &cz) {
+    int n[3] = {L.nx, L.ny, L.nz};
+    double h[3] = {L.hx, L.hy, L.hz};
+    double hmax = h[0];
+    if (h[1] > hmax) hmax = h[1];
+    if (h[2] > hmax) hmax = h[2];
+    int cc[3] = {0, 0, 0}, any = 0, best = -1;
+    for (int a = 0; a < 3; a++) {
+        int can = (n[a] % 2 == 0) && (n[a] >= 4);
+        if (can && 2.0 * h[a] <= hmax * (1.0 + 1e-9)) { cc[a] = 1; any = 1; }
+        if (can && (best < 0 || h[a] < h[best])) best = a;
+    }
+    if (!any && best >= 0) { cc[best] = 1; any = 1; }
+    cx = cc[0]; cy = cc[1]; cz = cc[2];
+    return any;
+}
+
+static void level_apply_host(const Level &L, const double *x, double *y) {
+    double cx = 1.0 / (L.hx * L.hx), cy = 1.0 / (L.hy * L.hy), cz = 1.0 / (L.hz * L.hz);
+    long pl = (long)L.nx * L.ny;
+    for (int k = 0; k < L.nz; k++)
+        for (int j = 0; j < L.ny; j++)
+            for (int i = 0; i < L.nx; i++) {
+                long id = ((long)k * L.ny + j) * L.nx + i;
+                double xc = x[id], s = 0.0;
+                if (i > 0) s += cx * (xc - x[id - 1]);
+                if (i < L.nx - 1) s += cx * (xc - x[id + 1]);
+                if (j > 0) s += cy * (xc - x[id - L.nx]);
+                if (j < L.ny - 1) s += cy * (xc - x[id + L.nx]);
+                if (k > 0) s += cz * (xc - x[id - pl]);
+                if (k < L.nz - 1) s += cz * (xc - x[id + pl]);
+                y[id] = s;
+            }
+}
+
+void tb_mg_free(tb_ctx *c) {
+    for (int l = 0; l < c->nlev; l++) {
+        Level &L = c->lev[l];
+        if (l > 0) { hipFree(L.x64); hipFree(L.b64); hipFree(L.r64); }
+        hipFree(L.x32); hipFree(L.b32); hipFree(L.r32);
+        L.x64 = L.b64 = L.r64 = nullptr;
+        L.x32 = L.b32 = L.r32 = nullptr;
+    }
+    hipFree(c->coarse_inv64); hipFree(c->coarse_inv32);
+    c->coarse_inv64 = nullptr; c->coarse_inv32 = nullptr;
+    c->nlev = 0;
+    c->mg_built = false;
+}
+
+int tb_mg_build(tb_ctx *c) {
+    if (c->mg_built) return 0;
+    Level &L0 = c->lev[0];
+    L0 = Level();
+    L0.nx = c->ncx; L0.ny = c->ncy; L0.nz = c->ncz;
+    L0.px = c->d.px; L0.plane = c->d.plane; L0.ntot = c->d.ntot;
+    L0.hx = c->cfg.lx / c->ncx; L0.hy = c->cfg.ly / c->ncy; L0.hz = c->cfg.lz / c->ncz;
+    int nl = 1;
+    while (nl < 16) {
+        Level &L = c->lev[nl - 1];
+        if ((long)L.nx * L.ny * L.nz <= 512) break;
+        int cx, cy, cz;
+        if (!choose(L, cx, cy, cz)) break;
+        L.cx = cx; L.cy = cy; L.cz = cz;
+        Level &C = c->lev[nl];
+        C = Level();
+        C.nx = cx ? L.nx / 2 : L.nx; C.hx = cx ? 2.0 * L.hx : L.hx;
+        C.ny = cy ? L.ny / 2 : L.ny; C.hy = cy ? 2.0 * L.hy : L.hy;
+        C.nz = cz ? L.nz / 2 : L.nz; C.hz = cz ? 2.0 * L.hz : L.hz;
+        C.px = (C.nx + 15) / 16 * 16;
+        C.plane = (long)C.px * C.ny;
+        C.ntot = C.plane * C.nz;
+        nl++;
+    }
+    c->nlev = nl;
+    for (int l = 0; l < nl; l++) {
+        Level &L = c->lev[l];
+        if (l > 0) {
+            TB_CHECK(c, hipMalloc((void **)&L.x64, sizeof(double) * L.ntot));
+            TB_CHECK(c, hipMalloc((void **)&L.b64, sizeof(double) * L.ntot));
+            TB_CHECK(c, hipMalloc((void **)&L.r64, sizeof(double) * L.ntot));
+            TB_CHECK(c, hipMemsetAsync(L.x64, 0, sizeof(double) * L.ntot, c->s));
+            TB_CHECK(c, hipMemsetAsync(L.b64, 0, sizeof(double) * L.ntot, c->s));
+            TB_CHECK(c, hipMemsetAsync(L.r64, 0, sizeof(double) * L.ntot, c->s));
+            c->bytes += 3 * sizeof(double) * L.ntot;
+        }
+        TB_CHECK(c, hipMalloc((void **)&L.x32, sizeof(float) * L.ntot));
+        TB_CHECK(c, hipMalloc((void **)&L.b32, sizeof(float) * L.ntot));
+        TB_CHECK(c, hipMalloc((void **)&L.r32, sizeof(float) * L.ntot));
+        TB_CHECK(c, hipMemsetAsync(L.x32, 0, sizeof(float) * L.ntot, c->s));
+        TB_CHECK(c, hipMemsetAsync(L.b32, 0, sizeof(float) * L.ntot, c->s));
+        TB_CHECK(c, hipMemsetAsync(L.r32, 0, sizeof(float) * L.ntot, c->s));
+        c->bytes += 3 * sizeof(float) * L.ntot;
+    }
+    // dense inverse of (A_c + J/n) on the coarsest level (Gauss-Jordan,
+    // partial pivoting), FP64 on the host
+    Level &Lc = c->lev[nl - 1];
+    int n = Lc.nx * Lc.ny * Lc.nz;
+    if (n > 1024) { c->err = "coarsest multigrid level too large (odd grid dimensions?)"; return -1; }
+    c->coarse_n = n;
+    std::vector<double> A((size_t)n * n), I((size_t)n * n, 0.0), e(n), col(n);
+    for (int a = 0; a < n; a++) {
+        std::fill(e.begin(), e.end(), 0.0);
+        e[a] = 1.0;
+        level_apply_host(Lc, e.data(), col.data());
+        for (int r = 0; r < n; r++) A[(size_t)r * n + a] = col[r] + 1.0 / n;
+        I[(size_t)a * n + a] = 1.0;
+    }
+    for (int cc = 0; cc < n; cc++) {
+        int piv = cc;
+        for (int r = cc + 1; r < n; r++) if (fabs(A[(size_t)r * n + cc]) > fabs(A[(size_t)piv * n + cc])) piv = r;
+        if (piv != cc)
+            for (int k = 0; k < n; k++) {
+                std::swap(A[(size_t)cc * n + k], A[(size_t)piv * n + k]);
+                std::swap(I[(size_t)cc * n + k], I[(size_t)piv * n + k]);
+            }
+        double dd = 1.0 / A[(size_t)cc * n + cc];
+        for (int k = 0; k < n; k++) { A[(size_t)cc * n + k] *= dd; I[(size_t)cc * n + k] *= dd; }
+        for (int r = 0; r < n; r++) {
+            if (r == cc) continue;
+            double f = A[(size_t)r * n + cc];
+            if (f == 0.0) continue;
+            for (int k = 0; k < n; k++) { A[(size_t)r * n + k] -= f * A[(size_t)cc * n + k]; I[(size_t)r * n + k] -= f * I[(size_t)cc * n + k]; }
+        }
+    }
+    std::vector<float> I32((size_t)n * n);
+    for (size_t a = 0; a < I.size(); a++) I32[a] = (float)I[a];
+    TB_CHECK(c, hipMalloc((void **)&c->coarse_inv64, sizeof(double) * (size_t)n * n));
+    TB_CHECK(c, hipMalloc((void **)&c->coarse_inv32, sizeof(float) * (size_t)n * n));
+    TB_CHECK(c, hipMemcpy(c->coarse_inv64, I.data(), sizeof(double) * (size_t)n * n, hipMemcpyHostToDevice));
+    TB_CHECK(c, hipMemcpy(c->coarse_inv32, I32.data(), sizeof(float) * (size_t)n * n, hipMemcpyHostToDevice));
+    TB_CHECK(c, hipStreamSynchronize(c->s));
+    c->mg_built = true;
+    return 0;
+}
+
+template <typename T>
+struct MGBuf {
+    T *x, *b, *r;
+};
+
+template <typename T>
+static MGBuf<T> bufs(tb_ctx *c, int l);
+template <>
+MGBuf<double> bufs<double>(tb_ctx *c, int l) { return {c->lev[l].x64, c->lev[l].b64, c->lev[l].r64}; }
+template <>
+MGBuf<float> bufs<float>(tb_ctx *c, int l) { return {c->lev[l].x32, c->lev[l].b32, c->lev[l].r32}; }
+
+template <typename T>
+static const T *coarse_inv(tb_ctx *c);
+template <>
+const double *coarse_inv<double>(tb_ctx *c) { return c->coarse_inv64; }
+template <>
+const float *coarse_inv<float>(tb_ctx *c) { return c->coarse_inv32; }
+
+template <typename T>
+static int vcycle_level(tb_ctx *c, int l) {
+    const Level &L = c->lev[l];
+    MGBuf<T> B = bufs<T>(c, l);
+    LV<T> lv = lv_of<T>(L);
+    if (l == c->nlev - 1) {
+        hipLaunchKernelGGL(k_coarse<T>, dim3(1), dim3(1024), 0, c->s, coarse_inv<T>(c), c->coarse_n, B.b, B.x);
+        TB_LAUNCH_CHECK(c);
+        return 0;
+    }
+    Tiles all = make_tiles(L.nx, L.ny, L.nz);
+    Tiles half = make_tiles((L.nx + 1) / 2, L.ny, L.nz);
+    hipLaunchKernelGGL(k_zero<T>, tiles_grid(all), tiles_block(), 0, c->s, lv, B.x, all);
+    for (int s = 0; s < c->cfg.mg_pre; s++) {
+        hipLaunchKernelGGL(k_rbgs<T>, tiles_grid(half), tiles_block(), 0, c->s, lv, B.x, (const T *)B.b, 0, half);
+        hipLaunchKernelGGL(k_rbgs<T>, tiles_grid(half), tiles_block(), 0, c->s, lv, B.x, (const T *)B.b, 1, half);
+    }
+    hipLaunchKernelGGL(k_mg_resid<T>, tiles_grid(all), tiles_block(), 0, c->s, lv, (const T *)B.x, (const T *)B.b,
+                       B.r, all);
+    const Level &Cl = c->lev[l + 1];
+    MGBuf<T> CB = bufs<T>(c, l + 1);
+    LV<T> cv = lv_of<T>(Cl);
+    Tiles call = make_tiles(Cl.nx, Cl.ny, Cl.nz);
+    hipLaunchKernelGGL(k_restrict<T>, tiles_grid(call), tiles_block(), 0, c->s, lv, cv, L.cx, L.cy, L.cz,
+                       (const T *)B.r, CB.b, call);
+    TB_LAUNCH_CHECK(c);
+    if (vcycle_level<T>(c, l + 1)) return -1;
+    hipLaunchKernelGGL(k_prolong<T>, tiles_grid(all), tiles_block(), 0, c->s, lv, cv, L.cx, L.cy, L.cz,
+                       (const T *)CB.x, B.x, all);
+    for (int s = 0; s < c->cfg.mg_post; s++) {
+        hipLaunchKernelGGL(k_rbgs<T>, tiles_grid(half), tiles_block(), 0, c->s, lv, B.x, (const T *)B.b, 1, half);
+        hipLaunchKernelGGL(k_rbgs<T>, tiles_grid(half), tiles_block(), 0, c->s, lv, B.x, (const T *)B.b, 0, half);
+    }
+    TB_LAUNCH_CHECK(c);
+    return 0;
+}
+
+// z = V(r) in FP64 with level 0 aliased onto the padded vectors
+int tb_vcycle64(tb_ctx *c, const double *r_pad, double *z_pad) {
+    if (tb_mg_build(c)) return -1;
+    c->lev[0].b64 = const_cast<double *>(r_pad);
+    c->lev[0].x64 = z_pad;
+    c->lev[0].r64 = c->kv[4];
+    return vcycle_level<double>(c, 0);
+}
+
+// lev[0].x32 = V(lev[0].b32) in FP32
+int tb_vcycle32(tb_ctx *c) {
+    if (tb_mg_build(c)) return -1;
+    return vcycle_level<float>(c, 0);
+}
+
+// ---------------------------------------------------------------------------
+// vector kernels of the Poisson solvers (grid-stride over the padded array)
+constexpr int PB = 1024;
+constexpr int PT = 256;
+
+// x += alpha p ; r -= alpha q ; partial r.r      (alpha = rz / pq)
+__global__ void k_pcg_xr(long n, double *__restrict__ x, double *__restrict__ r, const double *__restrict__ p,
+                         const double *__restrict__ q, const double *__restrict__ s, double *part) {
+    __shared__ double sh[16];
+    double alpha = s[S_RZ] / s[S_PQ];
+    double acc[1] = {0.0};
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        x[i] += alpha * p[i];
+        double ri = r[i] - alpha * q[i];
+        r[i] = ri;
+        acc[0] += ri * ri;
+    }
+    block_sum<1>(acc, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc[0];
+}
+
+// p = z + beta p with beta = rz_new / rz ; rz <- rz_new (thread 0 of block 0
+// writes it after every block has read the old value: done by the next launch)
+__global__ void k_pcg_p(long n, const double *__restrict__ z, double *__restrict__ p, const double *__restrict__ s) {
+    double beta = s[S_TMP0] / s[S_RZ];
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        p[i] = z[i] + beta * p[i];
+}
+
+__global__ void k_copy_scalar(double *s, int dst, int src) {
+    if (threadIdx.x == 0) s[dst] = s[src];
+}
+
+// x -= x[corner] on non-boundary cells ; boundary entries set to 0
+__global__ void __launch_bounds__(256) k_shift(Dims d, double *__restrict__ x, const double *__restrict__ s, Tiles t) {
+    int i, j, k;
+    if (!tile_point(t, i, j, k)) return;
+    long id = gidx(d, i, j, k);
+    bool bnd = (i == d.NX - 1) || (j == d.NY - 1) || (k == d.NZ - 1);
+    x[id] = bnd ? 0.0 : x[id] - s[S_CORNER];
+}
+
+// x += (double) e32
+__global__ void __launch_bounds__(256) k_add32(Dims d, double *__restrict__ x, const float *__restrict__ e, Tiles t) {
+    int i, j, k;
+    if (!tile_point(t, i, j, k)) return;
+    long id = gidx(d, i, j, k);
+    x[id] += (double)e[id];
+}
+
+static int read_s(tb_ctx *c, int first, int count) {
+    TB_CHECK(c, hipMemcpyAsync(c->hscal + first, c->scal + first, sizeof(double) * count, hipMemcpyDeviceToHost, c->s));
+    TB_CHECK(c, hipStreamSynchronize(c->s));
+    return 0;
+}
+
+struct PoisCtx {
+    Tiles all, cells;
+    PoisOp Pp, Pn;
+    long nblk;
+    long corner;
+};
+
+static PoisCtx pois_ctx(tb_ctx *c) {
+    PoisCtx P;
+    P.all = make_tiles(c->d.NX, c->d.NY, c->d.NZ);
+    P.cells = make_tiles(c->ncx, c->ncy, c->ncz);
+    P.Pp = make_pois(c, 0);
+    P.Pn = make_pois(c, 1);
+    P.nblk = P.all.per_xcd * NXCD;
+    P.corner = gidx(c->d, c->ncx - 1, c->ncy - 1, c->ncz - 1);
+    return P;
+}
+
+// shift so the pinned corner holds 0
+static int shift_corner(tb_ctx *c, const PoisCtx &P, double *x) {
+    TB_CHECK(c, hipMemcpyAsync(c->scal + S_CORNER, x + P.corner, sizeof(double), hipMemcpyDeviceToDevice, c->s));
+    hipLaunchKernelGGL(k_shift, tiles_grid(P.all), tiles_block(), 0, c->s, c->d, x, (const double *)c->scal, P.all);
+    TB_LAUNCH_CHECK(c);
+    return 0;
+}
+
+// r = b - A x (pinned or Neumann), returns ||r||^2 and ||b||^2 via hscal
+static int residual(tb_ctx *c, const PoisCtx &P, bool neumann, const double *x, const double *b, double *r,
+                    float *r32, double &rr, double &bb) {
+    hipLaunchKernelGGL(k_pois_resid, tiles_grid(P.all), tiles_block(), 0, c->s, neumann ? P.Pn : P.Pp, x, b, r, r32,
+                       c->red, P.all);
+    TB_LAUNCH_CHECK(c);
+    if (tb_reduce_finish(c, (int)P.nblk, 2, S_TMP1)) return -1;
+    if (read_s(c, S_TMP1, 2)) return -1;
+    rr = c->hscal[S_TMP1];
+    bb = c->hscal[S_TMP2];
+    return 0;
+}
+
+// FP64 GMG-preconditioned CG from the current x; returns iterations
+static int pcg64(tb_ctx *c, const PoisCtx &P, double *x, const double *b, double bn, double rtol, int maxit,
+                 int &its, int &vcy, double &res) {
+    long n = c->d.ntot;
+    double *r = c->kv[0], *z = c->kv[1], *p = c->kv[2], *q = c->kv[3];
+    double rr, bb;
+    // the V-cycle writes interior cells only; keep z's half-cell planes at 0
+    TB_CHECK(c, hipMemsetAsync(z, 0, sizeof(double) * n, c->s));
+    if (shift_corner(c, P, x)) return -1;
+    if (residual(c, P, false, x, b, r, nullptr, rr, bb)) return -1;
+    res = sqrt(rr) / bn;
+    while (res > rtol && its < maxit) {
+        if (tb_vcycle64(c, r, z)) return -1;
+        vcy++;
+        TB_CHECK(c, hipMemcpyAsync(p, z, sizeof(double) * n, hipMemcpyDeviceToDevice, c->s));
+        if (tb_dot(c, r, z, S_RZ)) return -1;
+        int inner = 0;
+        while (its < maxit) {
+            hipLaunchKernelGGL(k_pois_apply, tiles_grid(P.all), tiles_block(), 0, c->s, P.Pn, (const double *)p, q,
+                               c->red, P.all);
+            TB_LAUNCH_CHECK(c);
+            if (tb_reduce_finish(c, (int)P.nblk, 1, S_PQ)) return -1;
+            hipLaunchKernelGGL(k_pcg_xr, dim3(PB), dim3(PT), 0, c->s, n, x, r, (const double *)p, (const double *)q,
+                               (const double *)c->scal, c->red);
+            TB_LAUNCH_CHECK(c);
+            if (tb_reduce_finish(c, PB, 1, S_RR)) return -1;
+            its++;
+            inner++;
+            if (read_s(c, S_RR, 1)) return -1;
+            double rrel = sqrt(c->hscal[S_RR]) / bn;
+            if (!(rrel == rrel)) { c->err = "Poisson CG produced NaN"; return -1; }
+            if (rrel <= 0.5 * rtol || inner >= 200) break;
+            if (tb_vcycle64(c, r, z)) return -1;
+            vcy++;
+            if (tb_dot(c, r, z, S_TMP0)) return -1;
+            hipLaunchKernelGGL(k_pcg_p, dim3(PB), dim3(PT), 0, c->s, n, (const double *)z, p, (const double *)c->scal);
+            hipLaunchKernelGGL(k_copy_scalar, dim3(1), dim3(64), 0, c->s, c->scal, S_RZ, S_TMP0);
+            TB_LAUNCH_CHECK(c);
+        }
+        if (shift_corner(c, P, x)) return -1;
+        if (residual(c, P, false, x, b, r, nullptr, rr, bb)) return -1;
+        res = sqrt(rr) / bn;
+    }
+    return 0;
+}
+
+// FP32 V-cycles inside FP64 iterative refinement; stops at rtol, at maxit,
+// or when a cycle reduces the residual by less than 'stall'.
+static int mixed_ir(tb_ctx *c, const PoisCtx &P, double *x, const double *b, double bn, double rtol, int maxit,
+                    int &its, int &vcy, double &res, bool &stalled) {
+    double *r = c->kv[0];
+    Level &L0 = c->lev[0];
+    double rr, bb;
+    stalled = false;
+    if (tb_mg_build(c)) return -1;
+    if (shift_corner(c, P, x)) return -1;
+    if (residual(c, P, true, x, b, r, L0.b32, rr, bb)) return -1;
+    res = sqrt(rr) / bn;
+    double prev = res;
+    int slow = 0;
+    while (res > rtol && its < maxit) {
+        if (tb_vcycle32(c)) return -1;
+        vcy++;
+        hipLaunchKernelGGL(k_add32, tiles_grid(P.cells), tiles_block(), 0, c->s, c->d, x, (const float *)L0.x32, P.cells);
+        TB_LAUNCH_CHECK(c);
+        its++;
+        if (residual(c, P, true, x, b, r, L0.b32, rr, bb)) return -1;
+        res = sqrt(rr) / bn;
+        if (!(res == res)) { c->err = "Poisson IR produced NaN"; return -1; }
+        if (res > 0.6 * prev) { if (++slow >= 2) { stalled = true; break; } }
+        else slow = 0;
+        prev = res;
+    }
+    if (shift_corner(c, P, x)) return -1;
+    if (residual(c, P, false, x, b, r, nullptr, rr, bb)) return -1;
+    res = sqrt(rr) / bn;
+    return 0;
+}
+
+extern "C" int tb_poisson_solve(tb_ctx *c, tb_solve_stats *st) {
+    if (!c) return -1;
+    if (tb_mg_build(c)) return -1;
+    PoisCtx P = pois_ctx(c);
+    if (P.nblk > RED_MAX_BLOCKS) { c->err = "grid too large for the reduction buffer"; return -1; }
+    double *x = c->f[TB_F_PHI];
+    const double *b = c->f[TB_F_PRHS];
+    double rtol = c->cfg.pois_rtol;
+    int maxit = c->cfg.pois_maxit;
+    TB_CHECK(c, hipEventRecord(c->ev0, c->s));
+    double rr, bb;
+    if (shift_corner(c, P, x)) return -1;
+    if (residual(c, P, false, x, b, c->kv[0], nullptr, rr, bb)) return -1;
+    double bn = sqrt(bb);
+    tb_solve_stats loc;
+    memset(&loc, 0, sizeof(loc));
+    int its = 0, vcy = 0;
+    double res = 0.0;
+    if (bn == 0.0) {
+        TB_CHECK(c, hipMemsetAsync(x, 0, sizeof(double) * c->d.ntot, c->s));
+        loc.converged = 1;
+    } else {
+        loc.res0 = sqrt(rr) / bn;
+        int mode = c->cfg.pois_mode;
+        if (mode == TB_POIS_LEARNED_IR && c->net && c->n_hist >= 2) {
+            // learned initial guess: keep it only if it lowers the residual
+            double *guess = c->kv[5];
+            if (tb_net_predict(c, guess) == 0) {
+                double rg, bg;
+                if (shift_corner(c, P, guess)) return -1;
+                if (residual(c, P, false, guess, b, c->kv[0], nullptr, rg, bg)) return -1;
+                double resg = sqrt(rg) / bn;
+                if (resg < loc.res0) {
+                    TB_CHECK(c, hipMemcpyAsync(x, guess, sizeof(double) * c->d.ntot, hipMemcpyDeviceToDevice, c->s));
+                    loc.res0 = resg;
+                } else {
+                    loc.fallback = 2;   // guess rejected
+                }
+            }
+        }
+        if (mode == TB_POIS_FP64_PCG) {
+            if (pcg64(c, P, x, b, bn, rtol, maxit, its, vcy, res)) return -1;
+        } else {
+            bool stalled = false;
+            if (mixed_ir(c, P, x, b, bn, rtol, maxit, its, vcy, res, stalled)) return -1;
+            if (res > rtol) {
+                loc.fallback |= 1;   // plain FP64 GMG-PCG takes over
+                if (pcg64(c, P, x, b, bn, rtol, maxit, its, vcy, res)) return -1;
+            }
+        }
+        loc.converged = res <= rtol;
+    }
+    // history of solutions for the learned guess
+    TB_CHECK(c, hipMemcpyAsync(c->phi_hist[1], c->phi_hist[0], sizeof(double) * c->d.ntot, hipMemcpyDeviceToDevice, c->s));
+    TB_CHECK(c, hipMemcpyAsync(c->phi_hist[0], x, sizeof(double) * c->d.ntot, hipMemcpyDeviceToDevice, c->s));
+    if (c->n_hist < 2) c->n_hist++;
+    TB_CHECK(c, hipEventRecord(c->ev1, c->s));
+    TB_CHECK(c, hipStreamSynchronize(c->s));
+    loc.iterations = its;
+    loc.vcycles = vcy;
+    loc.res = (bn == 0.0) ? 0.0 : res;
+    loc.ms = tb_elapsed_ms(c);
+    if (st) *st = loc;
+    return loc.converged ? 0 : 1;
+}
+
+// ---------------------------------------------------------------------------
+// Pressure_project_velocity (Pressure.c:280-439)
+__global__ void __launch_bounds__(256) k_project(Dims d, GridDev g, double *__restrict__ u, double *__restrict__ v,
+                                                 double *__restrict__ w, double *__restrict__ dpx,
+                                                 double *__restrict__ dpy, double *__restrict__ dpz,
+                                                 const double *__restrict__ phi, const double *__restrict__ b,
+                                                 const uint8_t *__restrict__ su, const uint8_t *__restrict__ sv,
+                                                 const uint8_t *__restrict__ sw, double dt, double coef,
+                                                 double a_dEta, double a_dXi, double a_dPhi, Tiles t) {
+    int i, j, k;
+    if (!tile_point(t, i, j, k)) return;
+    long id = gidx(d, i, j, k);
+    double ph = phi[id], bc = b[id];
+    if (su[id] != TB_BOUNDARY) {
+        double metric_ex = g.mxu[i];
+        double dd = (ph - phi[id - 1]) * a_dEta * metric_ex;
+        u[id] -= dt * dd;
+        double lap = -(bc - b[id - 1]) * metric_ex * a_dEta;
+        dpx[id] += dd - coef * lap;
+    }
+    if (sv[id] != TB_BOUNDARY) {
+        double C_y = a_dXi * g.myv[j];
+        double dd = (ph - phi[id - d.px]) * C_y;
+        v[id] -= dt * dd;
+        double lap = -(bc - b[id - d.px]) * C_y;
+        dpy[id] += dd - coef * lap;
+    }
+    if (sw[id] != TB_BOUNDARY) {
+        double C_z = g.mzw[k] * a_dPhi;
+        double dd = (ph - phi[id - d.plane]) * C_z;
+        w[id] -= dt * dd;
+        double lap = -(bc - b[id - d.plane]) * C_z;
+        dpz[id] += dd - coef * lap;
+    }
+}
+
+extern "C" int tb_project(tb_ctx *c, double dt) {
+    if (!c) return -1;
+    Tiles t = make_tiles(c->d.NX, c->d.NY, c->d.NZ);
+    hipLaunchKernelGGL(k_project, tiles_grid(t), tiles_block(), 0, c->s, c->d, c->g, c->f[TB_F_U], c->f[TB_F_V],
+                       c->f[TB_F_W], c->f[TB_F_DPDX], c->f[TB_F_DPDY], c->f[TB_F_DPDZ], c->f[TB_F_PHI],
+                       c->f[TB_F_PRHS], c->st[0], c->st[1], c->st[2], dt, dt / c->cfg.re, 1.0 / c->dEta,
+                       1.0 / c->dXi, 1.0 / c->dPhi, t);
+    TB_LAUNCH_CHECK(c);
+    return 0;
+}
+
+// Pressure_compute_velocity_divergence (Pressure.c:462-586): |div| per cell
+// and the maximum with the last (k, j, i)-ordered cell on ties.
+__global__ void __launch_bounds__(256) k_div(Dims d, GridDev g, const double *__restrict__ u,
+                                             const double *__restrict__ v, const double *__restrict__ w,
+                                             const uint8_t *__restrict__ sc, double *__restrict__ div,
+                                             double *pmax, long *pidx, double a_dEta, double a_dXi, double a_dPhi,
+                                             Tiles t) {
+    __shared__ double sv[256];
+    __shared__ long si[256];
+    int tid = threadIdx.y * blockDim.x + threadIdx.x;
+    double best = -1.0;
+    long bidx = -1;
+    int i, j, k;
+    if (tile_point(t, i, j, k)) {
+        long id = gidx(d, i, j, k);
+        double dc = 0.0;
+        if (sc[id] != TB_BOUNDARY) {
+            double dudx = (u[id + 1] - u[id]) * a_dEta * g.mxc[i];
+            double dvdy = (v[id + d.px] - v[id]) * a_dXi * g.myc[j];
+            double dwdz = (w[id + d.plane] - w[id]) * a_dPhi * g.mzc[k];
+            dc = fabs(dudx + dvdy + dwdz);
+        }
+        div[id] = dc;
+        best = dc;
+        bidx = ((long)k * d.NY + j) * d.NX + i;
+    }
+    sv[tid] = best;
+    si[tid] = bidx;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (tid < s) {
+            double a = sv[tid], b2 = sv[tid + s];
+            long ia = si[tid], ib = si[tid + s];
+            if (b2 > a || (b2 == a && ib > ia)) { sv[tid] = b2; si[tid] = ib; }
+        }
+        __syncthreads();
+    }
+    if (tid == 0) { pmax[blockIdx.x] = sv[0]; pidx[blockIdx.x] = si[0]; }
+}
+
+__global__ void k_div_final(const double *pmax, const long *pidx, long n, double *out, long *oidx) {
+    __shared__ double sv[1024];
+    __shared__ long si[1024];
+    double best = -1.0;
+    long bidx = -1;
+    for (long b = threadIdx.x; b < n; b += blockDim.x) {
+        double a = pmax[b];
+        long ia = pidx[b];
+        if (a > best || (a == best && ia > bidx)) { best = a; bidx = ia; }
+    }
+    sv[threadIdx.x] = best;
+    si[threadIdx.x] = bidx;
+    __syncthreads();
+    for (int s = 512; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            double a = sv[threadIdx.x], b2 = sv[threadIdx.x + s];
+            long ia = si[threadIdx.x], ib = si[threadIdx.x + s];
+            if (b2 > a || (b2 == a && ib > ia)) { sv[threadIdx.x] = b2; si[threadIdx.x] = ib; }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) { out[0] = sv[0]; oidx[0] = si[0]; }
+}
+
+extern "C" int tb_div_max(tb_ctx *c, double *div_max, int32_t *ijk) {
+    if (!c) return -1;
+    Tiles t = make_tiles(c->d.NX, c->d.NY, c->d.NZ);
+    long nb = t.per_xcd * NXCD;
+    double *pm = nullptr;
+    long *pi = nullptr;
+    TB_CHECK(c, hipMalloc((void **)&pm, sizeof(double) * (nb + 1)));
+    TB_CHECK(c, hipMalloc((void **)&pi, sizeof(long) * (nb + 1)));
+    hipLaunchKernelGGL(k_div, tiles_grid(t), tiles_block(), 0, c->s, c->d, c->g, c->f[TB_F_U], c->f[TB_F_V],
+                       c->f[TB_F_W], c->st[3], c->f[TB_F_DIV], pm, pi, 1.0 / c->dEta, 1.0 / c->dXi, 1.0 / c->dPhi, t);
+    hipLaunchKernelGGL(k_div_final, dim3(1), dim3(1024), 0, c->s, pm, pi, nb, pm + nb, pi + nb);
+    TB_LAUNCH_CHECK(c);
+    double h;
+    long hi;
+    TB_CHECK(c, hipMemcpyAsync(&h, pm + nb, sizeof(double), hipMemcpyDeviceToHost, c->s));
+    TB_CHECK(c, hipMemcpyAsync(&hi, pi + nb, sizeof(long), hipMemcpyDeviceToHost, c->s));
+    TB_CHECK(c, hipStreamSynchronize(c->s));
+    hipFree(pm);
+    hipFree(pi);
+    if (div_max) *div_max = h;
+    if (ijk) {
+        ijk[0] = (int32_t)(hi % c->d.NX);
+        ijk[1] = (int32_t)((hi / c->d.NX) % c->d.NY);
+        ijk[2] = (int32_t)(hi / ((long)c->d.NX * c->d.NY));
+    }
+    return 0;
+}

@@ -1,0 +1,52 @@
+// tb_step.hip -- the SSP-RK2 time step (the role of the reference's missing
+// driver, gvg.c): per stage, explicit terms of every field from the
+// stage-start state, then the four Helmholtz solves, then the Poisson solve
+// and the projection; after both stages the 0.5/0.5 RK average
+// (TVD_RK2, definitions.h:148-149) and the divergence check.
+#include <cstring>
+
+#include "tb_common.h"
+
+int tb_eno_momentum(tb_ctx *c);
+int tb_eno_conc(tb_ctx *c);
+
+extern "C" int tb_step(tb_ctx *c, double dt, tb_step_stats *st) {
+    if (!c || !(dt > 0.0)) { if (c) c->err = "tb_step: bad argument"; return -1; }
+    tb_step_stats loc;
+    memset(&loc, 0, sizeof(loc));
+    hipEvent_t e0, e1;
+    TB_CHECK(c, hipEventCreate(&e0));
+    TB_CHECK(c, hipEventCreate(&e1));
+    TB_CHECK(c, hipEventRecord(e0, c->s));
+    int worst = 0;
+    if (tb_store_old(c)) return -1;
+    for (int stage = 0; stage < 2; stage++) {
+        if (tb_eno_momentum(c)) return -1;
+        if (tb_eno_conc(c)) return -1;
+        for (int q = 0; q < 4; q++)
+            if (tb_set_rhs(c, q, dt)) return -1;
+        for (int q = 0; q < 4; q++) {
+            int rc = tb_helmholtz_solve(c, q, dt, &loc.helm[stage][q]);
+            if (rc < 0) return -1;
+            if (rc > worst) worst = rc;
+        }
+        if (tb_poisson_rhs(c, dt)) return -1;
+        int rc = tb_poisson_solve(c, &loc.pois[stage]);
+        if (rc < 0) return -1;
+        if (rc > worst) worst = rc;
+        if (tb_project(c, dt)) return -1;
+    }
+    if (tb_rk_average(c, 0.5, 0.5)) return -1;
+    double dm = 0.0;
+    if (tb_div_max(c, &dm, nullptr)) return -1;
+    TB_CHECK(c, hipEventRecord(e1, c->s));
+    TB_CHECK(c, hipEventSynchronize(e1));
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    loc.div_max = dm;
+    loc.ms = ms;
+    if (st) *st = loc;
+    return worst;
+}
