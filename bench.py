@@ -1,0 +1,218 @@
+#!/usr/bin/env python3
+"""Timesteps/s of the TURBINS hot path on one MI355X.
+
+Workload: lock-exchange gravity current over an immersed Gaussian bump on a
+512 x 256 x 128 MAC grid (16 x 1 x 1 domain, Re = 2000, Sc = 1, 2nd-order
+ENO, dt = 1e-3), every Poisson solve to a 1e-8 relative residual, every
+Helmholtz solve to 1e-12.  Two device paths are timed on the same state:
+  plain   -- FP64 GMG-preconditioned CG
+  headline -- learned initial guess + FP32 V-cycles inside FP64 iterative
+             refinement (falls back to FP64 GMG-PCG if it stalls)
+The JSON line's `value` is the headline path's timesteps/s.  Inputs are
+resident in HBM before the timed region; each library step ends with a
+device synchronisation, so the wall clock brackets exactly K steps.
+
+`roofline` reports the instrumented kernel with the largest total time in
+the headline region: algorithmic bytes per launch / average launch time
+from HIP events on the library's stream.  `cpu_baseline` times the FP64
+oracle (the reference's algorithm: unpreconditioned BiCGStab, GMG-PCG
+Poisson) for one step of the same workload on the host cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "timesteps/s at 512x256x128, 1e-8 Poisson residual: plain vs learned+mixed"
+HBM_PEAK_GBS = 8000.0
+
+# algorithmic bytes per point (each array read or written once per point at
+# its storage width); see DESIGN.md for the table
+BYTES_PER_POINT = {
+    "ENO_MOM": 51, "ENO_CONC": 41, "PROJECT": 115, "BICG_UPD": 56, "PCG_UPD": 48, "POIS_OP": 16,
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def run_path(g, c, steps, warmup, mode, rtol, profile):
+    from turbins_amd import abi
+    g.set_poisson_mode(mode, rtol)
+    for _ in range(warmup):
+        g.step(c["dt"])
+    if profile:
+        g.profile(True)
+    stats = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        stats.append(g.step(c["dt"]))
+    t1 = time.perf_counter()
+    prof = {}
+    if profile:
+        for name, kid in abi.K.items():
+            ms, n = g.profile_read(kid)
+            if n:
+                prof[name] = (ms, n)
+        g.profile(False)
+    return (t1 - t0), stats, prof
+
+
+def roofline(prof, stats, c):
+    NT = (c["nx"] + 1) * (c["ny"] + 1) * (c["nz"] + 1)
+    ncell = c["nx"] * c["ny"] * c["nz"]
+    helm_its = sum(h.iterations for st in stats for stage in st.helm for h in stage)
+    best = None
+    for name, (ms, n) in prof.items():
+        if name in BYTES_PER_POINT:
+            total = BYTES_PER_POINT[name] * NT * n
+        elif name == "HELM_OP":
+            # 17 B/point (x, status, y) per launch; the rhat.v launch of each
+            # BiCGStab iteration also reads rhat (8 B/point)
+            total = 17 * NT * n + 8 * NT * helm_its
+        elif name == "RBGS0":
+            total = 16 * ncell * n      # x read, half of b read, half of x written
+        elif name == "RBGS0_F32":
+            total = 8 * ncell * n
+        else:
+            continue
+        if best is None or ms > best[1]:
+            best = (name, ms, n, total)
+    if best is None:
+        return None
+    name, ms, n, total = best
+    per_launch = total / n
+    avg_s = ms / n * 1e-3
+    achieved = per_launch / avg_s / 1e9
+    return {"kernel": name, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "bytes_per_launch": int(per_launch), "avg_launch_us": round(avg_s * 1e6, 2), "launches": n}
+
+
+def cpu_baseline(c, g, threads):
+    """One SSP-RK2 step of the same workload on the FP64 oracle, with the
+    device's geometry installed (the setup chain is not part of a step)."""
+    import oracle
+    from turbins_amd import workload as W
+    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+    cfg = W.config_for(c, pois_rtol=1e-8)
+    o = oracle.Oracle(cfg)
+    for q in range(4):
+        o.set_geometry(q, g.flags(q), g.ib_table(q))
+    for name in ["U", "V", "W", "C", "DPDX", "DPDY", "DPDZ", "PHI"]:
+        o.upload(name, g.download(name))
+    x = np.zeros(o.NT)
+    t = time.perf_counter()
+    for q in range(4):                  # assemble the matrices outside the timed step
+        o.helmholtz_apply(q, c["dt"], x)
+    o.poisson_apply(x)
+    o.gmg_levels()
+    log(f"cpu baseline: assembly {time.perf_counter() - t:.1f}s")
+    t0 = time.perf_counter()
+    st = o.step(c["dt"])
+    dt_s = time.perf_counter() - t0
+    o.close()
+    return 1.0 / dt_s, dt_s, st
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="bump512x256")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
+              "decomposition is a design (DESIGN.md), not a measured path", file=sys.stderr)
+        sys.exit(2)
+
+    from turbins_amd import Solver, abi, workload as W
+    c = W.case(args.workload)
+    cfg = W.config_for(c, pois_rtol=1e-8, helm_rtol=1e-12)
+    g = Solver(cfg)
+    t = time.perf_counter()
+    g.setup_geometry(W.topography(c))
+    log(f"geometry setup {time.perf_counter() - t:.2f}s, reinit iterations {g.reinit_iterations()}")
+    flags = [g.flags(q) for q in range(4)]
+    for k, v in W.initial_fields(c, flags).items():
+        g.upload(k, v)
+
+    t_plain, st_plain, _ = run_path(g, c, args.steps, args.warmup, abi.POIS_FP64_PCG, 1e-8, False)
+    plain_sps = args.steps / t_plain
+    learned = False
+    mode = abi.POIS_MIXED_IR
+    t_head, st_head, prof = run_path(g, c, args.steps, args.warmup, mode, 1e-8, True)
+    head_sps = args.steps / t_head
+
+    def pois_summary(stats):
+        its = [p.iterations for st in stats for p in st.pois]
+        res = [p.res for st in stats for p in st.pois]
+        fb = sum(1 for st in stats for p in st.pois if p.fallback & 1)
+        return {"iterations_mean": round(float(np.mean(its)), 2), "max_rel_residual": float(np.max(res)),
+                "fallbacks": fb}
+
+    helm_its = [h.iterations for st in st_head for stage in st.helm for h in stage]
+    log(f"plain: {plain_sps:.3f} steps/s ({1e3 / plain_sps:.1f} ms/step)  headline: {head_sps:.3f} steps/s "
+        f"({1e3 / head_sps:.1f} ms/step)  ratio {head_sps / plain_sps:.3f}")
+    log("poisson plain", pois_summary(st_plain), "headline", pois_summary(st_head))
+    for name, (ms, n) in sorted(prof.items(), key=lambda kv: -kv[1][0]):
+        log(f"  {name:10s} {n:6d} launches  {ms:9.2f} ms  {1e3 * ms / n:8.1f} us/launch")
+
+    rl = roofline(prof, st_head, c)
+    result = {
+        "metric": METRIC,
+        "value": round(head_sps, 4),
+        "unit": "timesteps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 / head_sps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded lock-exchange initial state, Gaussian bump topography)",
+        "config": {
+            "workload": f"{args.workload}: lock exchange over an immersed Gaussian bump, "
+                        f"{c['nx']}x{c['ny']}x{c['nz']} cells, domain {c['lx']}x{c['ly']}x{c['lz']}, "
+                        f"Re {c['re']}, ENO order 2, dt {c['dt']}",
+            "cells": c["nx"] * c["ny"] * c["nz"],
+            "cell_updates_per_s": round(head_sps * c["nx"] * c["ny"] * c["nz"], 1),
+            "poisson_rtol": 1e-8,
+            "helmholtz_rtol": 1e-12,
+            "headline_path": ("learned guess + " if learned else "") + "FP32 V-cycles in FP64 iterative refinement",
+            "learned_guess_loaded": learned,
+            "plain_path": "FP64 GMG-PCG",
+            "plain_timesteps_per_s": round(plain_sps, 4),
+            "plain_ms_per_step": round(1e3 / plain_sps, 3),
+            "headline_over_plain": round(head_sps / plain_sps, 4),
+            "poisson_headline": pois_summary(st_head),
+            "poisson_plain": pois_summary(st_plain),
+            "helmholtz_iterations_mean": round(float(np.mean(helm_its)), 2),
+            "precision": "FP64 fields/ENO/Helmholtz/outer Poisson; FP32 multigrid in the headline path",
+        },
+        "roofline": rl,
+    }
+    if not args.no_cpu:
+        sps, secs, st = cpu_baseline(c, g, args.cpu_threads)
+        log(f"cpu baseline: {secs:.1f}s per step")
+        result["cpu_baseline"] = {
+            "value": round(sps, 6), "unit": "timesteps/s", "cores": args.cpu_threads, "kind": "port",
+            "sample": f"1 SSP-RK2 step of {args.workload} ({c['nx']}x{c['ny']}x{c['nz']}) on the FP64 C oracle "
+                      "(unpreconditioned BiCGStab Helmholtz, GMG-PCG Poisson, OpenMP), device geometry installed",
+        }
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -175,6 +175,21 @@ int tb_rk_average(tb_ctx *ctx, double old_frac, double new_frac);
 /* composite SSP-RK2 step (the missing driver's role) */
 int tb_step(tb_ctx *ctx, double dt, tb_step_stats *st);
 
+/* solver controls (no reference counterpart: the reference fixes these in
+ * Solver.c / Input.c at build time) */
+int tb_set_poisson_mode(tb_ctx *ctx, int mode, double rtol);
+int tb_set_helmholtz_rtol(tb_ctx *ctx, double rtol);
+
+/* live per-kernel timing with HIP events on the context's stream (no
+ * reference counterpart; replaces PETSc's -log_summary for this path) */
+enum {
+    TB_K_ENO_MOM = 0, TB_K_ENO_CONC, TB_K_HELM_OP, TB_K_BICG_UPD, TB_K_POIS_OP,
+    TB_K_RBGS0, TB_K_RBGS0_F32, TB_K_PCG_UPD, TB_K_PROJECT, TB_K_RHS, TB_K_NET,
+    TB_K_COUNT
+};
+int tb_profile_enable(tb_ctx *ctx, int on);
+int tb_profile_read(tb_ctx *ctx, int kernel, double *total_ms, int64_t *calls);
+
 #ifdef __cplusplus
 }
 #endif

@@ -57,6 +57,9 @@ def lib():
         L.ora_rk_average.argtypes = [P, C.c_double, C.c_double]
         L.ora_step.argtypes = [P, C.c_double, C.POINTER(abi.StepStats)]
         L.ora_gmg_levels.argtypes = [P, P, C.c_int]
+        L.ora_helmholtz_matrix.argtypes = [P, C.c_int, C.c_double, P, P, P]
+        L.ora_poisson_matrix.argtypes = [P, P, P, P]
+        L.ora_set_geometry.argtypes = [P, C.c_int, P, P, C.c_int]
         _LIB = L
     return _LIB
 
@@ -106,6 +109,13 @@ class Oracle:
 
     def reinit_iterations(self):
         return self.L.ora_reinit_iterations(self.h)
+
+    def set_geometry(self, q, flags, ib_nodes):
+        """Install flags + IB table computed elsewhere (CPU-baseline timing)."""
+        fl = np.ascontiguousarray(flags, dtype=np.uint8)
+        n = len(ib_nodes)
+        arr = (abi.IBNode * max(n, 1))(*ib_nodes)
+        return self.L.ora_set_geometry(self.h, q, _ptr(fl), C.cast(arr, C.c_void_p), n)
 
     def flags(self, q):
         out = np.zeros(self.NT, dtype=np.uint8)
@@ -177,6 +187,29 @@ class Oracle:
         st = abi.StepStats()
         self.L.ora_step(self.h, dt, C.byref(st))
         return st
+
+    def _csr(self, cnt, col, val):
+        import scipy.sparse as sp
+        n = self.NT
+        mask = np.arange(9)[None, :] < cnt[:, None]
+        rows = np.repeat(np.arange(n), 9).reshape(n, 9)[mask]
+        return sp.csr_matrix((val.reshape(n, 9)[mask], (rows, col.reshape(n, 9)[mask])), shape=(n, n))
+
+    def helmholtz_matrix(self, q, dt):
+        n = self.NT
+        cnt = np.zeros(n, dtype=np.int32)
+        col = np.zeros(n * 9, dtype=np.int64)
+        val = np.zeros(n * 9)
+        self.L.ora_helmholtz_matrix(self.h, q, dt, _ptr(cnt), _ptr(col), _ptr(val))
+        return self._csr(cnt, col, val)
+
+    def poisson_matrix(self):
+        n = self.NT
+        cnt = np.zeros(n, dtype=np.int32)
+        col = np.zeros(n * 9, dtype=np.int64)
+        val = np.zeros(n * 9)
+        self.L.ora_poisson_matrix(self.h, _ptr(cnt), _ptr(col), _ptr(val))
+        return self._csr(cnt, col, val)
 
     def gmg_levels(self):
         dims = np.zeros(48, dtype=np.int32)

@@ -11,6 +11,7 @@
 #include "ora_internal.h"
 
 static double *dalloc(size_t n) { return (double *)calloc(n, sizeof(double)); }
+static void build_pois(ora_ctx *o);
 
 ora_ctx *ora_create(const tb_config *cfg) {
     if (!cfg || cfg->nx < 2 || cfg->ny < 2 || cfg->nz < 2) return NULL;
@@ -473,6 +474,25 @@ static void build_helm(ora_ctx *o, int q, double dt) {
 
 static void ensure_helm(ora_ctx *o, int q, double dt) {
     if (o->helm_dt[q] != dt || o->helm[q].n != o->NT) build_helm(o, q, dt);
+}
+
+/* Copy out the assembled rows (9 slots per row, columns sorted), e.g. for a
+ * scipy direct solve in the tests. */
+int ora_helmholtz_matrix(ora_ctx *o, int q, double dt, int32_t *cnt, int64_t *col, double *val) {
+    if (q < 0 || q > 3) return -1;
+    ensure_helm(o, q, dt);
+    memcpy(cnt, o->helm[q].cnt, sizeof(int32_t) * o->NT);
+    memcpy(col, o->helm[q].col, sizeof(int64_t) * o->NT * 9);
+    memcpy(val, o->helm[q].val, sizeof(double) * o->NT * 9);
+    return 0;
+}
+
+int ora_poisson_matrix(ora_ctx *o, int32_t *cnt, int64_t *col, double *val) {
+    if (!o->pois_built) build_pois(o);
+    memcpy(cnt, o->pois.cnt, sizeof(int32_t) * o->NT);
+    memcpy(col, o->pois.col, sizeof(int64_t) * o->NT * 9);
+    memcpy(val, o->pois.val, sizeof(double) * o->NT * 9);
+    return 0;
 }
 
 int ora_helmholtz_apply(ora_ctx *o, int q, double dt, const double *x, double *y) {

@@ -708,6 +708,20 @@ int ora_setup_geometry(ora_ctx *o, const double *h_xz) {
     return 0;
 }
 
+/* Install flags and an IB table computed elsewhere (for timing the per-step
+ * path at sizes where the setup chain would dominate the CPU sample). */
+int ora_set_geometry(ora_ctx *o, int q, const uint8_t *flags, const tb_ib_node *ib, int n) {
+    if (q < 0 || q > 3 || n < 0) return -1;
+    memcpy(o->st[q], flags, o->NT);
+    free(o->ib[q]);
+    o->ib[q] = (tb_ib_node *)calloc((size_t)(n > 0 ? n : 1), sizeof(tb_ib_node));
+    if (n > 0) memcpy(o->ib[q], ib, sizeof(tb_ib_node) * (size_t)n);
+    o->nib[q] = n;
+    o->helm_dt[q] = 0.0;
+    o->has_geometry = 1;
+    return 0;
+}
+
 int ora_reinit_iterations(ora_ctx *o) { return o->reinit_its; }
 
 int ora_get_flags(ora_ctx *o, int q, uint8_t *out) {

@@ -32,6 +32,7 @@ int ora_get_grid_array(ora_ctx *o, int which, double *out);
 
 int ora_setup_geometry(ora_ctx *o, const double *h_xz);
 int ora_reinit_iterations(ora_ctx *o);
+int ora_set_geometry(ora_ctx *o, int q, const uint8_t *flags, const tb_ib_node *ib, int n);
 int ora_get_flags(ora_ctx *o, int q, uint8_t *out);
 int ora_ib_count(ora_ctx *o, int q);
 int ora_get_ib(ora_ctx *o, int q, tb_ib_node *out, int cap);
@@ -48,6 +49,8 @@ void ora_gauss_jordan(int n, const double *a, double *a_inv);
 int ora_eno_conv(ora_ctx *o, int q);
 int ora_set_rhs(ora_ctx *o, int q, double dt);
 int ora_helmholtz_apply(ora_ctx *o, int q, double dt, const double *x, double *y);
+int ora_helmholtz_matrix(ora_ctx *o, int q, double dt, int32_t *cnt, int64_t *col, double *val);
+int ora_poisson_matrix(ora_ctx *o, int32_t *cnt, int64_t *col, double *val);
 int ora_helmholtz_solve(ora_ctx *o, int q, double dt, double rtol, int maxit,
                         tb_solve_stats *st);
 int ora_poisson_rhs(ora_ctx *o, double dt);

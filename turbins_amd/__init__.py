@@ -60,6 +60,10 @@ def lib():
     L.tb_store_old.argtypes = [P]
     L.tb_rk_average.argtypes = [P, C.c_double, C.c_double]
     L.tb_step.argtypes = [P, C.c_double, C.POINTER(abi.StepStats)]
+    L.tb_set_poisson_mode.argtypes = [P, C.c_int, C.c_double]
+    L.tb_set_helmholtz_rtol.argtypes = [P, C.c_double]
+    L.tb_profile_enable.argtypes = [P, C.c_int]
+    L.tb_profile_read.argtypes = [P, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
     _LIB = L
     return L
 
@@ -196,3 +200,17 @@ class Solver:
         st = abi.StepStats()
         self._ok(self.L.tb_step(self.h, dt, C.byref(st)), "tb_step")
         return st
+
+    def set_poisson_mode(self, mode, rtol):
+        return self._ok(self.L.tb_set_poisson_mode(self.h, mode, rtol), "tb_set_poisson_mode")
+
+    def set_helmholtz_rtol(self, rtol):
+        return self._ok(self.L.tb_set_helmholtz_rtol(self.h, rtol), "tb_set_helmholtz_rtol")
+
+    def profile(self, on=True):
+        return self._ok(self.L.tb_profile_enable(self.h, 1 if on else 0), "tb_profile_enable")
+
+    def profile_read(self, kernel):
+        ms, n = C.c_double(), C.c_int64()
+        self._ok(self.L.tb_profile_read(self.h, kernel, C.byref(ms), C.byref(n)), "tb_profile_read")
+        return ms.value, n.value

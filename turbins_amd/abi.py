@@ -6,6 +6,10 @@ OUTSIDE, SOLID, FLUID, IMMERSED, BOUNDARY, BUFFER_SOLID = 1, 2, 3, 5, 7, 11
 SALINITY, PARTICLE = 1, 2
 POIS_FP64_PCG, POIS_MIXED_IR, POIS_LEARNED_IR = 0, 1, 2
 
+KERNELS = ["ENO_MOM", "ENO_CONC", "HELM_OP", "BICG_UPD", "POIS_OP", "RBGS0", "RBGS0_F32",
+           "PCG_UPD", "PROJECT", "RHS", "NET"]
+K = {name: i for i, name in enumerate(KERNELS)}
+
 FIELDS = [
     "U", "V", "W", "C", "DPDX", "DPDY", "DPDZ", "PHI",
     "CONV_U", "CONV_V", "CONV_W", "CONV_C",

@@ -75,6 +75,50 @@ __device__ inline bool tile_point(const Tiles &t, int &i, int &j, int &k) {
     return i < t.nx && j < t.ny;
 }
 
+// Column tiles for k-marching stencils: a 64 x 4 block owns a tile of
+// columns and walks KCHUNK planes upwards, keeping the k-1, k, k+1 values of
+// its column in registers, so each plane is read from HBM once.
+constexpr int KCHUNK = 16;
+
+struct KTiles {
+    int nx, ny, nz;
+    int tx, ty, kc;     // tiles per axis, k chunks
+    long n, per_xcd;
+};
+
+inline KTiles make_ktiles(int nx, int ny, int nz) {
+    KTiles t;
+    t.nx = nx; t.ny = ny; t.nz = nz;
+    t.tx = (nx + TILE_X - 1) / TILE_X;
+    t.ty = (ny + TILE_Y - 1) / TILE_Y;
+    t.kc = (nz + KCHUNK - 1) / KCHUNK;
+    t.n = (long)t.tx * t.ty * t.kc;
+    t.per_xcd = (t.n + NXCD - 1) / NXCD;
+    return t;
+}
+
+inline dim3 ktiles_grid(const KTiles &t) { return dim3((unsigned)(t.per_xcd * NXCD)); }
+
+// Column (i, j) and plane range [k0, k1) of this thread.  Threads with
+// i >= nx or j >= ny, and whole blocks remapped past the last tile, get
+// active = false (they must still join block reductions).
+__device__ inline void ktile_column(const KTiles &t, int &i, int &j, int &k0, int &k1, bool &active) {
+    int b = blockIdx.x;
+    int per = (int)t.per_xcd;
+    int lt = (b % NXCD) * per + b / NXCD;
+    if (lt >= t.n) { i = j = k0 = k1 = 0; active = false; return; }
+    int tx = lt % t.tx;
+    int r = lt / t.tx;
+    int ty = r % t.ty;
+    int kc = r / t.ty;
+    i = tx * TILE_X + threadIdx.x;
+    j = ty * TILE_Y + threadIdx.y;
+    k0 = kc * KCHUNK;
+    k1 = k0 + KCHUNK < t.nz ? k0 + KCHUNK : t.nz;
+    active = i < t.nx && j < t.ny;
+    return;
+}
+
 // ---------------------------------------------------------------------------
 // Reductions: each block writes partial sums; a one-block kernel adds the
 // partials in a fixed order, so results do not depend on scheduling.
@@ -171,10 +215,26 @@ struct tb_ctx {
     int n_hist = 0;
     NetState *net = nullptr;
 
-    // timing of named regions
-    double t_accum[32] = {};
-    int t_count[32] = {};
+    // live per-kernel timing (event pairs around selected launches)
+    bool prof_on = false;
+    std::vector<hipEvent_t> prof_ev;
+    std::vector<int> prof_kid;
+    int prof_n = 0;
+    double prof_ms[TB_K_COUNT] = {};
+    int64_t prof_calls[TB_K_COUNT] = {};
 };
+
+void tb_prof_begin(tb_ctx *c, int kid);
+void tb_prof_end(tb_ctx *c, int kid);
+int tb_prof_flush(tb_ctx *c);
+
+// wraps one launch statement with a timing event pair when profiling is on
+#define TB_PROF(ctx, kid, stmt)                              \
+    do {                                                     \
+        if ((ctx)->prof_on) tb_prof_begin((ctx), (kid));     \
+        stmt;                                                \
+        if ((ctx)->prof_on) tb_prof_end((ctx), (kid));       \
+    } while (0)
 
 // device scalar slots (ctx->scal)
 enum {

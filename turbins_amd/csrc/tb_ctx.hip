@@ -143,6 +143,7 @@ extern "C" void tb_destroy(tb_ctx *c) {
     for (int q = 0; q < 4; q++) { hipFree(c->old[q]); hipFree(c->st[q]); free_ib(c, q); }
     for (int a = 0; a < 8; a++) hipFree(c->kv[a]);
     for (int a = 0; a < 2; a++) hipFree(c->phi_hist[a]);
+    for (auto &e : c->prof_ev) hipEventDestroy(e);
     hipFree(c->red);
     hipFree(c->scal);
     if (c->hscal) hipHostFree(c->hscal);
@@ -206,6 +207,73 @@ extern "C" int tb_download_ib_table(tb_ctx *c, int q, tb_ib_node *out, int capac
 }
 
 extern "C" int tb_reinit_iterations(tb_ctx *c) { return c ? c->reinit_its : -1; }
+
+// ---------------------------------------------------------------------------
+// live per-kernel timing: a pool of event pairs recorded around selected
+// launches on the context's stream, resolved when the pool fills or on read.
+constexpr int PROF_CAP = 2048;
+
+void tb_prof_begin(tb_ctx *c, int kid) {
+    if (c->prof_ev.empty()) {
+        c->prof_ev.resize(2 * PROF_CAP);
+        c->prof_kid.resize(PROF_CAP);
+        for (auto &e : c->prof_ev) hipEventCreate(&e);
+    }
+    if (c->prof_n >= PROF_CAP) tb_prof_flush(c);
+    hipEventRecord(c->prof_ev[2 * c->prof_n], c->s);
+    c->prof_kid[c->prof_n] = kid;
+}
+
+void tb_prof_end(tb_ctx *c, int kid) {
+    (void)kid;
+    hipEventRecord(c->prof_ev[2 * c->prof_n + 1], c->s);
+    c->prof_n++;
+}
+
+int tb_prof_flush(tb_ctx *c) {
+    if (c->prof_n == 0) return 0;
+    TB_CHECK(c, hipEventSynchronize(c->prof_ev[2 * (c->prof_n - 1) + 1]));
+    for (int a = 0; a < c->prof_n; a++) {
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, c->prof_ev[2 * a], c->prof_ev[2 * a + 1]);
+        int k = c->prof_kid[a];
+        if (k >= 0 && k < TB_K_COUNT) { c->prof_ms[k] += ms; c->prof_calls[k]++; }
+    }
+    c->prof_n = 0;
+    return 0;
+}
+
+extern "C" int tb_profile_enable(tb_ctx *c, int on) {
+    if (!c) return -1;
+    if (tb_prof_flush(c)) return -1;
+    c->prof_on = on != 0;
+    for (int k = 0; k < TB_K_COUNT; k++) { c->prof_ms[k] = 0.0; c->prof_calls[k] = 0; }
+    return 0;
+}
+
+extern "C" int tb_profile_read(tb_ctx *c, int kernel, double *total_ms, int64_t *calls) {
+    if (!c || kernel < 0 || kernel >= TB_K_COUNT) return -1;
+    if (tb_prof_flush(c)) return -1;
+    if (total_ms) *total_ms = c->prof_ms[kernel];
+    if (calls) *calls = c->prof_calls[kernel];
+    return 0;
+}
+
+extern "C" int tb_set_poisson_mode(tb_ctx *c, int mode, double rtol) {
+    if (!c || mode < TB_POIS_FP64_PCG || mode > TB_POIS_LEARNED_IR || !(rtol > 0.0)) {
+        if (c) c->err = "tb_set_poisson_mode: bad argument";
+        return -1;
+    }
+    c->cfg.pois_mode = mode;
+    c->cfg.pois_rtol = rtol;
+    return 0;
+}
+
+extern "C" int tb_set_helmholtz_rtol(tb_ctx *c, double rtol) {
+    if (!c || !(rtol > 0.0)) { if (c) c->err = "tb_set_helmholtz_rtol: bad argument"; return -1; }
+    c->cfg.helm_rtol = rtol;
+    return 0;
+}
 
 // ---------------------------------------------------------------------------
 // reductions

@@ -54,16 +54,25 @@ struct LineSpec {
 template <int R>
 __device__ inline void load_window(const LineSpec &L, const double *__restrict__ f, long base, int m0,
                                    double (&w)[7], double (&p)[7]) {
+    // every load first, at axis positions clamped into the interior span, so
+    // the loads are issued together; a position left of lo therefore holds
+    // the first interior value D0[order], one right of hi the last one
 #pragma unroll
     for (int o = -R; o <= R; o++) {
         int m = m0 + o;
-        double wv, pv;
-        if (m >= L.lo && m <= L.hi) {
-            wv = f[base + (long)o * L.stride];
-            pv = L.pos[m];
-        } else if (m < L.lo) {
+        int mc = m < L.lo ? L.lo : (m > L.hi ? L.hi : m);
+        w[3 + o] = f[base + (long)(mc - m0) * L.stride];
+        p[3 + o] = L.pos[mc];
+    }
+    if (m0 - R >= L.lo && m0 + R <= L.hi) return;
+    // wall ghosts where the window leaves the box
+#pragma unroll
+    for (int o = -R; o <= R; o++) {
+        int m = m0 + o;
+        if (m < L.lo) {
             int ex = R + m - L.lo;                  // ghost_index
-            double first = f[base + (long)(L.lo - m0) * L.stride];
+            double first = w[3 + o];
+            double wv, pv;
             if (L.kind == 0) {
                 if (ex == R - 1) { pv = L.wall_lo; wv = 0.0; }
                 else { pv = 2.0 * L.wall_lo - L.pos[L.lo + R - ex - 2]; wv = L.ux_quirk ? -first : 0.0; }
@@ -71,11 +80,14 @@ __device__ inline void load_window(const LineSpec &L, const double *__restrict__
                 pv = 2.0 * L.wall_lo - L.pos[L.lo + R - ex - 1];
                 wv = (L.sgn_lo < 0.0) ? -first : +first;
             }
-        } else {
+            w[3 + o] = wv;
+            p[3 + o] = pv;
+        } else if (m > L.hi) {
             int n = L.hi - L.lo + 1;
             int b = R + m - L.lo;
             int ex = n + 2 * R - 1 - b;
-            double last = f[base + (long)(L.hi - m0) * L.stride];
+            double last = w[3 + o];
+            double wv, pv;
             if (L.kind == 0) {
                 if (ex == R - 1) { pv = L.wall_hi; wv = 0.0; }
                 else { pv = 2.0 * L.wall_hi - L.pos[L.hi + ex + 2 - R]; wv = 0.0; }
@@ -84,9 +96,9 @@ __device__ inline void load_window(const LineSpec &L, const double *__restrict__
                 if (L.hi_mode) wv = L.hi_scale * last;
                 else wv = (L.sgn_hi < 0.0) ? -last : +last;
             }
+            w[3 + o] = wv;
+            p[3 + o] = pv;
         }
-        w[3 + o] = wv;
-        p[3 + o] = pv;
     }
 }
 
@@ -292,11 +304,13 @@ int tb_eno_momentum(tb_ctx *c) {
     a.cu = c->f[TB_F_CONV_U]; a.cv = c->f[TB_F_CONV_V]; a.cw = c->f[TB_F_CONV_W];
     for (int q = 0; q < 3; q++) a.L[q] = make_lines(c, q);
     Tiles t = make_tiles(c->d.NX, c->d.NY, c->d.NZ);
+    if (c->prof_on) tb_prof_begin(c, TB_K_ENO_MOM);
     switch (c->cfg.eno_order) {
     case 1: hipLaunchKernelGGL(k_eno_momentum<1>, tiles_grid(t), tiles_block(), 0, c->s, a, t); break;
     case 2: hipLaunchKernelGGL(k_eno_momentum<2>, tiles_grid(t), tiles_block(), 0, c->s, a, t); break;
     default: hipLaunchKernelGGL(k_eno_momentum<3>, tiles_grid(t), tiles_block(), 0, c->s, a, t); break;
     }
+    if (c->prof_on) tb_prof_end(c, TB_K_ENO_MOM);
     TB_LAUNCH_CHECK(c);
     return 0;
 }
@@ -311,11 +325,13 @@ int tb_eno_conc(tb_ctx *c) {
     a.particle = (c->cfg.conc_type == TB_PARTICLE);
     a.L = make_lines(c, TB_Q_C);
     Tiles t = make_tiles(c->d.NX, c->d.NY, c->d.NZ);
+    if (c->prof_on) tb_prof_begin(c, TB_K_ENO_CONC);
     switch (c->cfg.eno_order) {
     case 1: hipLaunchKernelGGL(k_eno_conc<1>, tiles_grid(t), tiles_block(), 0, c->s, a, t); break;
     case 2: hipLaunchKernelGGL(k_eno_conc<2>, tiles_grid(t), tiles_block(), 0, c->s, a, t); break;
     default: hipLaunchKernelGGL(k_eno_conc<3>, tiles_grid(t), tiles_block(), 0, c->s, a, t); break;
     }
+    if (c->prof_on) tb_prof_end(c, TB_K_ENO_CONC);
     TB_LAUNCH_CHECK(c);
     return 0;
 }

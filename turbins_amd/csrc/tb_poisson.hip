@@ -55,67 +55,81 @@ struct PoisOp {
     int neumann;
 };
 
-__device__ inline double pois_row(const PoisOp &P, int i, int j, int k, long id, const double *__restrict__ x) {
-    const Dims &d = P.d;
-    const GridDev &g = P.g;
-    int NI = d.NX - 1, NJ = d.NY - 1, NK = d.NZ - 1;
-    if (i == NI || j == NJ || k == NK) return P.ap_solid * x[id];   // BOUNDARY half-cell rows
-    const double Cx = P.Cx, Cy = P.Cy, Cz = P.Cz;
-    double mpx = g.mxc[i], mpy = g.myc[j], mpz = g.mzc[k];
-    double mex = g.mxu[i + 1], mwx = g.mxu[i], mny = g.myv[j + 1], msy = g.myv[j];
-    double mfz = g.mzw[k + 1], mbz = g.mzw[k];
-    double ap = -Cx * mpx * (mex + mwx) - Cy * mpy * (mny + msy) - Cz * mpz * (mbz + mfz);
-    double aw = Cx * mpx * mwx, ae = Cx * mpx * mex, an = Cy * mpy * mny;
-    double as = Cy * mpy * msy, ab = Cz * mpz * mbz, af = Cz * mpz * mfz;
-    bool hb = k > 0, hs = j > 0, hw = i > 0, he = i < NI - 1, hn = j < NJ - 1, hf = k < NK - 1;
-    if (!hb) ap += ab;
-    if (!hs) ap += as;
-    if (!hw) ap += aw;
-    if (!he && (P.neumann || !((i == NI - 1) && (j == NJ - 1) && (k == NK - 1)))) ap += ae;
-    if (!hn) ap += an;
-    if (!hf) ap += af;
-    double r = 0.0;
-    if (hb) r += ab * x[id - d.plane];
-    if (hs) r += as * x[id - d.px];
-    if (hw) r += aw * x[id - 1];
-    r += ap * x[id];
-    if (he) r += ae * x[id + 1];
-    if (hn) r += an * x[id + d.px];
-    if (hf) r += af * x[id + d.plane];
-    return r;
-}
-
-// y = A x, partial sum of x.y
-__global__ void __launch_bounds__(256) k_pois_apply(PoisOp P, const double *__restrict__ x, double *__restrict__ y,
-                                                    double *part, Tiles t) {
-    __shared__ double sh[16];
-    double acc[1] = {0.0};
-    int i, j, k;
-    if (tile_point(t, i, j, k)) {
-        long id = gidx(P.d, i, j, k);
-        double v = pois_row(P, i, j, k, id, x);
-        y[id] = v;
-        acc[0] = x[id] * v;
-    }
-    block_sum<1>(acc, sh);
-    if (threadIdx.x == 0 && threadIdx.y == 0) part[blockIdx.x] = acc[0];
-}
-
-// r = b - A x ; partials r.r and b.b ; optionally r32 = (float) r
-__global__ void __launch_bounds__(256) k_pois_resid(PoisOp P, const double *__restrict__ x,
-                                                    const double *__restrict__ b, double *__restrict__ r,
-                                                    float *__restrict__ r32, double *part, Tiles t) {
+// k-marching application of the operator.  RESID = 0: y = A x with partial
+// x.y ; RESID = 1: r = b - A x (and r32 = (float) r if given) with partials
+// r.r and b.b.  x and y parts of the coefficients are formed once per column,
+// z parts once per plane; k-1 / k / k+1 stay in registers.
+template <int RESID>
+__global__ void __launch_bounds__(256) k_pois_km(PoisOp P, const double *__restrict__ x, const double *__restrict__ b,
+                                                 double *__restrict__ y, float *__restrict__ y32, double *part,
+                                                 KTiles t) {
     __shared__ double sh[32];
     double acc[2] = {0.0, 0.0};
-    int i, j, k;
-    if (tile_point(t, i, j, k)) {
-        long id = gidx(P.d, i, j, k);
-        double bi = b[id];
-        double ri = bi - pois_row(P, i, j, k, id, x);
-        if (r) r[id] = ri;
-        if (r32) r32[id] = (float)ri;
-        acc[0] = ri * ri;
-        acc[1] = bi * bi;
+    int i, j, k0, k1;
+    bool act;
+    ktile_column(t, i, j, k0, k1, act);
+    if (act) {
+        const Dims &d = P.d;
+        const GridDev &g = P.g;
+        const int NI = d.NX - 1, NJ = d.NY - 1, NK = d.NZ - 1;
+        const double Cx = P.Cx, Cy = P.Cy, Cz = P.Cz;
+        const bool bnd_col = (i == NI) || (j == NJ);
+        const int ip1 = i + 1 <= NI ? i + 1 : NI, jp1 = j + 1 <= NJ ? j + 1 : NJ;
+        const double mpx = g.mxc[i], mex = g.mxu[ip1], mwx = g.mxu[i];
+        const double mpy = g.myc[j], mny = g.myv[jp1], msy = g.myv[j];
+        const double axy = -Cx * mpx * (mex + mwx) - Cy * mpy * (mny + msy);
+        const double aw = Cx * mpx * mwx, ae = Cx * mpx * mex, an = Cy * mpy * mny, as = Cy * mpy * msy;
+        const bool hs = j > 0, hw = i > 0, he = i < NI - 1, hn = j < NJ - 1;
+        const bool pin_col = P.neumann == 0 && i == NI - 1 && j == NJ - 1;
+        // neighbour offsets clamped to the point itself when absent, so all
+        // loads are issued unconditionally and overlap
+        const long os = hs ? d.px : 0, on = (j < NJ) ? d.px : 0;
+        const long ow = hw ? 1 : 0, oe = (i < NI) ? 1 : 0;
+        long id = gidx(d, i, j, k0);
+        double xm = x[id - ((k0 > 0) ? d.plane : 0)];
+        double xc = x[id];
+        for (int k = k0; k < k1; k++) {
+            double xp = x[id + ((k + 1 <= NK) ? d.plane : 0)];
+            const double xs = x[id - os], xw = x[id - ow], xe = x[id + oe], xn = x[id + on];
+            const double bi = RESID ? b[id] : 0.0;
+            double v;
+            if (bnd_col || k == NK) {
+                v = P.ap_solid * xc;   // BOUNDARY half-cell rows
+            } else {
+                const double mpz = g.mzc[k], mfz = g.mzw[k + 1], mbz = g.mzw[k];
+                double ap = axy - Cz * mpz * (mbz + mfz);
+                const double ab = Cz * mpz * mbz, af = Cz * mpz * mfz;
+                const bool hb = k > 0, hf = k < NK - 1;
+                if (!hb) ap += ab;
+                if (!hs) ap += as;
+                if (!hw) ap += aw;
+                if (!he && !(pin_col && k == NK - 1)) ap += ae;
+                if (!hn) ap += an;
+                if (!hf) ap += af;
+                double r = 0.0;
+                r = hb ? r + ab * xm : r;
+                r = hs ? r + as * xs : r;
+                r = hw ? r + aw * xw : r;
+                r += ap * xc;
+                r = he ? r + ae * xe : r;
+                r = hn ? r + an * xn : r;
+                r = hf ? r + af * xp : r;
+                v = r;
+            }
+            if (RESID) {
+                double ri = bi - v;
+                if (y) y[id] = ri;
+                if (y32) y32[id] = (float)ri;
+                acc[0] += ri * ri;
+                acc[1] += bi * bi;
+            } else {
+                y[id] = v;
+                acc[0] += xc * v;
+            }
+            xm = xc;
+            xc = xp;
+            id += d.plane;
+        }
     }
     block_sum<2>(acc, sh);
     if (threadIdx.x == 0 && threadIdx.y == 0) {
@@ -140,15 +154,11 @@ extern "C" int tb_poisson_apply(tb_ctx *c, int f_in, int f_out) {
         if (c) c->err = "tb_poisson_apply: bad argument";
         return -1;
     }
-    Tiles t = make_tiles(c->d.NX, c->d.NY, c->d.NZ);
-    if (t.per_xcd * NXCD > RED_MAX_BLOCKS * 64) { c->err = "grid too large for reduction buffer"; return -1; }
-    double *part = nullptr;
-    TB_CHECK(c, hipMalloc((void **)&part, sizeof(double) * t.per_xcd * NXCD));
-    hipLaunchKernelGGL(k_pois_apply, tiles_grid(t), tiles_block(), 0, c->s, make_pois(c, 0), c->f[f_in], c->f[f_out],
-                       part, t);
+    KTiles t = make_ktiles(c->d.NX, c->d.NY, c->d.NZ);
+    hipLaunchKernelGGL(k_pois_km<0>, ktiles_grid(t), tiles_block(), 0, c->s, make_pois(c, 0), (const double *)c->f[f_in],
+                       (const double *)nullptr, c->f[f_out], (float *)nullptr, c->red, t);
     TB_LAUNCH_CHECK(c);
     TB_CHECK(c, hipStreamSynchronize(c->s));
-    hipFree(part);
     return 0;
 }
 
@@ -188,13 +198,18 @@ __global__ void __launch_bounds__(256) k_rbgs(LV<T> L, T *__restrict__ x, const 
     if (i >= L.nx) return;
     long id = lidx(L, i, j, k);
     long pl = (long)L.px * L.py;
+    const bool hw = i > 0, he = i < L.nx - 1, hs = j > 0, hn = j < L.ny - 1, hb = k > 0, hf = k < L.nz - 1;
+    // all neighbour loads issued up front (clamped to the cell itself)
+    const T xw = x[id - (hw ? 1 : 0)], xe = x[id + (he ? 1 : 0)];
+    const T xs = x[id - (hs ? L.px : 0)], xn = x[id + (hn ? L.px : 0)];
+    const T xb = x[id - (hb ? pl : 0)], xf = x[id + (hf ? pl : 0)];
     T s = b[id], dg = (T)0;
-    if (i > 0) { s += L.cx * x[id - 1]; dg += L.cx; }
-    if (i < L.nx - 1) { s += L.cx * x[id + 1]; dg += L.cx; }
-    if (j > 0) { s += L.cy * x[id - L.px]; dg += L.cy; }
-    if (j < L.ny - 1) { s += L.cy * x[id + L.px]; dg += L.cy; }
-    if (k > 0) { s += L.cz * x[id - pl]; dg += L.cz; }
-    if (k < L.nz - 1) { s += L.cz * x[id + pl]; dg += L.cz; }
+    s = hw ? s + L.cx * xw : s; dg = hw ? dg + L.cx : dg;
+    s = he ? s + L.cx * xe : s; dg = he ? dg + L.cx : dg;
+    s = hs ? s + L.cy * xs : s; dg = hs ? dg + L.cy : dg;
+    s = hn ? s + L.cy * xn : s; dg = hn ? dg + L.cy : dg;
+    s = hb ? s + L.cz * xb : s; dg = hb ? dg + L.cz : dg;
+    s = hf ? s + L.cz * xf : s; dg = hf ? dg + L.cz : dg;
     x[id] = s / dg;
 }
 
@@ -205,14 +220,16 @@ __global__ void __launch_bounds__(256) k_mg_resid(LV<T> L, const T *__restrict__
     if (!tile_point(t, i, j, k)) return;
     long id = lidx(L, i, j, k);
     long pl = (long)L.px * L.py;
-    T xc = x[id], s = (T)0;
-    if (i > 0) s += L.cx * (xc - x[id - 1]);
-    if (i < L.nx - 1) s += L.cx * (xc - x[id + 1]);
-    if (j > 0) s += L.cy * (xc - x[id - L.px]);
-    if (j < L.ny - 1) s += L.cy * (xc - x[id + L.px]);
-    if (k > 0) s += L.cz * (xc - x[id - pl]);
-    if (k < L.nz - 1) s += L.cz * (xc - x[id + pl]);
-    r[id] = b[id] - s;
+    const bool hw = i > 0, he = i < L.nx - 1, hs = j > 0, hn = j < L.ny - 1, hb = k > 0, hf = k < L.nz - 1;
+    // absent neighbours are clamped to the cell itself, contributing xc - xc = 0
+    const T xc = x[id];
+    const T xw = x[id - (hw ? 1 : 0)], xe = x[id + (he ? 1 : 0)];
+    const T xs = x[id - (hs ? L.px : 0)], xn = x[id + (hn ? L.px : 0)];
+    const T xb = x[id - (hb ? pl : 0)], xf = x[id + (hf ? pl : 0)];
+    const T bb = b[id];
+    T s = L.cx * (xc - xw) + L.cx * (xc - xe) + L.cy * (xc - xs) + L.cy * (xc - xn) + L.cz * (xc - xb) +
+          L.cz * (xc - xf);
+    r[id] = bb - s;
 }
 
 __device__ inline int clampi(int a, int lo, int hi) { return a < lo ? lo : (a > hi ? hi : a); }
@@ -474,9 +491,15 @@ static int vcycle_level(tb_ctx *c, int l) {
     Tiles all = make_tiles(L.nx, L.ny, L.nz);
     Tiles half = make_tiles((L.nx + 1) / 2, L.ny, L.nz);
     hipLaunchKernelGGL(k_zero<T>, tiles_grid(all), tiles_block(), 0, c->s, lv, B.x, all);
+    const int kid = (l == 0) ? (sizeof(T) == 8 ? TB_K_RBGS0 : TB_K_RBGS0_F32) : -1;
+    auto smooth = [&](int color) {
+        if (c->prof_on && kid >= 0) tb_prof_begin(c, kid);
+        hipLaunchKernelGGL(k_rbgs<T>, tiles_grid(half), tiles_block(), 0, c->s, lv, B.x, (const T *)B.b, color, half);
+        if (c->prof_on && kid >= 0) tb_prof_end(c, kid);
+    };
     for (int s = 0; s < c->cfg.mg_pre; s++) {
-        hipLaunchKernelGGL(k_rbgs<T>, tiles_grid(half), tiles_block(), 0, c->s, lv, B.x, (const T *)B.b, 0, half);
-        hipLaunchKernelGGL(k_rbgs<T>, tiles_grid(half), tiles_block(), 0, c->s, lv, B.x, (const T *)B.b, 1, half);
+        smooth(0);
+        smooth(1);
     }
     hipLaunchKernelGGL(k_mg_resid<T>, tiles_grid(all), tiles_block(), 0, c->s, lv, (const T *)B.x, (const T *)B.b,
                        B.r, all);
@@ -491,8 +514,8 @@ static int vcycle_level(tb_ctx *c, int l) {
     hipLaunchKernelGGL(k_prolong<T>, tiles_grid(all), tiles_block(), 0, c->s, lv, cv, L.cx, L.cy, L.cz,
                        (const T *)CB.x, B.x, all);
     for (int s = 0; s < c->cfg.mg_post; s++) {
-        hipLaunchKernelGGL(k_rbgs<T>, tiles_grid(half), tiles_block(), 0, c->s, lv, B.x, (const T *)B.b, 1, half);
-        hipLaunchKernelGGL(k_rbgs<T>, tiles_grid(half), tiles_block(), 0, c->s, lv, B.x, (const T *)B.b, 0, half);
+        smooth(1);
+        smooth(0);
     }
     TB_LAUNCH_CHECK(c);
     return 0;
@@ -571,6 +594,7 @@ static int read_s(tb_ctx *c, int first, int count) {
 
 struct PoisCtx {
     Tiles all, cells;
+    KTiles kt;
     PoisOp Pp, Pn;
     long nblk;
     long corner;
@@ -582,7 +606,8 @@ static PoisCtx pois_ctx(tb_ctx *c) {
     P.cells = make_tiles(c->ncx, c->ncy, c->ncz);
     P.Pp = make_pois(c, 0);
     P.Pn = make_pois(c, 1);
-    P.nblk = P.all.per_xcd * NXCD;
+    P.kt = make_ktiles(c->d.NX, c->d.NY, c->d.NZ);
+    P.nblk = P.kt.per_xcd * NXCD;
     P.corner = gidx(c->d, c->ncx - 1, c->ncy - 1, c->ncz - 1);
     return P;
 }
@@ -598,8 +623,8 @@ static int shift_corner(tb_ctx *c, const PoisCtx &P, double *x) {
 // r = b - A x (pinned or Neumann), returns ||r||^2 and ||b||^2 via hscal
 static int residual(tb_ctx *c, const PoisCtx &P, bool neumann, const double *x, const double *b, double *r,
                     float *r32, double &rr, double &bb) {
-    hipLaunchKernelGGL(k_pois_resid, tiles_grid(P.all), tiles_block(), 0, c->s, neumann ? P.Pn : P.Pp, x, b, r, r32,
-                       c->red, P.all);
+    hipLaunchKernelGGL(k_pois_km<1>, ktiles_grid(P.kt), tiles_block(), 0, c->s, neumann ? P.Pn : P.Pp, x, b, r, r32,
+                       c->red, P.kt);
     TB_LAUNCH_CHECK(c);
     if (tb_reduce_finish(c, (int)P.nblk, 2, S_TMP1)) return -1;
     if (read_s(c, S_TMP1, 2)) return -1;
@@ -626,12 +651,14 @@ static int pcg64(tb_ctx *c, const PoisCtx &P, double *x, const double *b, double
         if (tb_dot(c, r, z, S_RZ)) return -1;
         int inner = 0;
         while (its < maxit) {
-            hipLaunchKernelGGL(k_pois_apply, tiles_grid(P.all), tiles_block(), 0, c->s, P.Pn, (const double *)p, q,
-                               c->red, P.all);
+            TB_PROF(c, TB_K_POIS_OP,
+                    hipLaunchKernelGGL(k_pois_km<0>, ktiles_grid(P.kt), tiles_block(), 0, c->s, P.Pn,
+                                       (const double *)p, (const double *)nullptr, q, (float *)nullptr, c->red, P.kt));
             TB_LAUNCH_CHECK(c);
             if (tb_reduce_finish(c, (int)P.nblk, 1, S_PQ)) return -1;
-            hipLaunchKernelGGL(k_pcg_xr, dim3(PB), dim3(PT), 0, c->s, n, x, r, (const double *)p, (const double *)q,
-                               (const double *)c->scal, c->red);
+            TB_PROF(c, TB_K_PCG_UPD,
+                    hipLaunchKernelGGL(k_pcg_xr, dim3(PB), dim3(PT), 0, c->s, n, x, r, (const double *)p,
+                                       (const double *)q, (const double *)c->scal, c->red));
             TB_LAUNCH_CHECK(c);
             if (tb_reduce_finish(c, PB, 1, S_RR)) return -1;
             its++;
@@ -792,10 +819,12 @@ __global__ void __launch_bounds__(256) k_project(Dims d, GridDev g, double *__re
 extern "C" int tb_project(tb_ctx *c, double dt) {
     if (!c) return -1;
     Tiles t = make_tiles(c->d.NX, c->d.NY, c->d.NZ);
+    if (c->prof_on) tb_prof_begin(c, TB_K_PROJECT);
     hipLaunchKernelGGL(k_project, tiles_grid(t), tiles_block(), 0, c->s, c->d, c->g, c->f[TB_F_U], c->f[TB_F_V],
                        c->f[TB_F_W], c->f[TB_F_DPDX], c->f[TB_F_DPDY], c->f[TB_F_DPDZ], c->f[TB_F_PHI],
                        c->f[TB_F_PRHS], c->st[0], c->st[1], c->st[2], dt, dt / c->cfg.re, 1.0 / c->dEta,
                        1.0 / c->dXi, 1.0 / c->dPhi, t);
+    if (c->prof_on) tb_prof_end(c, TB_K_PROJECT);
     TB_LAUNCH_CHECK(c);
     return 0;
 }
