@@ -60,6 +60,7 @@ def lib():
         L.ora_helmholtz_matrix.argtypes = [P, C.c_int, C.c_double, P, P, P]
         L.ora_poisson_matrix.argtypes = [P, P, P, P]
         L.ora_set_geometry.argtypes = [P, C.c_int, P, P, C.c_int]
+        L.ora_gmg_vcycle.argtypes = [P, P, P]
         _LIB = L
     return _LIB
 
@@ -210,6 +211,12 @@ class Oracle:
         val = np.zeros(n * 9)
         self.L.ora_poisson_matrix(self.h, _ptr(cnt), _ptr(col), _ptr(val))
         return self._csr(cnt, col, val)
+
+    def vcycle(self, r_cells):
+        r = np.ascontiguousarray(r_cells, dtype=np.float64)
+        z = np.zeros_like(r)
+        self.L.ora_gmg_vcycle(self.h, _ptr(r), _ptr(z))
+        return z
 
     def gmg_levels(self):
         dims = np.zeros(48, dtype=np.int32)

@@ -253,6 +253,12 @@ static void vcycle_level(ora_ctx *o, int l) {
     for (int s = 0; s < post; s++) { rbgs(L, L->x, L->b, 1); rbgs(L, L->x, L->b, 0); }
 }
 
+/* exported V-cycle on compact (nx*ny*nz, x fastest) cell arrays */
+int ora_gmg_vcycle(ora_ctx *o, const double *r, double *z) {
+    ora_vcycle(o, r, z);
+    return 0;
+}
+
 /* z = V(r): one symmetric V-cycle, output projected to zero mean */
 void ora_vcycle(ora_ctx *o, const double *r, double *z) {
     if (o->nlev == 0) ora_gmg_build(o);

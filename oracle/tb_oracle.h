@@ -65,6 +65,7 @@ int ora_step(ora_ctx *o, double dt, tb_step_stats *st);
 /* GMG hierarchy description (for tests): returns number of levels and fills
  * dims[3*l..3*l+2] */
 int ora_gmg_levels(ora_ctx *o, int32_t *dims, int cap);
+int ora_gmg_vcycle(ora_ctx *o, const double *r_compact, double *z_compact);
 
 #ifdef __cplusplus
 }
