@@ -7,7 +7,8 @@
  *      0  ok
  *     >0  finished but not converged / fallback taken (see the stats struct)
  *     <0  hard error; tb_last_error(ctx) holds the message.
- * Nothing here calls exit().  One host thread per context; the context owns
+ * Nothing here calls exit() (the reference aborts the process on any PETSc
+ * error, definitions.h:116-123).  One host thread per context; the context owns
  * one HIP stream and all device buffers.  Host pointers are borrowed for the
  * duration of a call only.  Calls are synchronous unless named _async.
  *
@@ -23,9 +24,9 @@
  *                        Include/Velocity.h and Conc_ENO_ghost_cells
  *                        Include/Conc.h, and the transposes
  *                        Velocity_transpose_velocities / Velocity_cell_center)
- *   tb_set_rhs           Velocity_u/v/w_set_RHS            Include/Velocity.h:18-20
+ *   tb_set_rhs           Velocity_u/v/w_set_RHS            Include/Velocity.h:15-17
  *                        Conc_set_RHS                      Include/Conc.h:30
- *   tb_helmholtz_solve   Velocity_solve                    Include/Velocity.h:15
+ *   tb_helmholtz_solve   Velocity_solve                    Include/Velocity.h:36
  *                        Conc_solve                        Include/Conc.h:29
  *                        (the diagonal update of Velocity_modify_diagonal /
  *                        Conc_modify_diagonal is applied from dt)
@@ -141,38 +142,82 @@ typedef struct tb_step_stats {
 typedef struct tb_ctx tb_ctx;
 
 /* lifecycle ------------------------------------------------------------- */
+/* Grid_create (Grid.c:21) + Velocity_create (Velocity.c:19, Velocity.h:4),
+ * Conc_create (Conc.c:22, Conc.h:25), Pressure_create (Pressure.c:15,
+ * Pressure.h:4):
+ * grid metrics, every field, the Helmholtz/Poisson operators (matrix-free)
+ * and the multigrid hierarchy are set up on the device. */
 tb_ctx *tb_create(const tb_config *cfg);
+/* Grid_destroy (Grid.c:387), Velocity_destroy (Velocity.c:239),
+ * Conc_destroy (Conc.c:221), Pressure_destroy (Pressure.c:98) */
 void tb_destroy(tb_ctx *ctx);
 const char *tb_last_error(tb_ctx *ctx);
 int tb_device_bytes(tb_ctx *ctx, int64_t *bytes);
 
 /* geometry: SDF reinitialisation, node classification and IB tables.
  * h_xz: bottom height at cell centres, (nz+1)*(nx+1) doubles, x fastest
- * (grid->interface_position before the offset), or NULL for no solid.  */
+ * (grid->interface_position before the offset), or NULL for no solid.
+ * Replaces Grid_describe_interface + Surface_initialize +
+ * Levelset_reinitialize + Surface_find_q_sdf (Grid.c:712-1121,
+ * Surface.c:96-334, Levelset.c:85-709), Grid_identify_geometry
+ * (Grid.c:1825-1872, Grid.h:20) and Immersed_setup_q_immersed_nodes for
+ * u, v, w, c (Immersed.c:546-617, Immersed.h:13). */
 int tb_setup_geometry(tb_ctx *ctx, const double *h_xz);
+/* grid->G_status / G_u_status ... as uint8 (Grid.c:159-179) */
 int tb_download_flags(tb_ctx *ctx, int q, uint8_t *out);
+/* Immersed->N_IB (DataTypes.h:548-557) */
 int tb_ib_count(tb_ctx *ctx, int q);
+/* Immersed->ImmersedNodes[] in k,j,i order (Immersed.c:423-444) */
 int tb_download_ib_table(tb_ctx *ctx, int q, tb_ib_node *out, int capacity);
+/* pseudo-time iterations taken by Levelset_reinitialize (Levelset.c:140) */
 int tb_reinit_iterations(tb_ctx *ctx);
 
 /* fields ------------------------------------------------------------------ */
+/* host layout = the reference's DA natural order (padded, x fastest, FP64);
+ * replaces the DAVecGetArray access to the G_data Vecs (e.g. Velocity.c:88) */
 int tb_upload_field(tb_ctx *ctx, int f, const double *host);
 int tb_download_field(tb_ctx *ctx, int f, double *host);
 
 /* per-operation mirrors of the reference entry points ---------------------- */
+/* advective term q -> TB_F_CONV_q for the whole field: replaces the
+ * per-line ENO_compute_convective_terms (ENO.c:103-245, ENO.h:6) together
+ * with its gathers, the wall ghosts (Velocity.c:2124-2801,
+ * Conc.c:1021-1220), Velocity_transpose_velocities (Velocity.c:3341-3663)
+ * and Velocity_cell_center (Velocity.c:1476-1602) */
 int tb_eno_conv(tb_ctx *ctx, int q);
+/* Velocity_u/v/w_set_RHS (Velocity.c:1607-2043, Velocity.h:15-17),
+ * Conc_set_RHS (Conc.c:864-992, Conc.h:30) */
 int tb_set_rhs(tb_ctx *ctx, int q, double dt);
+/* y = A_q x with the rows of Velocity_*_form_LHS_matrix
+ * (Velocity.c:449-1471) / Conc_form_LHS_matrix (Conc.c:390-730) after
+ * Velocity_modify_diagonal / Conc_modify_diagonal for dt
+ * (Velocity.c:374-443, Conc.c:293-354) */
 int tb_helmholtz_apply(tb_ctx *ctx, int q, double dt, int f_in, int f_out);
+/* Velocity_solve (Velocity.c:306-324, Velocity.h:36) / Conc_solve
+ * (Conc.c:840-859, Conc.h:29): BiCGStab, no preconditioner, previous field
+ * as the initial guess (Solver.c:39-86, 269-322) */
 int tb_helmholtz_solve(tb_ctx *ctx, int q, double dt, tb_solve_stats *st);
+/* Pressure_set_RHS (Pressure.c:171-276, Pressure.h:9) */
 int tb_poisson_rhs(tb_ctx *ctx, double dt);
+/* y = A_p x, rows of Pressure_form_LHS_matrix (Pressure.c:691-936) */
 int tb_poisson_apply(tb_ctx *ctx, int f_in, int f_out);
+/* Pressure_solve (Pressure.c:443-458, Pressure.h:11), which returns the
+ * iteration count; here st->iterations.  Krylov/multigrid method chosen by
+ * tb_set_poisson_mode (the reference uses GMRES + BoomerAMG, Solver.c:90-265) */
 int tb_poisson_solve(tb_ctx *ctx, tb_solve_stats *st);
+/* Pressure_project_velocity (Pressure.c:280-439, Pressure.h:10) */
 int tb_project(tb_ctx *ctx, double dt);
+/* Pressure_compute_velocity_divergence (Pressure.c:462-586, Pressure.h:12) */
 int tb_div_max(tb_ctx *ctx, double *div_max, int32_t *ijk);
+/* Velocity_store_old_data + Conc_store_old_data (Velocity.c:297-302,
+ * Velocity.h:6; Conc.c:997-1003, Conc.h:31) for u, v, w, c */
 int tb_store_old(tb_ctx *ctx);
+/* Velocity_rk_average + Conc_rk_average (Velocity.c:2047-2055,
+ * Velocity.h:19; Conc.c:1007-1017, Conc.h:19): q = old_frac*q_old + new_frac*q */
 int tb_rk_average(tb_ctx *ctx, double old_frac, double new_frac);
 
-/* composite SSP-RK2 step (the missing driver's role) */
+/* composite SSP-RK2 step: the per-step sequence of the unpublished driver
+ * (README.md:26), reconstructed from the TVD_RK2 scheme (definitions.h:148-149) */
 int tb_step(tb_ctx *ctx, double dt, tb_step_stats *st);
 
 /* solver controls (no reference counterpart: the reference fixes these in

@@ -207,6 +207,7 @@ struct tb_ctx {
     Level lev[16];
     double *coarse_inv64 = nullptr;
     float *coarse_inv32 = nullptr;
+    float *cg32[3] = {};         // p, q, e of the FP32 inner CG (level-0 layout)
     int coarse_n = 0;
     bool mg_built = false;
 

@@ -369,6 +369,7 @@ void tb_mg_free(tb_ctx *c) {
     }
     hipFree(c->coarse_inv64); hipFree(c->coarse_inv32);
     c->coarse_inv64 = nullptr; c->coarse_inv32 = nullptr;
+    for (int a = 0; a < 3; a++) { hipFree(c->cg32[a]); c->cg32[a] = nullptr; }
     c->nlev = 0;
     c->mg_built = false;
 }
@@ -416,6 +417,12 @@ int tb_mg_build(tb_ctx *c) {
         TB_CHECK(c, hipMemsetAsync(L.b32, 0, sizeof(float) * L.ntot, c->s));
         TB_CHECK(c, hipMemsetAsync(L.r32, 0, sizeof(float) * L.ntot, c->s));
         c->bytes += 3 * sizeof(float) * L.ntot;
+    }
+    // vectors of the FP32 inner CG; entries outside the interior cells stay 0
+    for (int a = 0; a < 3; a++) {
+        TB_CHECK(c, hipMalloc((void **)&c->cg32[a], sizeof(float) * L0.ntot));
+        TB_CHECK(c, hipMemsetAsync(c->cg32[a], 0, sizeof(float) * L0.ntot, c->s));
+        c->bytes += sizeof(float) * L0.ntot;
     }
     // dense inverse of (A_c + J/n) on the coarsest level (Gauss-Jordan,
     // partial pivoting), FP64 on the host
@@ -681,8 +688,144 @@ static int pcg64(tb_ctx *c, const PoisCtx &P, double *x, const double *b, double
     return 0;
 }
 
-// FP32 V-cycles inside FP64 iterative refinement; stops at rtol, at maxit,
-// or when a cycle reduces the residual by less than 'stall'.
+// ---------------------------------------------------------------------------
+// FP32 inner CG of the mixed-precision solve.  Vectors live in the level-0
+// layout (the padded field layout); every entry outside the interior cells is
+// 0 and stays 0, so the vector kernels sweep the whole array as float4.  Dot
+// products accumulate in FP64.
+
+// q = A p (Neumann -Laplacian of level 0) with the partial p.q
+__global__ void __launch_bounds__(256) k_cg32_apply(LV<float> L, const float *__restrict__ p, float *__restrict__ q,
+                                                    double *part, KTiles t) {
+    __shared__ double sh[16];
+    double acc[1] = {0.0};
+    int i, j, k0, k1;
+    bool act;
+    ktile_column(t, i, j, k0, k1, act);
+    if (act) {
+        const long pl = (long)L.px * L.py;
+        // absent neighbours are clamped to the cell itself and contribute 0
+        const long ow = i > 0 ? 1 : 0, oe = i < L.nx - 1 ? 1 : 0;
+        const long os = j > 0 ? L.px : 0, on = j < L.ny - 1 ? L.px : 0;
+        long id = lidx(L, i, j, k0);
+        float pm = p[id - (k0 > 0 ? pl : 0)];
+        float pc = p[id];
+        for (int k = k0; k < k1; k++) {
+            const float pp = p[id + (k < L.nz - 1 ? pl : 0)];
+            const float pw = p[id - ow], pe = p[id + oe], ps = p[id - os], pn = p[id + on];
+            const float v = L.cx * (pc - pw) + L.cx * (pc - pe) + L.cy * (pc - ps) + L.cy * (pc - pn) +
+                            L.cz * (pc - pm) + L.cz * (pc - pp);
+            q[id] = v;
+            acc[0] += (double)pc * (double)v;
+            pm = pc;
+            pc = pp;
+            id += pl;
+        }
+    }
+    block_sum<1>(acc, sh);
+    if (threadIdx.x == 0 && threadIdx.y == 0) part[blockIdx.x] = acc[0];
+}
+
+// e += alpha p ; r -= alpha q ; partial r.r      (alpha = rz / pq)
+__global__ void __launch_bounds__(PT) k_cg32_xr(long n4, float4 *__restrict__ e, float4 *__restrict__ r,
+                                                const float4 *__restrict__ p, const float4 *__restrict__ q,
+                                                const double *__restrict__ s, double *part) {
+    __shared__ double sh[16];
+    const float alpha = (float)(s[S_RZ] / s[S_PQ]);
+    double acc[1] = {0.0};
+    for (long a = blockIdx.x * (long)blockDim.x + threadIdx.x; a < n4; a += (long)gridDim.x * blockDim.x) {
+        const float4 pv = p[a], qv = q[a];
+        float4 ev = e[a], rv = r[a];
+        ev.x += alpha * pv.x; ev.y += alpha * pv.y; ev.z += alpha * pv.z; ev.w += alpha * pv.w;
+        rv.x -= alpha * qv.x; rv.y -= alpha * qv.y; rv.z -= alpha * qv.z; rv.w -= alpha * qv.w;
+        e[a] = ev;
+        r[a] = rv;
+        acc[0] += (double)rv.x * rv.x + (double)rv.y * rv.y + (double)rv.z * rv.z + (double)rv.w * rv.w;
+    }
+    block_sum<1>(acc, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc[0];
+}
+
+__global__ void __launch_bounds__(PT) k_dot32(long n4, const float4 *__restrict__ a, const float4 *__restrict__ b,
+                                              double *part) {
+    __shared__ double sh[16];
+    double acc[1] = {0.0};
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n4; i += (long)gridDim.x * blockDim.x) {
+        const float4 u = a[i], v = b[i];
+        acc[0] += (double)u.x * v.x + (double)u.y * v.y + (double)u.z * v.z + (double)u.w * v.w;
+    }
+    block_sum<1>(acc, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc[0];
+}
+
+// p = z + beta p, beta = rz_new / rz
+__global__ void __launch_bounds__(PT) k_cg32_p(long n4, const float4 *__restrict__ z, float4 *__restrict__ p,
+                                               const double *__restrict__ s) {
+    const float beta = (float)(s[S_TMP0] / s[S_RZ]);
+    for (long a = blockIdx.x * (long)blockDim.x + threadIdx.x; a < n4; a += (long)gridDim.x * blockDim.x) {
+        const float4 zv = z[a];
+        float4 pv = p[a];
+        pv.x = zv.x + beta * pv.x; pv.y = zv.y + beta * pv.y; pv.z = zv.z + beta * pv.z; pv.w = zv.w + beta * pv.w;
+        p[a] = pv;
+    }
+}
+
+// Inner solve A e = r on level 0 by FP32 CG preconditioned with FP32
+// V-cycles, from e = 0, until ||r|| <= tol_rel * ||r0|| or maxit iterations.
+// r is lev[0].b32 (updated in place), z is lev[0].x32, e is cg32[2].
+static int cg32_inner(tb_ctx *c, const KTiles &kt, double r0n, double tol_rel, int maxit, int &its, int &vcy) {
+    Level &L0 = c->lev[0];
+    const long n4 = L0.ntot / 4;
+    float *r = L0.b32, *z = L0.x32, *p = c->cg32[0], *q = c->cg32[1], *e = c->cg32[2];
+    const LV<float> lv = lv_of<float>(L0);
+    const int nblk = (int)(kt.per_xcd * NXCD);
+    TB_CHECK(c, hipMemsetAsync(e, 0, sizeof(float) * L0.ntot, c->s));
+    if (tb_vcycle32(c)) return -1;
+    vcy++;
+    TB_CHECK(c, hipMemcpyAsync(p, z, sizeof(float) * L0.ntot, hipMemcpyDeviceToDevice, c->s));
+    hipLaunchKernelGGL(k_dot32, dim3(PB), dim3(PT), 0, c->s, n4, (const float4 *)r, (const float4 *)z, c->red);
+    TB_LAUNCH_CHECK(c);
+    if (tb_reduce_finish(c, PB, 1, S_RZ)) return -1;
+    int inner = 0;
+    while (inner < maxit) {
+        TB_PROF(c, TB_K_POIS_OP,
+                hipLaunchKernelGGL(k_cg32_apply, ktiles_grid(kt), tiles_block(), 0, c->s, lv, (const float *)p, q,
+                                   c->red, kt));
+        TB_LAUNCH_CHECK(c);
+        if (tb_reduce_finish(c, nblk, 1, S_PQ)) return -1;
+        TB_PROF(c, TB_K_PCG_UPD,
+                hipLaunchKernelGGL(k_cg32_xr, dim3(PB), dim3(PT), 0, c->s, n4, (float4 *)e, (float4 *)r,
+                                   (const float4 *)p, (const float4 *)q, (const double *)c->scal, c->red));
+        TB_LAUNCH_CHECK(c);
+        if (tb_reduce_finish(c, PB, 1, S_RR)) return -1;
+        inner++;
+        its++;
+        if (read_s(c, S_RR, 1)) return -1;
+        const double rel = sqrt(c->hscal[S_RR]) / r0n;
+        if (!(rel == rel)) { c->err = "Poisson FP32 CG produced NaN"; return -1; }
+        if (rel <= tol_rel || inner >= maxit) break;
+        if (tb_vcycle32(c)) return -1;
+        vcy++;
+        hipLaunchKernelGGL(k_dot32, dim3(PB), dim3(PT), 0, c->s, n4, (const float4 *)r, (const float4 *)z, c->red);
+        TB_LAUNCH_CHECK(c);
+        if (tb_reduce_finish(c, PB, 1, S_TMP0)) return -1;
+        hipLaunchKernelGGL(k_cg32_p, dim3(PB), dim3(PT), 0, c->s, n4, (const float4 *)z, (float4 *)p,
+                           (const double *)c->scal);
+        hipLaunchKernelGGL(k_copy_scalar, dim3(1), dim3(64), 0, c->s, c->scal, S_RZ, S_TMP0);
+        TB_LAUNCH_CHECK(c);
+    }
+    return 0;
+}
+
+// Mixed precision: FP64 iterative refinement (true residual r = b - A x in
+// FP64, rounded to FP32) around the FP32 inner CG, x += e in FP64.  Each inner
+// solve asks for the residual reduction still missing (times 0.5), but never
+// less than IR_INNER_REDUCTION, which FP32 reaches reliably.  Stops at rtol, at
+// maxit inner iterations, or when an outer pass reduces the FP64 residual by
+// less than half (stalled: the caller falls back to FP64 PCG).
+constexpr double IR_INNER_REDUCTION = 1e-4;
+constexpr int IR_INNER_MAXIT = 40;
+
 static int mixed_ir(tb_ctx *c, const PoisCtx &P, double *x, const double *b, double bn, double rtol, int maxit,
                     int &its, int &vcy, double &res, bool &stalled) {
     double *r = c->kv[0];
@@ -690,23 +833,24 @@ static int mixed_ir(tb_ctx *c, const PoisCtx &P, double *x, const double *b, dou
     double rr, bb;
     stalled = false;
     if (tb_mg_build(c)) return -1;
+    const KTiles kt0 = make_ktiles(L0.nx, L0.ny, L0.nz);
+    if (kt0.per_xcd * NXCD > RED_MAX_BLOCKS) { c->err = "grid too large for the reduction buffer"; return -1; }
     if (shift_corner(c, P, x)) return -1;
     if (residual(c, P, true, x, b, r, L0.b32, rr, bb)) return -1;
     res = sqrt(rr) / bn;
-    double prev = res;
-    int slow = 0;
     while (res > rtol && its < maxit) {
-        if (tb_vcycle32(c)) return -1;
-        vcy++;
-        hipLaunchKernelGGL(k_add32, tiles_grid(P.cells), tiles_block(), 0, c->s, c->d, x, (const float *)L0.x32, P.cells);
+        const double prev = res;
+        double want = 0.5 * rtol / res;
+        if (want < IR_INNER_REDUCTION) want = IR_INNER_REDUCTION;
+        int room = maxit - its;
+        if (cg32_inner(c, kt0, sqrt(rr), want, room < IR_INNER_MAXIT ? room : IR_INNER_MAXIT, its, vcy)) return -1;
+        hipLaunchKernelGGL(k_add32, tiles_grid(P.cells), tiles_block(), 0, c->s, c->d, x, (const float *)c->cg32[2],
+                           P.cells);
         TB_LAUNCH_CHECK(c);
-        its++;
         if (residual(c, P, true, x, b, r, L0.b32, rr, bb)) return -1;
         res = sqrt(rr) / bn;
         if (!(res == res)) { c->err = "Poisson IR produced NaN"; return -1; }
-        if (res > 0.6 * prev) { if (++slow >= 2) { stalled = true; break; } }
-        else slow = 0;
-        prev = res;
+        if (res > 0.5 * prev) { stalled = true; break; }
     }
     if (shift_corner(c, P, x)) return -1;
     if (residual(c, P, false, x, b, r, nullptr, rr, bb)) return -1;
