@@ -225,6 +225,42 @@ int tb_step(tb_ctx *ctx, double dt, tb_step_stats *st);
 int tb_set_poisson_mode(tb_ctx *ctx, int mode, double rtol);
 int tb_set_helmholtz_rtol(tb_ctx *ctx, double rtol);
 
+/* learned initial guess of the Poisson solve (no reference counterpart: the
+ * reference starts GMRES from the previous phi, Solver.c:256).  A 3-level
+ * 3-D conv encoder-decoder (16/32/64 channels, 3x3x3, BF16 weights, FP32
+ * accumulation) maps (b_n, phi_{n-1}, phi_{n-2}) to a correction of
+ * phi_{n-1}.  The network lives on the device of its creator; a context
+ * uses it in TB_POIS_LEARNED_IR mode once attached.  Cell counts of every
+ * grid it runs on must be multiples of 4.                                  */
+typedef struct tb_net tb_net;
+tb_net *tb_net_create(uint64_t seed);            /* He-uniform initial weights */
+void tb_net_destroy(tb_net *net);
+const char *tb_net_last_error(tb_net *net);
+int64_t tb_net_param_count(void);
+int tb_net_get_weights(tb_net *net, float *host, int64_t n);
+int tb_net_set_weights(tb_net *net, const float *host, int64_t n);
+int tb_net_attach(tb_ctx *ctx, tb_net *net);     /* NULL detaches */
+/* guess for the current PRHS and solution history into field f_out */
+int tb_net_predict_field(tb_ctx *ctx, int f_out);
+/* training data: the solver's own (b_n, phi_{n-1}, phi_{n-2}) -> phi_n
+ * pairs, taken after each solve of a context on an nx*ny*nz cell grid */
+int tb_net_reserve(tb_net *net, int capacity, int nx, int ny, int nz);
+int tb_net_add_snapshot(tb_net *net, tb_ctx *ctx);
+/* from now on every Poisson solve of ctx adds its snapshot (until the store
+ * is full); NULL stops */
+int tb_net_record(tb_ctx *ctx, tb_net *net);
+int tb_net_snapshot_count(tb_net *net);
+/* Adam on random (cx,cy,cz) crops, 'batch' crops per step; mean-square
+ * loss of the first and the last step */
+int tb_net_train(tb_net *net, int steps, int batch, int cx, int cy, int cz, double lr, uint64_t seed,
+                 double *loss_first, double *loss_last);
+int tb_net_eval_snapshots(tb_net *net, int first, int count, double *mean_loss);
+/* host-buffer forward / loss gradient on an nx*ny*nz grid (in3: x fastest,
+ * 3 channels per voxel), for checks against a CPU restatement */
+int tb_net_forward(tb_net *net, const float *in3, int nx, int ny, int nz, float *out);
+int tb_net_gradient(tb_net *net, const float *in3, const float *target, int nx, int ny, int nz,
+                    double *loss, float *grad);
+
 /* live per-kernel timing with HIP events on the context's stream (no
  * reference counterpart; replaces PETSc's -log_summary for this path) */
 enum {

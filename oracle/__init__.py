@@ -61,6 +61,9 @@ def lib():
         L.ora_poisson_matrix.argtypes = [P, P, P, P]
         L.ora_set_geometry.argtypes = [P, C.c_int, P, P, C.c_int]
         L.ora_gmg_vcycle.argtypes = [P, P, P]
+        L.ora_net_param_count.restype = C.c_int64
+        L.ora_net_forward.argtypes = [P, P, C.c_int, C.c_int, C.c_int, C.c_int, P]
+        L.ora_net_gradient.argtypes = [P, P, P, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double), P]
         _LIB = L
     return _LIB
 
@@ -239,3 +242,33 @@ def gauss_jordan(a):
     out = np.zeros_like(a)
     lib().ora_gauss_jordan(n, _ptr(a), _ptr(out))
     return out
+
+
+def net_param_count():
+    return int(lib().ora_net_param_count())
+
+
+def net_forward(w, in3, round_bf16=True):
+    """in3: (nz, ny, nx, 3) -> out (nz, ny, nx)"""
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    in3 = np.ascontiguousarray(in3, dtype=np.float64)
+    nz, ny, nx, _ = in3.shape
+    out = np.zeros((nz, ny, nx))
+    rc = lib().ora_net_forward(_ptr(w), _ptr(in3), nx, ny, nz, int(round_bf16), _ptr(out))
+    if rc:
+        raise ValueError("network dimensions must be multiples of 4")
+    return out
+
+
+def net_gradient(w, in3, target, round_bf16=True):
+    """mean-square loss and its gradient with respect to w"""
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    in3 = np.ascontiguousarray(in3, dtype=np.float64)
+    target = np.ascontiguousarray(target, dtype=np.float64)
+    nz, ny, nx, _ = in3.shape
+    g = np.zeros_like(w)
+    loss = C.c_double()
+    rc = lib().ora_net_gradient(_ptr(w), _ptr(in3), _ptr(target), nx, ny, nz, int(round_bf16), C.byref(loss), _ptr(g))
+    if rc:
+        raise ValueError("network dimensions must be multiples of 4")
+    return loss.value, g

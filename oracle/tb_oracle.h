@@ -67,6 +67,14 @@ int ora_step(ora_ctx *o, double dt, tb_step_stats *st);
 int ora_gmg_levels(ora_ctx *o, int32_t *dims, int cap);
 int ora_gmg_vcycle(ora_ctx *o, const double *r_compact, double *z_compact);
 
+/* learned-guess network (ora_net.c): parameters w (ora_net_param_count
+ * doubles, the device layout), in3 [nz][ny][nx][3], out/target [nz][ny][nx];
+ * round = 1 reproduces the device's BF16/FP32 rounding points */
+int64_t ora_net_param_count(void);
+int ora_net_forward(const double *w, const double *in3, int nx, int ny, int nz, int round, double *out);
+int ora_net_gradient(const double *w, const double *in3, const double *target, int nx, int ny, int nz, int round,
+                     double *loss, double *grad);
+
 #ifdef __cplusplus
 }
 #endif

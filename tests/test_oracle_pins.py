@@ -385,3 +385,100 @@ def test_oracle_step_runs_and_converges():
         st = o.step(c["dt"])
     assert all(p.converged for p in st.pois)
     assert all(h.converged for stage in st.helm for h in stage)
+
+
+# ---------------------------------------------------------------------------
+# learned-guess network: oracle (exact mode) against torch autograd (FP64)
+# and finite differences
+
+NET_LAYERS = [(16, 16), (16, 32), (32, 32), (32, 64), (64, 64), (96, 32), (48, 16), (16, 1)]
+
+
+def _net_params(seed=0, scale=1.0):
+    rng = np.random.default_rng(seed)
+    w = []
+    for ci, co in NET_LAYERS:
+        a = np.sqrt(6.0 / (27 * ci)) * scale
+        w.append(rng.uniform(-a, a, 27 * ci * co))
+        w.append(rng.uniform(-0.1, 0.1, co))
+    w = np.concatenate(w)
+    assert w.size == oracle.net_param_count()
+    return w
+
+
+def _torch_net(w, in3):
+    import torch
+    import torch.nn.functional as F
+    ws, o = [], 0
+    for ci, co in NET_LAYERS:
+        W = torch.tensor(w[o:o + 27 * ci * co].reshape(3, 3, 3, ci, co)).permute(4, 3, 0, 1, 2).contiguous()
+        o += 27 * ci * co
+        b = torch.tensor(w[o:o + co])
+        o += co
+        W.requires_grad_(True)
+        b.requires_grad_(True)
+        ws.append((W, b))
+    x = torch.zeros((1, 16) + in3.shape[:3], dtype=torch.float64)
+    x[0, :3] = torch.tensor(in3).permute(3, 0, 1, 2)
+
+    def cv(l, t, relu=True):
+        y = F.conv3d(t, ws[l][0], ws[l][1], padding=1)
+        return F.relu(y) if relu else y
+
+    up = lambda t: F.interpolate(t, scale_factor=2, mode="nearest")
+    x0 = cv(0, x)
+    x1 = cv(1, F.avg_pool3d(x0, 2))
+    x1b = cv(2, x1)
+    x2 = cv(3, F.avg_pool3d(x1b, 2))
+    x2b = cv(4, x2)
+    y1 = cv(5, torch.cat([up(x2b), x1b], 1))
+    y0 = cv(6, torch.cat([up(y1), x0], 1))
+    out = cv(7, y0, relu=False)[0, 0]
+    return out, ws
+
+
+def test_net_oracle_matches_torch_forward_and_gradient():
+    import torch
+    rng = np.random.default_rng(1)
+    nz, ny, nx = 8, 8, 12
+    in3 = rng.standard_normal((nz, ny, nx, 3))
+    tgt = rng.standard_normal((nz, ny, nx))
+    w = _net_params(2)
+    out_o = oracle.net_forward(w, in3, round_bf16=False)
+    loss_o, g_o = oracle.net_gradient(w, in3, tgt, round_bf16=False)
+    out_t, ws = _torch_net(w, in3)
+    np.testing.assert_allclose(out_o, out_t.detach().numpy(), rtol=1e-11, atol=1e-12)
+    loss_t = ((out_t - torch.tensor(tgt)) ** 2).mean()
+    loss_t.backward()
+    assert abs(loss_o - loss_t.item()) < 1e-12 * abs(loss_t.item())
+    g_t = np.concatenate([np.concatenate([W.grad.permute(2, 3, 4, 1, 0).reshape(-1).numpy(), b.grad.numpy()])
+                          for W, b in ws])
+    np.testing.assert_allclose(g_o, g_t, rtol=1e-9, atol=1e-13)
+
+
+def test_net_oracle_gradient_finite_differences():
+    rng = np.random.default_rng(3)
+    nz, ny, nx = 4, 8, 8
+    in3 = rng.standard_normal((nz, ny, nx, 3))
+    tgt = rng.standard_normal((nz, ny, nx))
+    w = _net_params(4)
+    _, g = oracle.net_gradient(w, in3, tgt, round_bf16=False)
+    for q in rng.choice(w.size, 12, replace=False):
+        h = 1e-6
+        wp, wm = w.copy(), w.copy()
+        wp[q] += h
+        wm[q] -= h
+        lp, _ = oracle.net_gradient(wp, in3, tgt, round_bf16=False)
+        lm, _ = oracle.net_gradient(wm, in3, tgt, round_bf16=False)
+        fd = (lp - lm) / (2 * h)
+        assert abs(fd - g[q]) <= 1e-6 * max(1.0, abs(g[q])) + 1e-9, (q, fd, g[q])
+
+
+def test_net_oracle_rounding_mode_is_close_to_exact():
+    rng = np.random.default_rng(5)
+    in3 = rng.standard_normal((8, 8, 8, 3))
+    w = _net_params(6)
+    a = oracle.net_forward(w, in3, round_bf16=False)
+    b = oracle.net_forward(w, in3, round_bf16=True)
+    assert np.linalg.norm(a - b) / np.linalg.norm(a) < 3e-2
+    assert not np.array_equal(a, b)

@@ -64,6 +64,25 @@ def lib():
     L.tb_set_helmholtz_rtol.argtypes = [P, C.c_double]
     L.tb_profile_enable.argtypes = [P, C.c_int]
     L.tb_profile_read.argtypes = [P, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
+    L.tb_net_create.restype = P
+    L.tb_net_create.argtypes = [C.c_uint64]
+    L.tb_net_destroy.argtypes = [P]
+    L.tb_net_last_error.restype = C.c_char_p
+    L.tb_net_last_error.argtypes = [P]
+    L.tb_net_param_count.restype = C.c_int64
+    L.tb_net_get_weights.argtypes = [P, P, C.c_int64]
+    L.tb_net_set_weights.argtypes = [P, P, C.c_int64]
+    L.tb_net_attach.argtypes = [P, P]
+    L.tb_net_predict_field.argtypes = [P, C.c_int]
+    L.tb_net_reserve.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_int]
+    L.tb_net_add_snapshot.argtypes = [P, P]
+    L.tb_net_record.argtypes = [P, P]
+    L.tb_net_snapshot_count.argtypes = [P]
+    L.tb_net_train.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_uint64,
+                               C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.tb_net_eval_snapshots.argtypes = [P, C.c_int, C.c_int, C.POINTER(C.c_double)]
+    L.tb_net_forward.argtypes = [P, P, C.c_int, C.c_int, C.c_int, P]
+    L.tb_net_gradient.argtypes = [P, P, P, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double), P]
     _LIB = L
     return L
 
@@ -74,7 +93,7 @@ def exported_symbols():
     import re
     names = []
     for line in open(hdr):
-        m = re.match(r"^(?:int|void|const char \*|tb_ctx \*)\s*\*?(tb_\w+)\(", line.strip())
+        m = re.match(r"^(?:int|int64_t|void|const char \*|tb_ctx \*|tb_net \*)\s*\*?(tb_\w+)\(", line.strip())
         if m:
             names.append(m.group(1))
     return names
@@ -214,3 +233,100 @@ class Solver:
         ms, n = C.c_double(), C.c_int64()
         self._ok(self.L.tb_profile_read(self.h, kernel, C.byref(ms), C.byref(n)), "tb_profile_read")
         return ms.value, n.value
+
+    def attach_net(self, net):
+        """Use `net` (a Net) for the learned initial guess; None detaches."""
+        self._net = net
+        return self._ok(self.L.tb_net_attach(self.h, net.h if net is not None else None), "tb_net_attach")
+
+    def record_snapshots(self, net):
+        """Every later Poisson solve adds its snapshot to net's store."""
+        return self._ok(self.L.tb_net_record(self.h, net.h if net is not None else None), "tb_net_record")
+
+    def predict_guess(self, name="SCRATCH0"):
+        """Learned guess for the current PRHS and pressure history -> field."""
+        self._ok(self.L.tb_net_predict_field(self.h, abi.F[name]), "tb_net_predict_field")
+        return self.download(name)
+
+
+class Net:
+    """Learned Poisson initial guess: weights, snapshot store and training,
+    all on the device (tb_net_* entry points)."""
+
+    def __init__(self, seed=0):
+        self.L = lib()
+        self.h = self.L.tb_net_create(seed)
+        if not self.h:
+            raise TBError("tb_net_create failed")
+        self.np = int(self.L.tb_net_param_count())
+
+    def _ok(self, rc, what):
+        if rc < 0:
+            raise TBError(f"{what}: {self.L.tb_net_last_error(self.h).decode()}")
+        return rc
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.tb_net_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_weights(self):
+        w = np.zeros(self.np, dtype=np.float32)
+        self._ok(self.L.tb_net_get_weights(self.h, _ptr(w), self.np), "tb_net_get_weights")
+        return w
+
+    def set_weights(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        if w.size != self.np:
+            raise ValueError(f"expected {self.np} weights, got {w.size}")
+        self._ok(self.L.tb_net_set_weights(self.h, _ptr(w), self.np), "tb_net_set_weights")
+
+    def save(self, path):
+        self.get_weights().tofile(path)
+
+    def load(self, path):
+        self.set_weights(np.fromfile(path, dtype=np.float32))
+
+    def reserve(self, capacity, nx, ny, nz):
+        self._ok(self.L.tb_net_reserve(self.h, capacity, nx, ny, nz), "tb_net_reserve")
+
+    def add_snapshot(self, solver):
+        self._ok(self.L.tb_net_add_snapshot(self.h, solver.h), "tb_net_add_snapshot")
+
+    def snapshot_count(self):
+        return int(self.L.tb_net_snapshot_count(self.h))
+
+    def train(self, steps, batch, crop, lr=1e-3, seed=0):
+        l0, l1 = C.c_double(), C.c_double()
+        self._ok(self.L.tb_net_train(self.h, steps, batch, crop[0], crop[1], crop[2], lr, seed,
+                                     C.byref(l0), C.byref(l1)), "tb_net_train")
+        return l0.value, l1.value
+
+    def eval_snapshots(self, first, count):
+        v = C.c_double()
+        self._ok(self.L.tb_net_eval_snapshots(self.h, first, count, C.byref(v)), "tb_net_eval_snapshots")
+        return v.value
+
+    def forward(self, in3):
+        """in3: (nz, ny, nx, 3) float32 -> (nz, ny, nx)"""
+        in3 = np.ascontiguousarray(in3, dtype=np.float32)
+        nz, ny, nx, _ = in3.shape
+        out = np.zeros((nz, ny, nx), dtype=np.float32)
+        self._ok(self.L.tb_net_forward(self.h, _ptr(in3), nx, ny, nz, _ptr(out)), "tb_net_forward")
+        return out
+
+    def gradient(self, in3, target):
+        in3 = np.ascontiguousarray(in3, dtype=np.float32)
+        target = np.ascontiguousarray(target, dtype=np.float32)
+        nz, ny, nx, _ = in3.shape
+        g = np.zeros(self.np, dtype=np.float32)
+        loss = C.c_double()
+        self._ok(self.L.tb_net_gradient(self.h, _ptr(in3), _ptr(target), nx, ny, nz, C.byref(loss), _ptr(g)),
+                 "tb_net_gradient")
+        return loss.value, g

@@ -165,7 +165,6 @@ struct Level {
 };
 
 // ---------------------------------------------------------------------------
-struct NetState;    // learned initial guess (tb_net.hip)
 
 struct tb_ctx {
     tb_config cfg;
@@ -212,9 +211,10 @@ struct tb_ctx {
     bool mg_built = false;
 
     // Poisson history for the learned guess
-    double *phi_hist[2] = {};
+    double *phi_hist[3] = {};    // last three Poisson solutions, newest first
     int n_hist = 0;
-    NetState *net = nullptr;
+    tb_net *net = nullptr;       // attached learned guess (not owned)
+    tb_net *net_record = nullptr;    // snapshot sink after every solve (not owned)
 
     // live per-kernel timing (event pairs around selected launches)
     bool prof_on = false;
@@ -270,6 +270,6 @@ int tb_mg_build(tb_ctx *c);
 void tb_mg_free(tb_ctx *c);
 int tb_vcycle64(tb_ctx *c, const double *r_pad, double *z_pad);
 int tb_vcycle32(tb_ctx *c);
-int tb_net_predict(tb_ctx *c, double *phi_out);
-void tb_net_free(tb_ctx *c);
+int tb_net_predict(tb_ctx *c, double *guess);
+int tb_net_record_one(tb_ctx *c);
 double tb_elapsed_ms(tb_ctx *c);

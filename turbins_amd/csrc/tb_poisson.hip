@@ -911,9 +911,15 @@ extern "C" int tb_poisson_solve(tb_ctx *c, tb_solve_stats *st) {
         loc.converged = res <= rtol;
     }
     // history of solutions for the learned guess
-    TB_CHECK(c, hipMemcpyAsync(c->phi_hist[1], c->phi_hist[0], sizeof(double) * c->d.ntot, hipMemcpyDeviceToDevice, c->s));
+    {
+        double *oldest = c->phi_hist[2];
+        c->phi_hist[2] = c->phi_hist[1];
+        c->phi_hist[1] = c->phi_hist[0];
+        c->phi_hist[0] = oldest;
+    }
     TB_CHECK(c, hipMemcpyAsync(c->phi_hist[0], x, sizeof(double) * c->d.ntot, hipMemcpyDeviceToDevice, c->s));
-    if (c->n_hist < 2) c->n_hist++;
+    if (c->n_hist < 3) c->n_hist++;
+    if (c->net_record && c->n_hist >= 3 && tb_net_record_one(c)) return -1;
     TB_CHECK(c, hipEventRecord(c->ev1, c->s));
     TB_CHECK(c, hipStreamSynchronize(c->s));
     loc.iterations = its;
