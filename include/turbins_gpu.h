@@ -240,6 +240,13 @@ int64_t tb_net_param_count(void);
 int tb_net_get_weights(tb_net *net, float *host, int64_t n);
 int tb_net_set_weights(tb_net *net, const float *host, int64_t n);
 int tb_net_attach(tb_ctx *ctx, tb_net *net);     /* NULL detaches */
+/* The learned mode starts each solve from the Galerkin (A-norm optimal)
+ * combination of phi_{n-1}, phi_{n-2} and the network's correction.  The
+ * first 4 solves with the network estimate the iterations its direction
+ * saves (energy of the guesses with and without it); if those iterations
+ * cost less time than the inference, the context stops calling the network
+ * (tb_net_attach re-opens).  This reports that judgement.                  */
+int tb_net_gate(tb_ctx *ctx, int *open, int *trials, double *saved_iters_per_solve, double *net_ms_per_call);
 /* guess for the current PRHS and solution history into field f_out */
 int tb_net_predict_field(tb_ctx *ctx, int f_out);
 /* training data: the solver's own (b_n, phi_{n-1}, phi_{n-2}) -> phi_n

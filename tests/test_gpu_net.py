@@ -117,3 +117,29 @@ def test_snapshots_training_and_guess():
     assert np.isfinite(guess).all()
     g.close()
     n.close()
+
+
+def test_trained_network_gain():
+    """What the committed network gains over the method without it, on a
+    held-out 256x128x128 run (bump height / lock position drawn with seeds
+    the training never used): Poisson iterations and initial residuals
+    starting from phi_{n-1} (the reference's guess), from the Galerkin
+    combination of phi_{n-1}, phi_{n-2}, and from that combination plus the
+    network's direction.  The learned path must never cost iterations, and
+    the run-time gate must close exactly when the saved iterations are worth
+    less than the inference time."""
+    import os
+    from turbins_amd import learn
+    n = Net(seed=0)
+    n.load(learn.DEFAULT_WEIGHTS)
+    res = learn.compare_iterations(n, "learn256", steps=10, seed=2024)
+    n.close()
+    prev, sub, lrn = res["previous"], res["subspace"], res["learned"]
+    print("iterations previous / subspace / learned:", prev["iterations_mean"], sub["iterations_mean"],
+          lrn["iterations_mean"], "gate", lrn["gate"])
+    assert lrn["iterations_mean"] <= prev["iterations_mean"]
+    assert lrn["res0_median"] <= prev["res0_median"]
+    g = lrn["gate"]
+    assert g["trials"] == 4
+    worth = g["saved_iterations_per_solve"] * (lrn["solve_ms_mean"] / max(lrn["iterations_mean"], 1))
+    assert g["open"] == (worth >= g["net_ms_per_call"]) or abs(worth - g["net_ms_per_call"]) < 0.05

@@ -73,6 +73,8 @@ def lib():
     L.tb_net_get_weights.argtypes = [P, P, C.c_int64]
     L.tb_net_set_weights.argtypes = [P, P, C.c_int64]
     L.tb_net_attach.argtypes = [P, P]
+    L.tb_net_gate.argtypes = [P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double),
+                              C.POINTER(C.c_double)]
     L.tb_net_predict_field.argtypes = [P, C.c_int]
     L.tb_net_reserve.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_int]
     L.tb_net_add_snapshot.argtypes = [P, P]
@@ -238,6 +240,13 @@ class Solver:
         """Use `net` (a Net) for the learned initial guess; None detaches."""
         self._net = net
         return self._ok(self.L.tb_net_attach(self.h, net.h if net is not None else None), "tb_net_attach")
+
+    def net_gate(self):
+        """(open, trials, iterations saved per solve, inference ms per call)"""
+        o, t, sv, ms = C.c_int(), C.c_int(), C.c_double(), C.c_double()
+        self._ok(self.L.tb_net_gate(self.h, C.byref(o), C.byref(t), C.byref(sv), C.byref(ms)), "tb_net_gate")
+        return {"open": bool(o.value), "trials": t.value, "saved_iterations_per_solve": sv.value,
+                "net_ms_per_call": ms.value}
 
     def record_snapshots(self, net):
         """Every later Poisson solve adds its snapshot to net's store."""

@@ -213,8 +213,18 @@ struct tb_ctx {
     // Poisson history for the learned guess
     double *phi_hist[3] = {};    // last three Poisson solutions, newest first
     int n_hist = 0;
+    double *b_hist[2] = {};      // their right-hand sides (learned mode only)
+    int nb_hist = 0;
     tb_net *net = nullptr;       // attached learned guess (not owned)
     tb_net *net_record = nullptr;    // snapshot sink after every solve (not owned)
+    // run-time judgement of the network: iterations its direction saves
+    // (from the energy of the 2- and 3-direction guesses) against its cost
+    bool net_open = true;
+    int net_trials = 0;
+    double net_saved = 0.0, net_cost_ms = 0.0, net_iter_ms = 0.0;
+    double net_J2 = 0.0, net_J3 = 0.0;
+    bool net_used = false;
+    hipEvent_t net_ev[2] = {nullptr, nullptr};
 
     // live per-kernel timing (event pairs around selected launches)
     bool prof_on = false;
@@ -240,7 +250,7 @@ int tb_prof_flush(tb_ctx *c);
 // device scalar slots (ctx->scal)
 enum {
     S_RHO = 0, S_ALPHA, S_OMEGA, S_BETA, S_RZ, S_PQ, S_RR, S_TS, S_TT, S_RHV,
-    S_BB, S_MEAN, S_CORNER, S_TMP0, S_TMP1, S_TMP2, S_COUNT = 32
+    S_BB, S_MEAN, S_CORNER, S_TMP0, S_TMP1, S_TMP2, S_SUB = 16, S_COUNT = 32
 };
 
 // error helpers -------------------------------------------------------------
@@ -270,6 +280,6 @@ int tb_mg_build(tb_ctx *c);
 void tb_mg_free(tb_ctx *c);
 int tb_vcycle64(tb_ctx *c, const double *r_pad, double *z_pad);
 int tb_vcycle32(tb_ctx *c);
-int tb_net_predict(tb_ctx *c, double *guess);
+int tb_net_predict(tb_ctx *c, double *out, int correction_only);
 int tb_net_record_one(tb_ctx *c);
 double tb_elapsed_ms(tb_ctx *c);

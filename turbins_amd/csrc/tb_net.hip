@@ -531,16 +531,17 @@ __global__ void k_pack_input(Dims d, int ncx, int ncy, int ncz, const double *__
     }
 }
 
-// guess = phi1 + s_p * out   on cells
+// guess = phi1 + s_p * out on cells (only s_p * out when correction_only)
 __global__ void k_unpack_guess(Dims d, int ncx, int ncy, int ncz, const float *__restrict__ out,
-                               const double *__restrict__ p1, const double *__restrict__ ss, double *__restrict__ guess) {
+                               const double *__restrict__ p1, const double *__restrict__ ss, int correction_only,
+                               double *__restrict__ guess) {
     const long n = (long)ncx * ncy * ncz;
     const double sp = 1.0 / inv_rms(ss[1], n);
     for (long v = blockIdx.x * (long)blockDim.x + threadIdx.x; v < n; v += (long)gridDim.x * blockDim.x) {
         int x, y, z;
         vox_xyz(v, ncx, ncy, x, y, z);
         const long id = gidx(d, x, y, z);
-        guess[id] = p1[id] + sp * (double)out[v];
+        guess[id] = (correction_only ? 0.0 : p1[id]) + sp * (double)out[v];
     }
 }
 
@@ -1141,12 +1142,18 @@ int tb_net_record_one(tb_ctx *c) {
 extern "C" int tb_net_attach(tb_ctx *c, tb_net *net) {
     if (!c) return -1;
     if (net && (c->ncx % 4 || c->ncy % 4 || c->ncz % 4)) { c->err = "network needs cell counts divisible by 4"; return -1; }
+    for (int e = 0; e < 2; e++)
+        if (!c->net_ev[e]) TB_CHECK(c, hipEventCreate(&c->net_ev[e]));
     c->net = net;
+    c->net_open = true;
+    c->net_trials = 0;
+    c->net_saved = c->net_cost_ms = c->net_iter_ms = 0.0;
     return 0;
 }
 
-// guess from PRHS, phi_{n-1} = hist[0], phi_{n-2} = hist[1], on the context's stream
-int tb_net_predict(tb_ctx *c, double *guess) {
+// guess from PRHS, phi_{n-1} = hist[0], phi_{n-2} = hist[1], on the
+// context's stream; cells only (the caller owns the other entries)
+int tb_net_predict(tb_ctx *c, double *guess, int correction_only) {
     tb_net *net = c->net;
     if (!net || c->n_hist < 2) return -1;
     if (acts_ensure(net, net->inf, c->ncx, c->ncy, c->ncz, false)) { c->err = net->err; return -1; }
@@ -1159,7 +1166,8 @@ int tb_net_predict(tb_ctx *c, double *guess) {
                        (const double *)c->phi_hist[1], (const double *)(c->scal + S_TMP1), A.in16);
     if (forward(net, c->s, A)) { c->err = net->err; return -1; }
     hipLaunchKernelGGL(k_unpack_guess, grid1d(A.v0), dim3(256), 0, c->s, c->d, c->ncx, c->ncy, c->ncz,
-                       (const float *)A.out, (const double *)c->phi_hist[0], (const double *)(c->scal + S_TMP1), guess);
+                       (const float *)A.out, (const double *)c->phi_hist[0], (const double *)(c->scal + S_TMP1),
+                       correction_only, guess);
     if (c->prof_on) tb_prof_end(c, TB_K_NET);
     TB_LAUNCH_CHECK(c);
     return 0;
@@ -1170,7 +1178,7 @@ extern "C" int tb_net_predict_field(tb_ctx *c, int f_out) {
     if (!c->net) { c->err = "no network attached"; return -1; }
     if (c->n_hist < 2) { c->err = "need two solved pressures"; return -1; }
     TB_CHECK(c, hipMemsetAsync(c->f[f_out], 0, sizeof(double) * c->d.ntot, c->s));
-    if (tb_net_predict(c, c->f[f_out])) return -1;
+    if (tb_net_predict(c, c->f[f_out], 0)) return -1;
     TB_CHECK(c, hipStreamSynchronize(c->s));
     return 0;
 }

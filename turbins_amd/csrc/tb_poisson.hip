@@ -10,6 +10,7 @@
 // preconditioner, then shifts the result so the pinned cell holds 0.  For a
 // compatible right-hand side (sum b = 0, which the divergence RHS satisfies
 // up to rounding) that is the unique solution of the pinned system.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -858,6 +859,170 @@ static int mixed_ir(tb_ctx *c, const PoisCtx &P, double *x, const double *b, dou
     return 0;
 }
 
+// ---------------------------------------------------------------------------
+// Initial guess of the learned mode: the A-norm-optimal combination
+//   x0 = a0 phi_{n-1} + a1 phi_{n-2} + a2 d
+// of the last two solutions and the network's correction d (when a network
+// is attached), i.e. the Galerkin solution of A x = b on that subspace.
+// A phi_{n-k} = b_{n-k} holds to the solve tolerance, so only d needs an
+// operator application.  Returns the true relative residual of x0 (pinned
+// operator) in resg and the number of directions used in ndir (0: none
+// usable; the caller then restores phi_{n-1}).
+
+// the 12 sums g_i . A g_j (i, j < 3) and g_i . b
+__global__ void __launch_bounds__(PT) k_subspace_dots(long n, const double *__restrict__ h0, const double *__restrict__ h1,
+                                                      const double *__restrict__ dd, const double *__restrict__ b0,
+                                                      const double *__restrict__ b1, const double *__restrict__ q,
+                                                      const double *__restrict__ b, int use_d, double *part) {
+    __shared__ double sh[12 * 16];
+    double a[12];
+#pragma unroll
+    for (int k = 0; k < 12; k++) a[k] = 0.0;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const double g0 = h0[i], g1 = h1[i], g2 = use_d ? dd[i] : 0.0;
+        const double A0 = b0[i], A1 = b1[i], A2 = use_d ? q[i] : 0.0, bi = b[i];
+        a[0] += g0 * A0; a[1] += g0 * A1; a[2] += g0 * A2;
+        a[3] += g1 * A0; a[4] += g1 * A1; a[5] += g1 * A2;
+        a[6] += g2 * A0; a[7] += g2 * A1; a[8] += g2 * A2;
+        a[9] += g0 * bi; a[10] += g1 * bi; a[11] += g2 * bi;
+    }
+    block_sum<12>(a, sh);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int k = 0; k < 12; k++) part[(long)blockIdx.x * 12 + k] = a[k];
+}
+
+// out[k] = sum over blocks of part[blk * ncol + k], fixed order
+__global__ void k_sum_cols(const double *part, int nblk, int ncol, double *out) {
+    const int k = threadIdx.x;
+    if (k >= ncol) return;
+    double s = 0.0;
+    for (int bI = 0; bI < nblk; bI++) s += part[(long)bI * ncol + k];
+    out[k] = s;
+}
+
+__global__ void k_combine3(long n, double *__restrict__ x, const double *__restrict__ h0, const double *__restrict__ h1,
+                           const double *__restrict__ dd, double a0, double a1, double a2) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        x[i] = a0 * h0[i] + a1 * h1[i] + (a2 != 0.0 ? a2 * dd[i] : 0.0);
+}
+
+// solve the m x m system M a = r by Gaussian elimination with partial
+// pivoting; false when a pivot falls below 1e-12 of the largest entry
+static bool small_solve(int m, double M[3][3], double r[3], double a[3]) {
+    double A[3][4];
+    double mx = 0.0;
+    for (int i = 0; i < m; i++) {
+        for (int j = 0; j < m; j++) { A[i][j] = M[i][j]; mx = std::max(mx, std::fabs(M[i][j])); }
+        A[i][m] = r[i];
+    }
+    if (!(mx > 0.0)) return false;
+    for (int col = 0; col < m; col++) {
+        int piv = col;
+        for (int i = col + 1; i < m; i++)
+            if (std::fabs(A[i][col]) > std::fabs(A[piv][col])) piv = i;
+        if (!(std::fabs(A[piv][col]) > 1e-12 * mx)) return false;
+        for (int j = 0; j <= m; j++) std::swap(A[col][j], A[piv][j]);
+        for (int i = 0; i < m; i++) {
+            if (i == col) continue;
+            const double f = A[i][col] / A[col][col];
+            for (int j = col; j <= m; j++) A[i][j] -= f * A[col][j];
+        }
+    }
+    for (int i = 0; i < m; i++) a[i] = A[i][m] / A[i][i];
+    return true;
+}
+
+static int learned_guess(tb_ctx *c, const PoisCtx &P, double *x, const double *b, double bn, double &resg, int &ndir) {
+    const long n = c->d.ntot;
+    double *dd = c->kv[5], *q = c->kv[6];
+    ndir = 0;
+    int use_d = 0;
+    c->net_used = false;
+    if (c->net && c->net_open) {
+        TB_CHECK(c, hipMemsetAsync(dd, 0, sizeof(double) * n, c->s));
+        TB_CHECK(c, hipEventRecord(c->net_ev[0], c->s));
+        if (tb_net_predict(c, dd, 1)) return -1;
+        TB_CHECK(c, hipEventRecord(c->net_ev[1], c->s));
+        hipLaunchKernelGGL(k_pois_km<0>, ktiles_grid(P.kt), tiles_block(), 0, c->s, P.Pn, (const double *)dd,
+                           (const double *)nullptr, q, (float *)nullptr, c->red, P.kt);
+        TB_LAUNCH_CHECK(c);
+        use_d = 1;
+    }
+    hipLaunchKernelGGL(k_subspace_dots, dim3(PB), dim3(PT), 0, c->s, n, (const double *)c->phi_hist[0],
+                       (const double *)c->phi_hist[1], (const double *)dd, (const double *)c->b_hist[0],
+                       (const double *)c->b_hist[1], (const double *)q, b, use_d, c->red);
+    hipLaunchKernelGGL(k_sum_cols, dim3(1), dim3(64), 0, c->s, (const double *)c->red, PB, 12, c->scal + S_SUB);
+    TB_LAUNCH_CHECK(c);
+    if (read_s(c, S_SUB, 12)) return -1;
+    const double *h = c->hscal + S_SUB;
+    double M[3][3], r[3] = {h[9], h[10], h[11]}, a[3] = {0.0, 0.0, 0.0};
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) M[i][j] = 0.5 * (h[3 * i + j] + h[3 * j + i]);
+    for (int m = use_d ? 3 : 2; m >= 1; m--) {
+        if (small_solve(m, M, r, a)) { ndir = m; break; }
+        a[0] = a[1] = a[2] = 0.0;
+    }
+    if (ndir == 0) return 0;
+    if (ndir == 3) {
+        // energies J = -a.r/2 of the Galerkin guesses with and without d
+        double a2[3] = {0.0, 0.0, 0.0};
+        if (small_solve(2, M, r, a2)) {
+            c->net_J2 = -0.5 * (a2[0] * r[0] + a2[1] * r[1]);
+            c->net_J3 = -0.5 * (a[0] * r[0] + a[1] * r[1] + a[2] * r[2]);
+            c->net_used = true;
+        }
+    }
+    hipLaunchKernelGGL(k_combine3, dim3(PB), dim3(PT), 0, c->s, n, x, (const double *)c->phi_hist[0],
+                       (const double *)c->phi_hist[1], (const double *)dd, a[0], ndir > 1 ? a[1] : 0.0,
+                       ndir > 2 ? a[2] : 0.0);
+    TB_LAUNCH_CHECK(c);
+    double rr, bb;
+    if (shift_corner(c, P, x)) return -1;
+    if (residual(c, P, false, x, b, c->kv[0], nullptr, rr, bb)) return -1;
+    resg = sqrt(rr) / bn;
+    return 0;
+}
+
+// After a solve that used the network's direction: its A-norm error
+// reduction is sqrt(E2/E3) with E = 2 (J - J*), J* = -b.x/2 at the solution;
+// divided by the solve's own per-iteration reduction that gives the
+// iterations it saved.  After NET_TRIALS such solves the network stays on
+// only if the saved iterations are worth more time than its inference.
+constexpr int NET_TRIALS = 4;
+
+static int judge_network(tb_ctx *c, const double *x, const double *b, double res0, double res, int its,
+                         double solve_ms) {
+    if (!c->net_used || c->net_trials >= NET_TRIALS) return 0;
+    if (tb_dot(c, b, x, S_TMP0)) return -1;
+    if (read_s(c, S_TMP0, 1)) return -1;
+    float net_ms = 0.f;
+    TB_CHECK(c, hipEventElapsedTime(&net_ms, c->net_ev[0], c->net_ev[1]));
+    const double Js = -0.5 * c->hscal[S_TMP0];
+    const double E2 = std::max(2.0 * (c->net_J2 - Js), 1e-300), E3 = std::max(2.0 * (c->net_J3 - Js), 1e-300);
+    const double orders = 0.5 * std::log10(E2 / E3);
+    const double per_it = (its > 0 && res > 0.0 && res0 > res) ? std::log10(res0 / res) / its : 1.0;
+    c->net_saved += std::max(orders, 0.0) / per_it;
+    c->net_cost_ms += net_ms;
+    c->net_iter_ms += its > 0 ? (solve_ms - net_ms) / its : 0.0;
+    c->net_trials++;
+    if (c->net_trials == NET_TRIALS && c->net_saved * (c->net_iter_ms / NET_TRIALS) < c->net_cost_ms)
+        c->net_open = false;
+    return 0;
+}
+
+// state of that judgement: open (1) or closed (0), trials so far, mean
+// iterations saved per solve and mean inference time per call
+extern "C" int tb_net_gate(tb_ctx *c, int *open, int *trials, double *saved_per_solve, double *net_ms_per_call) {
+    if (!c) return -1;
+    const int t = c->net_trials;
+    if (open) *open = c->net_open ? 1 : 0;
+    if (trials) *trials = t;
+    if (saved_per_solve) *saved_per_solve = t ? c->net_saved / t : 0.0;
+    if (net_ms_per_call) *net_ms_per_call = t ? c->net_cost_ms / t : 0.0;
+    return 0;
+}
+
 extern "C" int tb_poisson_solve(tb_ctx *c, tb_solve_stats *st) {
     if (!c) return -1;
     if (tb_mg_build(c)) return -1;
@@ -882,20 +1047,16 @@ extern "C" int tb_poisson_solve(tb_ctx *c, tb_solve_stats *st) {
     } else {
         loc.res0 = sqrt(rr) / bn;
         int mode = c->cfg.pois_mode;
-        if (mode == TB_POIS_LEARNED_IR && c->net && c->n_hist >= 2) {
-            // learned initial guess: keep it only if it lowers the residual
-            double *guess = c->kv[5];
-            if (tb_net_predict(c, guess) == 0) {
-                double rg, bg;
-                if (shift_corner(c, P, guess)) return -1;
-                if (residual(c, P, false, guess, b, c->kv[0], nullptr, rg, bg)) return -1;
-                double resg = sqrt(rg) / bn;
-                if (resg < loc.res0) {
-                    TB_CHECK(c, hipMemcpyAsync(x, guess, sizeof(double) * c->d.ntot, hipMemcpyDeviceToDevice, c->s));
-                    loc.res0 = resg;
-                } else {
-                    loc.fallback = 2;   // guess rejected
-                }
+        if (mode == TB_POIS_LEARNED_IR && c->n_hist >= 2 && c->nb_hist >= 2) {
+            double resg = 0.0;
+            int ndir = 0;
+            if (learned_guess(c, P, x, b, bn, resg, ndir)) return -1;
+            if (ndir > 0 && resg < loc.res0) {
+                loc.res0 = resg;
+            } else {
+                // keep the previous solution (still in phi_hist[0])
+                TB_CHECK(c, hipMemcpyAsync(x, c->phi_hist[0], sizeof(double) * c->d.ntot, hipMemcpyDeviceToDevice, c->s));
+                loc.fallback = 2;
             }
         }
         if (mode == TB_POIS_FP64_PCG) {
@@ -909,6 +1070,11 @@ extern "C" int tb_poisson_solve(tb_ctx *c, tb_solve_stats *st) {
             }
         }
         loc.converged = res <= rtol;
+        if (c->net_used) {
+            TB_CHECK(c, hipEventRecord(c->ev1, c->s));
+            TB_CHECK(c, hipStreamSynchronize(c->s));
+            if (judge_network(c, x, b, loc.res0, res, its, tb_elapsed_ms(c))) return -1;
+        }
     }
     // history of solutions for the learned guess
     {
@@ -919,6 +1085,13 @@ extern "C" int tb_poisson_solve(tb_ctx *c, tb_solve_stats *st) {
     }
     TB_CHECK(c, hipMemcpyAsync(c->phi_hist[0], x, sizeof(double) * c->d.ntot, hipMemcpyDeviceToDevice, c->s));
     if (c->n_hist < 3) c->n_hist++;
+    if (c->cfg.pois_mode == TB_POIS_LEARNED_IR) {
+        std::swap(c->b_hist[0], c->b_hist[1]);
+        TB_CHECK(c, hipMemcpyAsync(c->b_hist[0], b, sizeof(double) * c->d.ntot, hipMemcpyDeviceToDevice, c->s));
+        if (c->nb_hist < 2) c->nb_hist++;
+    } else {
+        c->nb_hist = 0;
+    }
     if (c->net_record && c->n_hist >= 3 && tb_net_record_one(c)) return -1;
     TB_CHECK(c, hipEventRecord(c->ev1, c->s));
     TB_CHECK(c, hipStreamSynchronize(c->s));
