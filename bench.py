@@ -5,9 +5,13 @@ Workload: lock-exchange gravity current over an immersed Gaussian bump on a
 512 x 256 x 128 MAC grid (16 x 1 x 1 domain, Re = 2000, Sc = 1, 2nd-order
 ENO, dt = 1e-3), every Poisson solve to a 1e-8 relative residual, every
 Helmholtz solve to 1e-12.  Two device paths are timed on the same state:
-  plain   -- FP64 GMG-preconditioned CG
-  headline -- learned initial guess + FP32 V-cycles inside FP64 iterative
-             refinement (falls back to FP64 GMG-PCG if it stalls)
+  plain    -- FP64 GMG-preconditioned CG from the previous solution (the
+              reference's initial guess)
+  headline -- learned mode: Galerkin guess on phi_{n-1}, phi_{n-2} and the
+              trained network's direction (the network is dropped at run
+              time if its saved iterations cost more than its inference),
+              then FP32 CG with FP32 V-cycles inside FP64 iterative
+              refinement (FP64 GMG-PCG takes over if it stalls)
 The JSON line's `value` is the headline path's timesteps/s.  Inputs are
 resident in HBM before the timed region; each library step ends with a
 device synchronisation, so the wall clock brackets exactly K steps.
@@ -36,7 +40,9 @@ HBM_PEAK_GBS = 8000.0
 # its storage width); see DESIGN.md for the table
 BYTES_PER_POINT = {
     "ENO_MOM": 51, "ENO_CONC": 41, "PROJECT": 115, "BICG_UPD": 56, "PCG_UPD": 48, "POIS_OP": 16,
+    "CG32_OP": 8, "CG32_UPD": 24,
 }
+WEIGHTS = os.path.join(ROOT, "turbins_amd", "weights", "poisson_guess.f32")
 
 
 def log(*a):
@@ -77,6 +83,10 @@ def roofline(prof, stats, c):
             # 17 B/point (x, status, y) per launch; the rhat.v launch of each
             # BiCGStab iteration also reads rhat (8 B/point)
             total = 17 * NT * n + 8 * NT * helm_its
+        elif name == "HELM_GS":
+            # one red + one black launch: each reads x (8 B) and writes half of
+            # x (4 B), reads half of b (4 B) and of the status bytes (0.5 B)
+            total = 33 * NT * n
         elif name == "RBGS0":
             total = 16 * ncell * n      # x read, half of b read, half of x written
         elif name == "RBGS0_F32":
@@ -126,7 +136,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=4)
     ap.add_argument("--workload", default="bump512x256")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
@@ -149,17 +159,25 @@ def main():
 
     t_plain, st_plain, _ = run_path(g, c, args.steps, args.warmup, abi.POIS_FP64_PCG, 1e-8, False)
     plain_sps = args.steps / t_plain
-    learned = False
-    mode = abi.POIS_MIXED_IR
+    from turbins_amd import Net
+    net = None
+    if os.path.exists(WEIGHTS):
+        net = Net(seed=0)
+        net.load(WEIGHTS)
+        g.attach_net(net)
+    learned = net is not None
+    mode = abi.POIS_LEARNED_IR
     t_head, st_head, prof = run_path(g, c, args.steps, args.warmup, mode, 1e-8, True)
     head_sps = args.steps / t_head
+    gate = g.net_gate() if learned else None
 
     def pois_summary(stats):
         its = [p.iterations for st in stats for p in st.pois]
         res = [p.res for st in stats for p in st.pois]
         fb = sum(1 for st in stats for p in st.pois if p.fallback & 1)
+        r0 = [p.res0 for st in stats for p in st.pois]
         return {"iterations_mean": round(float(np.mean(its)), 2), "max_rel_residual": float(np.max(res)),
-                "fallbacks": fb}
+                "initial_residual_median": float(np.median(r0)), "fallbacks": fb}
 
     helm_its = [h.iterations for st in st_head for stage in st.helm for h in stage]
     log(f"plain: {plain_sps:.3f} steps/s ({1e3 / plain_sps:.1f} ms/step)  headline: {head_sps:.3f} steps/s "
@@ -190,8 +208,10 @@ def main():
             "cell_updates_per_s": round(head_sps * c["nx"] * c["ny"] * c["nz"], 1),
             "poisson_rtol": 1e-8,
             "helmholtz_rtol": 1e-12,
-            "headline_path": ("learned guess + " if learned else "") + "FP32 V-cycles in FP64 iterative refinement",
+            "headline_path": "Galerkin guess from phi_{n-1}, phi_{n-2}" + (" + network direction" if learned else "")
+                             + ", FP32 CG + FP32 V-cycles inside FP64 iterative refinement",
             "learned_guess_loaded": learned,
+            "network_gate": gate,
             "plain_path": "FP64 GMG-PCG",
             "plain_timesteps_per_s": round(plain_sps, 4),
             "plain_ms_per_step": round(1e3 / plain_sps, 3),

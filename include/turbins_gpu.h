@@ -224,6 +224,16 @@ int tb_step(tb_ctx *ctx, double dt, tb_step_stats *st);
  * Solver.c / Input.c at build time) */
 int tb_set_poisson_mode(tb_ctx *ctx, int mode, double rtol);
 int tb_set_helmholtz_rtol(tb_ctx *ctx, double rtol);
+/* Helmholtz solver: TB_HELM_GS (default) red-black Gauss-Seidel sweeps with
+ * BiCGStab taking over if they stop contracting; TB_HELM_BICGSTAB the
+ * reference's method alone (KSPIBCGS, Solver.c:39-86, 269-322).  Both stop
+ * at ||b - A x|| <= rtol ||b||. */
+#define TB_HELM_GS 0
+#define TB_HELM_BICGSTAB 1
+int tb_set_helmholtz_method(tb_ctx *ctx, int method);
+/* mixed Poisson mode: smallest residual reduction one FP32 inner solve is
+ * asked for (default 1e-5) and its iteration cap (default 40) */
+int tb_set_mixed_inner(tb_ctx *ctx, double reduction_floor, int maxit);
 
 /* learned initial guess of the Poisson solve (no reference counterpart: the
  * reference starts GMRES from the previous phi, Solver.c:256).  A 3-level
@@ -273,6 +283,7 @@ int tb_net_gradient(tb_net *net, const float *in3, const float *target, int nx, 
 enum {
     TB_K_ENO_MOM = 0, TB_K_ENO_CONC, TB_K_HELM_OP, TB_K_BICG_UPD, TB_K_POIS_OP,
     TB_K_RBGS0, TB_K_RBGS0_F32, TB_K_PCG_UPD, TB_K_PROJECT, TB_K_RHS, TB_K_NET,
+    TB_K_CG32_OP, TB_K_CG32_UPD, TB_K_HELM_GS,
     TB_K_COUNT
 };
 int tb_profile_enable(tb_ctx *ctx, int on);

@@ -5,9 +5,10 @@ Q_U, Q_V, Q_W, Q_C = 0, 1, 2, 3
 OUTSIDE, SOLID, FLUID, IMMERSED, BOUNDARY, BUFFER_SOLID = 1, 2, 3, 5, 7, 11
 SALINITY, PARTICLE = 1, 2
 POIS_FP64_PCG, POIS_MIXED_IR, POIS_LEARNED_IR = 0, 1, 2
+HELM_GS, HELM_BICGSTAB = 0, 1
 
 KERNELS = ["ENO_MOM", "ENO_CONC", "HELM_OP", "BICG_UPD", "POIS_OP", "RBGS0", "RBGS0_F32",
-           "PCG_UPD", "PROJECT", "RHS", "NET"]
+           "PCG_UPD", "PROJECT", "RHS", "NET", "CG32_OP", "CG32_UPD", "HELM_GS"]
 K = {name: i for i, name in enumerate(KERNELS)}
 
 FIELDS = [

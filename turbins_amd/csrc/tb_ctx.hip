@@ -115,7 +115,7 @@ extern "C" tb_ctx *tb_create(const tb_config *cfg) {
     for (int a = 0; ok && a < 2; a++) ok = dev_alloc(c, (void **)&c->b_hist[a], fb) == 0;
     ok = ok && dev_alloc(c, (void **)&c->red, sizeof(double) * RED_SLOTS * RED_MAX_BLOCKS) == 0;
     ok = ok && dev_alloc(c, (void **)&c->scal, sizeof(double) * S_COUNT) == 0;
-    ok = ok && hipHostMalloc((void **)&c->hscal, sizeof(double) * S_COUNT, hipHostMallocDefault) == hipSuccess;
+    ok = ok && hipHostMalloc((void **)&c->hscal, sizeof(double) * S_COUNT, hipHostMallocCoherent) == hipSuccess;
     ok = ok && hipStreamSynchronize(c->s) == hipSuccess;
     if (!ok) {
         tb_destroy(c);
@@ -269,6 +269,15 @@ extern "C" int tb_set_poisson_mode(tb_ctx *c, int mode, double rtol) {
     }
     c->cfg.pois_mode = mode;
     c->cfg.pois_rtol = rtol;
+    return 0;
+}
+
+extern "C" int tb_set_helmholtz_method(tb_ctx *c, int method) {
+    if (!c || (method != TB_HELM_GS && method != TB_HELM_BICGSTAB)) {
+        if (c) c->err = "tb_set_helmholtz_method: bad method";
+        return -1;
+    }
+    c->helm_method = method;
     return 0;
 }
 

@@ -190,7 +190,7 @@ __global__ void __launch_bounds__(256) k_eno_momentum(MomArgs a, Tiles t) {
             r += eno_line_point<R>(a.L[0].ax[1], u, id, j, vu);
             r += eno_line_point<R>(a.L[0].ax[2], u, id, k, wu);
         }
-        a.cu[id] = r;
+        st_stream(&a.cu[id], r);
     }
     // v node
     {
@@ -204,7 +204,7 @@ __global__ void __launch_bounds__(256) k_eno_momentum(MomArgs a, Tiles t) {
             r += eno_line_point<R>(a.L[1].ax[1], v, id, j, v[id]);
             r += eno_line_point<R>(a.L[1].ax[2], v, id, k, wv);
         }
-        a.cv[id] = r;
+        st_stream(&a.cv[id], r);
     }
     // w node
     {
@@ -218,7 +218,7 @@ __global__ void __launch_bounds__(256) k_eno_momentum(MomArgs a, Tiles t) {
             r += eno_line_point<R>(a.L[2].ax[1], w, id, j, vw);
             r += eno_line_point<R>(a.L[2].ax[2], w, id, k, w[id]);
         }
-        a.cw[id] = r;
+        st_stream(&a.cw[id], r);
     }
 }
 
@@ -251,7 +251,7 @@ __global__ void __launch_bounds__(256) k_eno_conc(ConcArgs a, Tiles t) {
         r += eno_line_point<R>(a.L.ax[1], a.c, id, j, vbc);
         r += eno_line_point<R>(a.L.ax[2], a.c, id, k, wbc);
     }
-    a.cc[id] = r;
+    st_stream(&a.cc[id], r);
 }
 
 // Line specs per quantity (the ghost rules of Velocity.c:2124-2801 and

@@ -919,7 +919,7 @@ extern "C" tb_net *tb_net_create(uint64_t seed) {
     ok = ok && hipMalloc((void **)&net->v, sizeof(float) * net->np) == hipSuccess;
     ok = ok && hipMalloc((void **)&net->red, sizeof(double) * NET_RED_BLOCKS) == hipSuccess;
     ok = ok && hipMalloc((void **)&net->dscal, sizeof(double) * 8) == hipSuccess;
-    ok = ok && hipHostMalloc((void **)&net->hscal, sizeof(double) * 8) == hipSuccess;
+    ok = ok && hipHostMalloc((void **)&net->hscal, sizeof(double) * 8, hipHostMallocCoherent) == hipSuccess;
     for (int l = 0; ok && l < NLAYER - 1; l++) {
         const int ci = LAYERS[l].cin, co = LAYERS[l].cout;
         net->nch[l] = (27 * ci + 31) / 32;

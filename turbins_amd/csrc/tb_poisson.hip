@@ -119,12 +119,12 @@ __global__ void __launch_bounds__(256) k_pois_km(PoisOp P, const double *__restr
             }
             if (RESID) {
                 double ri = bi - v;
-                if (y) y[id] = ri;
-                if (y32) y32[id] = (float)ri;
+                if (y) st_stream(&y[id], ri);
+                if (y32) st_stream(&y32[id], (float)ri);
                 acc[0] += ri * ri;
                 acc[1] += bi * bi;
             } else {
-                y[id] = v;
+                st_stream(&y[id], v);
                 acc[0] += xc * v;
             }
             xm = xc;
@@ -716,7 +716,7 @@ __global__ void __launch_bounds__(256) k_cg32_apply(LV<float> L, const float *__
             const float pw = p[id - ow], pe = p[id + oe], ps = p[id - os], pn = p[id + on];
             const float v = L.cx * (pc - pw) + L.cx * (pc - pe) + L.cy * (pc - ps) + L.cy * (pc - pn) +
                             L.cz * (pc - pm) + L.cz * (pc - pp);
-            q[id] = v;
+            st_stream(&q[id], v);
             acc[0] += (double)pc * (double)v;
             pm = pc;
             pc = pp;
@@ -789,12 +789,12 @@ static int cg32_inner(tb_ctx *c, const KTiles &kt, double r0n, double tol_rel, i
     if (tb_reduce_finish(c, PB, 1, S_RZ)) return -1;
     int inner = 0;
     while (inner < maxit) {
-        TB_PROF(c, TB_K_POIS_OP,
+        TB_PROF(c, TB_K_CG32_OP,
                 hipLaunchKernelGGL(k_cg32_apply, ktiles_grid(kt), tiles_block(), 0, c->s, lv, (const float *)p, q,
                                    c->red, kt));
         TB_LAUNCH_CHECK(c);
         if (tb_reduce_finish(c, nblk, 1, S_PQ)) return -1;
-        TB_PROF(c, TB_K_PCG_UPD,
+        TB_PROF(c, TB_K_CG32_UPD,
                 hipLaunchKernelGGL(k_cg32_xr, dim3(PB), dim3(PT), 0, c->s, n4, (float4 *)e, (float4 *)r,
                                    (const float4 *)p, (const float4 *)q, (const double *)c->scal, c->red));
         TB_LAUNCH_CHECK(c);
@@ -821,11 +821,18 @@ static int cg32_inner(tb_ctx *c, const KTiles &kt, double r0n, double tol_rel, i
 // Mixed precision: FP64 iterative refinement (true residual r = b - A x in
 // FP64, rounded to FP32) around the FP32 inner CG, x += e in FP64.  Each inner
 // solve asks for the residual reduction still missing (times 0.5), but never
-// less than IR_INNER_REDUCTION, which FP32 reaches reliably.  Stops at rtol, at
-// maxit inner iterations, or when an outer pass reduces the FP64 residual by
-// less than half (stalled: the caller falls back to FP64 PCG).
-constexpr double IR_INNER_REDUCTION = 1e-4;
-constexpr int IR_INNER_MAXIT = 40;
+// less than c->ir_floor (default 1e-5, which FP32 reaches reliably).  Stops at
+// rtol, at maxit inner iterations, or when an outer pass reduces the FP64
+// residual by less than half (stalled: the caller falls back to FP64 PCG).
+extern "C" int tb_set_mixed_inner(tb_ctx *c, double reduction_floor, int maxit) {
+    if (!c || !(reduction_floor > 0.0 && reduction_floor < 1.0) || maxit < 1) {
+        if (c) c->err = "tb_set_mixed_inner: bad argument";
+        return -1;
+    }
+    c->ir_floor = reduction_floor;
+    c->ir_maxit = maxit;
+    return 0;
+}
 
 static int mixed_ir(tb_ctx *c, const PoisCtx &P, double *x, const double *b, double bn, double rtol, int maxit,
                     int &its, int &vcy, double &res, bool &stalled) {
@@ -842,9 +849,9 @@ static int mixed_ir(tb_ctx *c, const PoisCtx &P, double *x, const double *b, dou
     while (res > rtol && its < maxit) {
         const double prev = res;
         double want = 0.5 * rtol / res;
-        if (want < IR_INNER_REDUCTION) want = IR_INNER_REDUCTION;
+        if (want < c->ir_floor) want = c->ir_floor;
         int room = maxit - its;
-        if (cg32_inner(c, kt0, sqrt(rr), want, room < IR_INNER_MAXIT ? room : IR_INNER_MAXIT, its, vcy)) return -1;
+        if (cg32_inner(c, kt0, sqrt(rr), want, room < c->ir_maxit ? room : c->ir_maxit, its, vcy)) return -1;
         hipLaunchKernelGGL(k_add32, tiles_grid(P.cells), tiles_block(), 0, c->s, c->d, x, (const float *)c->cg32[2],
                            P.cells);
         TB_LAUNCH_CHECK(c);
