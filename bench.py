@@ -17,7 +17,8 @@ resident in HBM before the timed region; each library step ends with a
 device synchronisation, so the wall clock brackets exactly K steps.
 
 `roofline` reports the instrumented kernel with the largest total time in
-the headline region: algorithmic bytes per launch / average launch time
+a few further headline-path steps run with HIP events on (outside the timed
+region): algorithmic bytes per launch / average launch time
 from HIP events on the library's stream.  `cpu_baseline` times the FP64
 oracle (the reference's algorithm: unpreconditioned BiCGStab, GMG-PCG
 Poisson) for one step of the same workload on the host cores.
@@ -49,26 +50,32 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run_path(g, c, steps, warmup, mode, rtol, profile):
-    from turbins_amd import abi
+def run_path(g, c, steps, warmup, mode, rtol):
     g.set_poisson_mode(mode, rtol)
     for _ in range(warmup):
         g.step(c["dt"])
-    if profile:
-        g.profile(True)
     stats = []
     t0 = time.perf_counter()
     for _ in range(steps):
         stats.append(g.step(c["dt"]))
     t1 = time.perf_counter()
+    return (t1 - t0), stats
+
+
+def profile_path(g, c, steps):
+    """Per-kernel device time from HIP events on the library's stream over a
+    few further steps (kept out of the timed region: with events on, the
+    V-cycles launch kernel by kernel instead of replaying their graphs)."""
+    from turbins_amd import abi
+    g.profile(True)
+    stats = [g.step(c["dt"]) for _ in range(steps)]
     prof = {}
-    if profile:
-        for name, kid in abi.K.items():
-            ms, n = g.profile_read(kid)
-            if n:
-                prof[name] = (ms, n)
-        g.profile(False)
-    return (t1 - t0), stats, prof
+    for name, kid in abi.K.items():
+        ms, n = g.profile_read(kid)
+        if n:
+            prof[name] = (ms, n)
+    g.profile(False)
+    return prof, stats
 
 
 def roofline(prof, stats, c):
@@ -157,7 +164,7 @@ def main():
     for k, v in W.initial_fields(c, flags).items():
         g.upload(k, v)
 
-    t_plain, st_plain, _ = run_path(g, c, args.steps, args.warmup, abi.POIS_FP64_PCG, 1e-8, False)
+    t_plain, st_plain = run_path(g, c, args.steps, args.warmup, abi.POIS_FP64_PCG, 1e-8)
     plain_sps = args.steps / t_plain
     from turbins_amd import Net
     net = None
@@ -167,8 +174,9 @@ def main():
         g.attach_net(net)
     learned = net is not None
     mode = abi.POIS_LEARNED_IR
-    t_head, st_head, prof = run_path(g, c, args.steps, args.warmup, mode, 1e-8, True)
+    t_head, st_head = run_path(g, c, args.steps, args.warmup, mode, 1e-8)
     head_sps = args.steps / t_head
+    prof, st_prof = profile_path(g, c, max(2, min(args.steps, 4)))
     gate = g.net_gate() if learned else None
 
     def pois_summary(stats):
@@ -180,13 +188,19 @@ def main():
                 "initial_residual_median": float(np.median(r0)), "fallbacks": fb}
 
     helm_its = [h.iterations for st in st_head for stage in st.helm for h in stage]
+
+    def phase_ms(stats):
+        helm = sum(h.ms for st in stats for stage in st.helm for h in stage) / len(stats)
+        pois = sum(p.ms for st in stats for p in st.pois) / len(stats)
+        return {"helmholtz": round(helm, 2), "poisson": round(pois, 2)}
     log(f"plain: {plain_sps:.3f} steps/s ({1e3 / plain_sps:.1f} ms/step)  headline: {head_sps:.3f} steps/s "
         f"({1e3 / head_sps:.1f} ms/step)  ratio {head_sps / plain_sps:.3f}")
     log("poisson plain", pois_summary(st_plain), "headline", pois_summary(st_head))
+    log("device ms per step: plain", phase_ms(st_plain), "headline", phase_ms(st_head))
     for name, (ms, n) in sorted(prof.items(), key=lambda kv: -kv[1][0]):
         log(f"  {name:10s} {n:6d} launches  {ms:9.2f} ms  {1e3 * ms / n:8.1f} us/launch")
 
-    rl = roofline(prof, st_head, c)
+    rl = roofline(prof, st_prof, c)
     result = {
         "metric": METRIC,
         "value": round(head_sps, 4),
@@ -219,6 +233,9 @@ def main():
             "poisson_headline": pois_summary(st_head),
             "poisson_plain": pois_summary(st_plain),
             "helmholtz_iterations_mean": round(float(np.mean(helm_its)), 2),
+            "helmholtz_method": "red-black Gauss-Seidel sweeps (BiCGStab fallback), rtol 1e-12",
+            "solve_ms_per_step_plain": phase_ms(st_plain),
+            "solve_ms_per_step_headline": phase_ms(st_head),
             "precision": "FP64 fields/ENO/Helmholtz/outer Poisson; FP32 multigrid in the headline path",
         },
         "roofline": rl,

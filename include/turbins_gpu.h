@@ -234,6 +234,9 @@ int tb_set_helmholtz_method(tb_ctx *ctx, int method);
 /* mixed Poisson mode: smallest residual reduction one FP32 inner solve is
  * asked for (default 1e-5) and its iteration cap (default 40) */
 int tb_set_mixed_inner(tb_ctx *ctx, double reduction_floor, int maxit);
+/* multigrid V-cycles replayed from HIP graphs (default 1) or launched
+ * kernel by kernel (0); results are identical */
+int tb_set_graphs(tb_ctx *ctx, int on);
 
 /* learned initial guess of the Poisson solve (no reference counterpart: the
  * reference starts GMRES from the previous phi, Solver.c:256).  A 3-level

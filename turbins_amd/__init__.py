@@ -64,6 +64,7 @@ def lib():
     L.tb_set_helmholtz_rtol.argtypes = [P, C.c_double]
     L.tb_set_mixed_inner.argtypes = [P, C.c_double, C.c_int]
     L.tb_set_helmholtz_method.argtypes = [P, C.c_int]
+    L.tb_set_graphs.argtypes = [P, C.c_int]
     L.tb_profile_enable.argtypes = [P, C.c_int]
     L.tb_profile_read.argtypes = [P, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
     L.tb_net_create.restype = P
@@ -232,6 +233,9 @@ class Solver:
 
     def set_helmholtz_method(self, method):
         return self._ok(self.L.tb_set_helmholtz_method(self.h, method), "tb_set_helmholtz_method")
+
+    def set_graphs(self, on):
+        return self._ok(self.L.tb_set_graphs(self.h, 1 if on else 0), "tb_set_graphs")
 
     def set_mixed_inner(self, reduction_floor, maxit=40):
         return self._ok(self.L.tb_set_mixed_inner(self.h, reduction_floor, maxit), "tb_set_mixed_inner")

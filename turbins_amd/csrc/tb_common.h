@@ -215,6 +215,10 @@ struct tb_ctx {
     double *coarse_inv64 = nullptr;
     float *coarse_inv32 = nullptr;
     float *cg32[3] = {};         // p, q, e of the FP32 inner CG (level-0 layout)
+    // V-cycles replayed from HIP graphs (0: FP64, 1: FP32); the FP64 graph is
+    // tied to the level-0 vectors it was captured with
+    bool use_graphs = true;
+    hipGraphExec_t vc_exec[2] = {nullptr, nullptr};
     int helm_method = TB_HELM_GS;    // Helmholtz solver
     double ir_floor = 1e-5;      // smallest residual reduction asked of one inner FP32 solve
     int ir_maxit = 40;           // inner iterations per refinement pass

@@ -189,21 +189,25 @@ __global__ void __launch_bounds__(256) k_zero(LV<T> L, T *__restrict__ x, Tiles 
     x[lidx(L, i, j, k)] = (T)0;
 }
 
-// red-black Gauss-Seidel half sweep; threads tile (ceil(nx/2), ny, nz)
+// Per-cell bodies of the multigrid operations, shared by the one-launch-per-
+// step kernels of the large levels and the single-workgroup kernel of the
+// small ones (so both give the same bits).
+
+// red-black Gauss-Seidel update of cell (i, j, k); zero_init: the other
+// colour still holds the zero initial guess, so no neighbour is read
 template <typename T>
-__global__ void __launch_bounds__(256) k_rbgs(LV<T> L, T *__restrict__ x, const T *__restrict__ b, int color,
-                                              Tiles t) {
-    int ih, j, k;
-    if (!tile_point(t, ih, j, k)) return;
-    int i = 2 * ih + ((j + k + color) & 1);
-    if (i >= L.nx) return;
-    long id = lidx(L, i, j, k);
-    long pl = (long)L.px * L.py;
+__device__ inline void rbgs_cell(const LV<T> &L, T *__restrict__ x, const T *__restrict__ b, int i, int j, int k,
+                                 bool zero_init) {
+    const long id = lidx(L, i, j, k);
+    const long pl = (long)L.px * L.py;
     const bool hw = i > 0, he = i < L.nx - 1, hs = j > 0, hn = j < L.ny - 1, hb = k > 0, hf = k < L.nz - 1;
-    // all neighbour loads issued up front (clamped to the cell itself)
-    const T xw = x[id - (hw ? 1 : 0)], xe = x[id + (he ? 1 : 0)];
-    const T xs = x[id - (hs ? L.px : 0)], xn = x[id + (hn ? L.px : 0)];
-    const T xb = x[id - (hb ? pl : 0)], xf = x[id + (hf ? pl : 0)];
+    T xw = (T)0, xe = (T)0, xs = (T)0, xn = (T)0, xb = (T)0, xf = (T)0;
+    if (!zero_init) {
+        // all neighbour loads issued up front (clamped to the cell itself)
+        xw = x[id - (hw ? 1 : 0)]; xe = x[id + (he ? 1 : 0)];
+        xs = x[id - (hs ? L.px : 0)]; xn = x[id + (hn ? L.px : 0)];
+        xb = x[id - (hb ? pl : 0)]; xf = x[id + (hf ? pl : 0)];
+    }
     T s = b[id], dg = (T)0;
     s = hw ? s + L.cx * xw : s; dg = hw ? dg + L.cx : dg;
     s = he ? s + L.cx * xe : s; dg = he ? dg + L.cx : dg;
@@ -215,12 +219,10 @@ __global__ void __launch_bounds__(256) k_rbgs(LV<T> L, T *__restrict__ x, const 
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) k_mg_resid(LV<T> L, const T *__restrict__ x, const T *__restrict__ b,
-                                                  T *__restrict__ r, Tiles t) {
-    int i, j, k;
-    if (!tile_point(t, i, j, k)) return;
-    long id = lidx(L, i, j, k);
-    long pl = (long)L.px * L.py;
+__device__ inline void resid_cell(const LV<T> &L, const T *__restrict__ x, const T *__restrict__ b,
+                                  T *__restrict__ r, int i, int j, int k) {
+    const long id = lidx(L, i, j, k);
+    const long pl = (long)L.px * L.py;
     const bool hw = i > 0, he = i < L.nx - 1, hs = j > 0, hn = j < L.ny - 1, hb = k > 0, hf = k < L.nz - 1;
     // absent neighbours are clamped to the cell itself, contributing xc - xc = 0
     const T xc = x[id];
@@ -238,10 +240,8 @@ __device__ inline int clampi(int a, int lo, int hi) { return a < lo ? lo : (a > 
 // restriction: adjoint of the prolongation / 2 per halved axis,
 // 1-D weights (1/8, 3/8, 3/8, 1/8) with reflection at the walls
 template <typename T>
-__global__ void __launch_bounds__(256) k_restrict(LV<T> F, LV<T> C, int cx, int cy, int cz,
-                                                  const T *__restrict__ rf, T *__restrict__ bc, Tiles t) {
-    int I, J, K;
-    if (!tile_point(t, I, J, K)) return;
+__device__ inline void restrict_cell(const LV<T> &F, const LV<T> &C, int cx, int cy, int cz, const T *__restrict__ rf,
+                                     T *__restrict__ bc, int I, int J, int K) {
     const T W[4] = {(T)0.125, (T)0.375, (T)0.375, (T)0.125};
     T s = (T)0;
     int tk0 = cz ? -1 : 0, tk1 = cz ? 2 : 0;
@@ -267,10 +267,8 @@ __global__ void __launch_bounds__(256) k_restrict(LV<T> F, LV<T> C, int cx, int 
 
 // prolongation: (3/4, 1/4) per halved axis, reflecting; x_f += P x_c
 template <typename T>
-__global__ void __launch_bounds__(256) k_prolong(LV<T> F, LV<T> C, int cx, int cy, int cz, const T *__restrict__ xc,
-                                                 T *__restrict__ xf, Tiles t) {
-    int i, j, k;
-    if (!tile_point(t, i, j, k)) return;
+__device__ inline void prolong_cell(const LV<T> &F, const LV<T> &C, int cx, int cy, int cz, const T *__restrict__ xc,
+                                    T *__restrict__ xf, int i, int j, int k) {
     int K0, K1, J0, J1, I0, I1;
     T wk1, wj1, wi1;
     if (cz) { K0 = k >> 1; K1 = clampi((k & 1) ? K0 + 1 : K0 - 1, 0, C.nz - 1); wk1 = (T)0.25; } else { K0 = K1 = k; wk1 = (T)0; }
@@ -284,34 +282,160 @@ __global__ void __launch_bounds__(256) k_prolong(LV<T> F, LV<T> C, int cx, int c
     xf[lidx(F, i, j, k)] += s;
 }
 
-// coarsest level: x = Minv (b - mean b), then x -= mean x  (one block)
+// coarsest level: x = Minv (b - mean b), then x -= mean x (whole block;
+// blockDim.x >= n, n <= 1024); MinvT is the inverse stored transposed
 template <typename T>
-__global__ void k_coarse(const T *__restrict__ Minv, int n, const T *__restrict__ b, T *__restrict__ x) {
-    __shared__ double sh[16];
-    __shared__ double bb[1024];
-    __shared__ double mean_s;
+__device__ inline void coarse_solve(const T *__restrict__ Minv, int n, const T *__restrict__ b, T *__restrict__ x,
+                                    double *sh, double *bb, double *mean_s) {
     double v[1] = {0.0};
     for (int a = threadIdx.x; a < n; a += blockDim.x) v[0] += (double)b[a];
     block_sum<1>(v, sh);
-    if (threadIdx.x == 0) mean_s = v[0] / n;
+    if (threadIdx.x == 0) *mean_s = v[0] / n;
     __syncthreads();
-    double mb = mean_s;
+    double mb = *mean_s;
     for (int a = threadIdx.x; a < n; a += blockDim.x) bb[a] = (double)b[a] - mb;
     __syncthreads();
     double xr[1] = {0.0};
     double mine = 0.0;
     if ((int)threadIdx.x < n) {
         double s = 0.0;
-        const T *row = Minv + (long)threadIdx.x * n;
-        for (int cc = 0; cc < n; cc++) s += (double)row[cc] * bb[cc];
+        for (int cc = 0; cc < n; cc++) s += (double)Minv[(long)cc * n + threadIdx.x] * bb[cc];
         mine = s;
         xr[0] = s;
     }
     __syncthreads();
     block_sum<1>(xr, sh);
-    if (threadIdx.x == 0) mean_s = xr[0] / n;
+    if (threadIdx.x == 0) *mean_s = xr[0] / n;
     __syncthreads();
-    if ((int)threadIdx.x < n) x[threadIdx.x] = (T)(mine - mean_s);
+    if ((int)threadIdx.x < n) x[threadIdx.x] = (T)(mine - *mean_s);
+    __syncthreads();
+}
+
+// red-black Gauss-Seidel half sweep; threads tile (ceil(nx/2), ny, nz)
+template <typename T>
+__global__ void __launch_bounds__(256) k_rbgs(LV<T> L, T *__restrict__ x, const T *__restrict__ b, int color,
+                                              int zero_init, Tiles t) {
+    int ih, j, k;
+    if (!tile_point(t, ih, j, k)) return;
+    int i = 2 * ih + ((j + k + color) & 1);
+    if (i >= L.nx) return;
+    rbgs_cell(L, x, b, i, j, k, zero_init != 0);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_mg_resid(LV<T> L, const T *__restrict__ x, const T *__restrict__ b,
+                                                  T *__restrict__ r, Tiles t) {
+    int i, j, k;
+    if (!tile_point(t, i, j, k)) return;
+    resid_cell(L, x, b, r, i, j, k);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_restrict(LV<T> F, LV<T> C, int cx, int cy, int cz,
+                                                  const T *__restrict__ rf, T *__restrict__ bc, Tiles t) {
+    int I, J, K;
+    if (!tile_point(t, I, J, K)) return;
+    restrict_cell(F, C, cx, cy, cz, rf, bc, I, J, K);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_prolong(LV<T> F, LV<T> C, int cx, int cy, int cz, const T *__restrict__ xc,
+                                                 T *__restrict__ xf, Tiles t) {
+    int i, j, k;
+    if (!tile_point(t, i, j, k)) return;
+    prolong_cell(F, C, cx, cy, cz, xc, xf, i, j, k);
+}
+
+template <typename T>
+__global__ void k_coarse(const T *__restrict__ Minv, int n, const T *__restrict__ b, T *__restrict__ x) {
+    __shared__ double sh[16];
+    __shared__ double bb[1024];
+    __shared__ double mean_s;
+    coarse_solve(Minv, n, b, x, sh, bb, &mean_s);
+}
+
+// The small levels of a V-cycle (at most SMALL_LEVEL_CELLS cells each) run
+// in one workgroup of 1024 threads, one barrier between phases: their
+// launches (~5 us each, 12 per level) would otherwise cost more than their
+// work.  Larger levels are latency-bound in a single workgroup (a
+// 32768-cell level made the fused kernel slower than the launches).
+constexpr long SMALL_LEVEL_CELLS = 4096;
+constexpr int MAXLEV = 16;
+
+template <typename T>
+struct SmallCycle {
+    int l0, nl, pre, post, coarse_n;
+    LV<T> lv[MAXLEV];
+    T *x[MAXLEV], *b[MAXLEV], *r[MAXLEV];
+    int cx[MAXLEV], cy[MAXLEV], cz[MAXLEV];
+    const T *Minv;
+};
+
+template <typename T>
+__device__ inline void block_sweep(const LV<T> &L, T *x, const T *b, int color, bool zero_init) {
+    const int hx = (L.nx + 1) / 2;
+    const int n = hx * L.ny * L.nz;
+    for (int q = threadIdx.x; q < n; q += blockDim.x) {
+        const int ih = q % hx;
+        const int rr = q / hx;
+        const int j = rr % L.ny, k = rr / L.ny;
+        const int i = 2 * ih + ((j + k + color) & 1);
+        if (i < L.nx) rbgs_cell(L, x, b, i, j, k, zero_init);
+    }
+    __syncthreads();
+}
+
+template <typename T>
+__global__ void __launch_bounds__(1024) k_small_cycle(SmallCycle<T> A) {
+    __shared__ double sh[16];
+    __shared__ double bb[1024];
+    __shared__ double mean_s;
+    const int last = A.nl - 1;
+    for (int l = A.l0; l < last; l++) {
+        const LV<T> &L = A.lv[l];
+        const int n = L.nx * L.ny * L.nz;
+        if (A.pre > 0) {
+            block_sweep(L, A.x[l], A.b[l], 0, true);
+            block_sweep(L, A.x[l], A.b[l], 1, false);
+            for (int s = 1; s < A.pre; s++) {
+                block_sweep(L, A.x[l], A.b[l], 0, false);
+                block_sweep(L, A.x[l], A.b[l], 1, false);
+            }
+        } else {
+            for (int q = threadIdx.x; q < n; q += blockDim.x) {
+                const int i = q % L.nx, rr = q / L.nx;
+                A.x[l][lidx(L, i, rr % L.ny, rr / L.ny)] = (T)0;
+            }
+            __syncthreads();
+        }
+        for (int q = threadIdx.x; q < n; q += blockDim.x) {
+            const int i = q % L.nx, rr = q / L.nx;
+            resid_cell(L, A.x[l], A.b[l], A.r[l], i, rr % L.ny, rr / L.ny);
+        }
+        __syncthreads();
+        const LV<T> &C = A.lv[l + 1];
+        const int nc = C.nx * C.ny * C.nz;
+        for (int q = threadIdx.x; q < nc; q += blockDim.x) {
+            const int I = q % C.nx, rr = q / C.nx;
+            restrict_cell(L, C, A.cx[l], A.cy[l], A.cz[l], A.r[l], A.b[l + 1], I, rr % C.ny, rr / C.ny);
+        }
+        __syncthreads();
+    }
+    coarse_solve(A.Minv, A.coarse_n, A.b[last], A.x[last], sh, bb, &mean_s);
+    for (int l = last - 1; l >= A.l0; l--) {
+        const LV<T> &L = A.lv[l];
+        const LV<T> &C = A.lv[l + 1];
+        const int n = L.nx * L.ny * L.nz;
+        for (int q = threadIdx.x; q < n; q += blockDim.x) {
+            const int i = q % L.nx, rr = q / L.nx;
+            prolong_cell(L, C, A.cx[l], A.cy[l], A.cz[l], A.x[l + 1], A.x[l], i, rr % L.ny, rr / L.ny);
+        }
+        __syncthreads();
+        for (int s = 0; s < A.post; s++) {
+            block_sweep(L, A.x[l], A.b[l], 1, false);
+            block_sweep(L, A.x[l], A.b[l], 0, false);
+        }
+    }
 }
 
 template <typename T>
@@ -370,6 +494,10 @@ void tb_mg_free(tb_ctx *c) {
     }
     hipFree(c->coarse_inv64); hipFree(c->coarse_inv32);
     c->coarse_inv64 = nullptr; c->coarse_inv32 = nullptr;
+    for (int a = 0; a < 2; a++) {
+        if (c->vc_exec[a]) hipGraphExecDestroy(c->vc_exec[a]);
+        c->vc_exec[a] = nullptr;
+    }
     for (int a = 0; a < 3; a++) { hipFree(c->cg32[a]); c->cg32[a] = nullptr; }
     c->nlev = 0;
     c->mg_built = false;
@@ -456,6 +584,12 @@ int tb_mg_build(tb_ctx *c) {
             for (int k = 0; k < n; k++) { A[(size_t)r * n + k] -= f * A[(size_t)cc * n + k]; I[(size_t)r * n + k] -= f * I[(size_t)cc * n + k]; }
         }
     }
+    // stored transposed (column cc contiguous) so that the threads of the
+    // coarse solve, one per row, read consecutive addresses
+    std::vector<double> IT((size_t)n * n);
+    for (int r = 0; r < n; r++)
+        for (int cc = 0; cc < n; cc++) IT[(size_t)cc * n + r] = I[(size_t)r * n + cc];
+    I.swap(IT);
     std::vector<float> I32((size_t)n * n);
     for (size_t a = 0; a < I.size(); a++) I32[a] = (float)I[a];
     TB_CHECK(c, hipMalloc((void **)&c->coarse_inv64, sizeof(double) * (size_t)n * n));
@@ -491,23 +625,44 @@ static int vcycle_level(tb_ctx *c, int l) {
     const Level &L = c->lev[l];
     MGBuf<T> B = bufs<T>(c, l);
     LV<T> lv = lv_of<T>(L);
-    if (l == c->nlev - 1) {
-        hipLaunchKernelGGL(k_coarse<T>, dim3(1), dim3(1024), 0, c->s, coarse_inv<T>(c), c->coarse_n, B.b, B.x);
+    if ((long)L.nx * L.ny * L.nz <= SMALL_LEVEL_CELLS || l == c->nlev - 1) {
+        SmallCycle<T> A;
+        A.l0 = l;
+        A.nl = c->nlev;
+        A.pre = c->cfg.mg_pre;
+        A.post = c->cfg.mg_post;
+        A.coarse_n = c->coarse_n;
+        A.Minv = coarse_inv<T>(c);
+        for (int m = l; m < c->nlev; m++) {
+            MGBuf<T> Bm = bufs<T>(c, m);
+            A.lv[m] = lv_of<T>(c->lev[m]);
+            A.x[m] = Bm.x; A.b[m] = Bm.b; A.r[m] = Bm.r;
+            A.cx[m] = c->lev[m].cx; A.cy[m] = c->lev[m].cy; A.cz[m] = c->lev[m].cz;
+        }
+        hipLaunchKernelGGL(k_small_cycle<T>, dim3(1), dim3(1024), 0, c->s, A);
         TB_LAUNCH_CHECK(c);
         return 0;
     }
     Tiles all = make_tiles(L.nx, L.ny, L.nz);
     Tiles half = make_tiles((L.nx + 1) / 2, L.ny, L.nz);
-    hipLaunchKernelGGL(k_zero<T>, tiles_grid(all), tiles_block(), 0, c->s, lv, B.x, all);
     const int kid = (l == 0) ? (sizeof(T) == 8 ? TB_K_RBGS0 : TB_K_RBGS0_F32) : -1;
-    auto smooth = [&](int color) {
+    auto smooth = [&](int color, int zero_init) {
         if (c->prof_on && kid >= 0) tb_prof_begin(c, kid);
-        hipLaunchKernelGGL(k_rbgs<T>, tiles_grid(half), tiles_block(), 0, c->s, lv, B.x, (const T *)B.b, color, half);
+        hipLaunchKernelGGL(k_rbgs<T>, tiles_grid(half), tiles_block(), 0, c->s, lv, B.x, (const T *)B.b, color,
+                           zero_init, half);
         if (c->prof_on && kid >= 0) tb_prof_end(c, kid);
     };
-    for (int s = 0; s < c->cfg.mg_pre; s++) {
-        smooth(0);
-        smooth(1);
+    // zero initial guess: the first red half sweep reads no neighbour and
+    // the black one only red cells, so x needs no clearing pass
+    if (c->cfg.mg_pre > 0) {
+        smooth(0, 1);
+        smooth(1, 0);
+        for (int s = 1; s < c->cfg.mg_pre; s++) {
+            smooth(0, 0);
+            smooth(1, 0);
+        }
+    } else {
+        hipLaunchKernelGGL(k_zero<T>, tiles_grid(all), tiles_block(), 0, c->s, lv, B.x, all);
     }
     hipLaunchKernelGGL(k_mg_resid<T>, tiles_grid(all), tiles_block(), 0, c->s, lv, (const T *)B.x, (const T *)B.b,
                        B.r, all);
@@ -522,26 +677,64 @@ static int vcycle_level(tb_ctx *c, int l) {
     hipLaunchKernelGGL(k_prolong<T>, tiles_grid(all), tiles_block(), 0, c->s, lv, cv, L.cx, L.cy, L.cz,
                        (const T *)CB.x, B.x, all);
     for (int s = 0; s < c->cfg.mg_post; s++) {
-        smooth(1);
-        smooth(0);
+        smooth(1, 0);
+        smooth(0, 0);
     }
     TB_LAUNCH_CHECK(c);
     return 0;
 }
 
 // z = V(r) in FP64 with level 0 aliased onto the padded vectors
+// A V-cycle is ~10 levels x ~12 launches, most of them on grids far too
+// small to fill the chip, so it is captured once into a HIP graph and
+// replayed with one launch.  The graph holds kernels only (no memset nodes).
+// Profiling runs launch the kernels directly, so their events are recorded.
+template <typename T>
+static int vcycle_run(tb_ctx *c, int which) {
+    if (!c->use_graphs || c->prof_on) return vcycle_level<T>(c, 0);
+    if (!c->vc_exec[which]) {
+        hipGraph_t g = nullptr;
+        TB_CHECK(c, hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal));
+        const int rc = vcycle_level<T>(c, 0);
+        const hipError_t e = hipStreamEndCapture(c->s, &g);
+        if (rc || e != hipSuccess) {
+            if (e == hipSuccess && g) hipGraphDestroy(g);
+            if (!rc) c->err = std::string("V-cycle capture: ") + hipGetErrorString(e);
+            return -1;
+        }
+        hipGraphExec_t ex = nullptr;
+        const hipError_t ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        if (ei != hipSuccess) { c->err = std::string("V-cycle graph: ") + hipGetErrorString(ei); return -1; }
+        c->vc_exec[which] = ex;
+    }
+    TB_CHECK(c, hipGraphLaunch(c->vc_exec[which], c->s));
+    return 0;
+}
+
 int tb_vcycle64(tb_ctx *c, const double *r_pad, double *z_pad) {
     if (tb_mg_build(c)) return -1;
+    const bool same = c->lev[0].b64 == r_pad && c->lev[0].x64 == z_pad && c->lev[0].r64 == c->kv[4];
+    if (!same && c->vc_exec[0]) {        // captured with other level-0 vectors
+        hipGraphExecDestroy(c->vc_exec[0]);
+        c->vc_exec[0] = nullptr;
+    }
     c->lev[0].b64 = const_cast<double *>(r_pad);
     c->lev[0].x64 = z_pad;
     c->lev[0].r64 = c->kv[4];
-    return vcycle_level<double>(c, 0);
+    return vcycle_run<double>(c, 0);
 }
 
 // lev[0].x32 = V(lev[0].b32) in FP32
 int tb_vcycle32(tb_ctx *c) {
     if (tb_mg_build(c)) return -1;
-    return vcycle_level<float>(c, 0);
+    return vcycle_run<float>(c, 1);
+}
+
+extern "C" int tb_set_graphs(tb_ctx *c, int on) {
+    if (!c) return -1;
+    c->use_graphs = on != 0;
+    return 0;
 }
 
 // ---------------------------------------------------------------------------
