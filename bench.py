@@ -44,6 +44,10 @@ BYTES_PER_POINT = {
     "CG32_OP": 8, "CG32_UPD": 24,
 }
 WEIGHTS = os.path.join(ROOT, "turbins_amd", "weights", "poisson_guess.f32")
+# HBM bytes per instrumented launch measured with rocprofv3 FETCH_SIZE /
+# WRITE_SIZE on this workload, scaled by a plain copy of known size
+# (profiles/hbm_traffic_bump512x256.md); only valid for the kernels as built
+MEASURED_TRAFFIC = {"HELM_GS": 1.0233e9}
 
 
 def log(*a):
@@ -108,8 +112,9 @@ def roofline(prof, stats, c):
     per_launch = total / n
     avg_s = ms / n * 1e-3
     achieved = per_launch / avg_s / 1e9
+    traffic = MEASURED_TRAFFIC.get(name) if c["nx"] == 512 and c["ny"] == 256 and c["nz"] == 128 else None
     return {"kernel": name, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
             "bytes_per_launch": int(per_launch), "avg_launch_us": round(avg_s * 1e6, 2), "launches": n}
 
 
@@ -146,6 +151,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=4)
     ap.add_argument("--workload", default="bump512x256")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="launch the V-cycle kernels one by one (for tracing tools that do not follow HIP graphs)")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
     args = ap.parse_args()
     if args.gpus != 1:
@@ -157,6 +164,8 @@ def main():
     c = W.case(args.workload)
     cfg = W.config_for(c, pois_rtol=1e-8, helm_rtol=1e-12)
     g = Solver(cfg)
+    if args.no_graphs:
+        g.set_graphs(False)
     t = time.perf_counter()
     g.setup_geometry(W.topography(c))
     log(f"geometry setup {time.perf_counter() - t:.2f}s, reinit iterations {g.reinit_iterations()}")
