@@ -378,8 +378,28 @@ __global__ void k_relu_mask(bf16_t *__restrict__ g, const bf16_t *__restrict__ y
 // ---------------------------------------------------------------------------
 // 16 -> 1 head (FP32 weights rounded to BF16 like every other layer)
 
-__global__ void k_head_fwd(const bf16_t *__restrict__ y0, const float *__restrict__ W, const float *__restrict__ b,
-                           int nx, int ny, int nz, float *__restrict__ out) {
+__device__ inline void unpack16(const bf16_t *p, float (&v)[16]) {
+    const v4u a = *(const v4u *)p, b = *(const v4u *)(p + 8);
+    const uint32_t w[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+        v[2 * e] = bf2f((bf16_t)(w[e] & 0xffff));
+        v[2 * e + 1] = bf2f((bf16_t)(w[e] >> 16));
+    }
+}
+
+// the head's 432 weights rounded to BF16, once per workgroup
+__device__ inline void load_head_weights(const float *__restrict__ W, float *w) {
+    for (int q = threadIdx.x; q < 432; q += blockDim.x) w[q] = bf2f(f2bf(W[q]));
+    __syncthreads();
+}
+
+__global__ void __launch_bounds__(256) k_head_fwd(const bf16_t *__restrict__ y0, const float *__restrict__ W,
+                                                  const float *__restrict__ b, int nx, int ny, int nz,
+                                                  float *__restrict__ out) {
+    __shared__ float w[432];
+    load_head_weights(W, w);
+    const float bias = b[0];
     const long n = (long)nx * ny * nz;
     for (long v = blockIdx.x * (long)blockDim.x + threadIdx.x; v < n; v += (long)gridDim.x * blockDim.x) {
         int x, y, z;
@@ -388,16 +408,21 @@ __global__ void k_head_fwd(const bf16_t *__restrict__ y0, const float *__restric
         for (int t = 0; t < 27; t++) {
             int xx, yy, zz;
             if (!tap_neighbour(t, x, y, z, nx, ny, nz, xx, yy, zz)) continue;
-            const bf16_t *p = y0 + (((long)zz * ny + yy) * nx + xx) * 16;
-            for (int c = 0; c < 16; c++) s += bf2f(f2bf(W[t * 16 + c])) * bf2f(p[c]);
+            float a[16];
+            unpack16(y0 + (((long)zz * ny + yy) * nx + xx) * 16, a);
+#pragma unroll
+            for (int c = 0; c < 16; c++) s += w[t * 16 + c] * a[c];
         }
-        out[v] = s + b[0];
+        out[v] = s + bias;
     }
 }
 
 // g_y0[v][c] = (y0 > 0) ? sum_t W[t][c] * g_out[v - off(t)] : 0
-__global__ void k_head_bwd_data(const float *__restrict__ gout, const float *__restrict__ W,
-                                const bf16_t *__restrict__ y0, int nx, int ny, int nz, bf16_t *__restrict__ gy0) {
+__global__ void __launch_bounds__(256) k_head_bwd_data(const float *__restrict__ gout, const float *__restrict__ W,
+                                                       const bf16_t *__restrict__ y0, int nx, int ny, int nz,
+                                                       bf16_t *__restrict__ gy0) {
+    __shared__ float w[432];
+    load_head_weights(W, w);
     const long n = (long)nx * ny * nz;
     for (long v = blockIdx.x * (long)blockDim.x + threadIdx.x; v < n; v += (long)gridDim.x * blockDim.x) {
         int x, y, z;
@@ -409,12 +434,12 @@ __global__ void k_head_bwd_data(const float *__restrict__ gout, const float *__r
             // the output voxel whose tap t reads v is v - off(t) = tap 26 - t of v
             if (!tap_neighbour(26 - t, x, y, z, nx, ny, nz, xx, yy, zz)) continue;
             const float g = gout[((long)zz * ny + yy) * nx + xx];
-            for (int c = 0; c < 16; c++) s[c] += bf2f(f2bf(W[t * 16 + c])) * g;
+#pragma unroll
+            for (int c = 0; c < 16; c++) s[c] += w[t * 16 + c] * g;
         }
-        for (int c = 0; c < 16; c++) {
-            const float m = bf2f(y0[v * 16 + c]);
-            gy0[v * 16 + c] = f2bf(m > 0.f ? s[c] : 0.f);
-        }
+        float m[16];
+        unpack16(y0 + v * 16, m);
+        for (int c = 0; c < 16; c++) gy0[v * 16 + c] = f2bf(m[c] > 0.f ? s[c] : 0.f);
     }
 }
 
