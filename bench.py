@@ -82,6 +82,28 @@ def profile_path(g, c, steps):
     return prof, stats
 
 
+def mixed_vs_fp64(g, c, mode):
+    """One step from the same state with the headline Poisson path and with
+    the plain FP64 one: relative L2 difference of the fields after the step
+    (both solve to the same 1e-8 residual)."""
+    from turbins_amd import abi
+    names = ["U", "V", "W", "C", "DPDX", "DPDY", "DPDZ", "PHI"]
+    state = {n: g.download(n) for n in names}
+    g.set_poisson_mode(mode, 1e-8)
+    g.step(c["dt"])
+    a = {n: g.download(n) for n in names}
+    for n in names:
+        g.upload(n, state[n])
+    g.set_poisson_mode(abi.POIS_FP64_PCG, 1e-8)
+    g.step(c["dt"])
+    out = {}
+    for n in names:
+        b = g.download(n)
+        den = float(np.linalg.norm(b))
+        out[n] = float(np.linalg.norm(a[n] - b) / den) if den > 0 else 0.0
+    return out
+
+
 def roofline(prof, stats, c):
     NT = (c["nx"] + 1) * (c["ny"] + 1) * (c["nz"] + 1)
     ncell = c["nx"] * c["ny"] * c["nz"]
@@ -187,6 +209,7 @@ def main():
     head_sps = args.steps / t_head
     prof, st_prof = profile_path(g, c, max(2, min(args.steps, 4)))
     gate = g.net_gate() if learned else None
+    mixed_err = mixed_vs_fp64(g, c, mode)
 
     def pois_summary(stats):
         its = [p.iterations for st in stats for p in st.pois]
@@ -244,6 +267,7 @@ def main():
             "helmholtz_iterations_mean": round(float(np.mean(helm_its)), 2),
             "helmholtz_method": "red-black Gauss-Seidel sweeps (BiCGStab fallback), rtol 1e-12",
             "solve_ms_per_step_plain": phase_ms(st_plain),
+            "headline_vs_fp64_step_rel_l2": mixed_err,
             "solve_ms_per_step_headline": phase_ms(st_head),
             "precision": "FP64 fields/ENO/Helmholtz/outer Poisson; FP32 multigrid in the headline path",
         },

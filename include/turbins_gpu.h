@@ -85,6 +85,16 @@ enum {
 #define TB_POIS_MIXED_IR 1      /* FP32 V-cycles inside FP64 iterative refinement */
 #define TB_POIS_LEARNED_IR 2    /* learned initial guess + mode 1, fallback to mode 0 */
 
+/* Level-set reinitialisation, near-interface gradient (Levelset.c:426-456):
+ * the reference drops a central difference whose stencil width |DeltaX| is
+ * below an absolute 0.01 (Levelset.c:282,428-448).  That also drops interior
+ * differences once 2*d_c < 0.01 (more than 200 cells over a unit length),
+ * which sets D = phi/0.01 next to a flat interface and makes the pseudo-time
+ * march diverge from the bottom wall.  SCALED takes min(0.01, 1.5*d_c) as the
+ * guard; it equals the reference bit for bit whenever every d_c >= 0.01/1.5. */
+#define TB_REINIT_SCALED 0
+#define TB_REINIT_REFERENCE 1
+
 typedef struct tb_config {
     int32_t nx, ny, nz;         /* physical cells in x, y, z (params->NX..NZ) */
     double lx, ly, lz;          /* domain lengths; origin at 0 (params->Lx..Lz) */
@@ -106,7 +116,8 @@ typedef struct tb_config {
     int32_t pois_mode;          /* TB_POIS_* */
     int32_t bottom_offset;      /* 1: topography is lifted by 2.001*dy_c[0] (Grid.c:913) */
     int32_t mg_pre, mg_post;    /* red-black Gauss-Seidel sweeps per V-cycle side */
-    int32_t reserved[8];
+    int32_t reinit_guard;       /* TB_REINIT_SCALED (0) or TB_REINIT_REFERENCE */
+    int32_t reserved[7];
 } tb_config;
 
 /* One immersed-boundary node (ImmersedNode, DataTypes.h:520-546), flattened. */

@@ -220,9 +220,19 @@ static int levelset_reinit(ora_ctx *o, const double *p0, double *p, int max_iter
                     if (((p_p * p_w) < 0.0) || ((p_p * p_e) < 0.0) || ((p_p * p_n) < 0.0) ||
                         ((p_p * p_s) < 0.0) || ((p_p * p_b) < 0.0) || ((p_p * p_f) < 0.0)) {
                         const double tol = 0.01;
-                        double Dphi_dx = (fabs(DeltaX) > tol) ? (p_e - p_w) / DeltaX : 0.0;
-                        double Dphi_dy = (fabs(DeltaY) > tol) ? (p_n - p_s) / DeltaY : 0.0;
-                        double Dphi_dz = (fabs(DeltaZ) > tol) ? (p_f - p_b) / DeltaZ : 0.0;
+                        /* TB_REINIT_SCALED: the stencil-width guard is
+                         * min(tol, 1.5 d_c), so an interior central difference
+                         * (width 2 d_c) is never dropped; identical to the
+                         * reference's absolute tol whenever every d_c >= tol/1.5 */
+                        double gx = tol, gy = tol, gz = tol;
+                        if (o->cfg.reinit_guard == TB_REINIT_SCALED) {
+                            gx = ORA_MIN(tol, 1.5 * o->dx_c[i]);
+                            gy = ORA_MIN(tol, 1.5 * o->dy_c[j]);
+                            gz = ORA_MIN(tol, 1.5 * o->dz_c[k]);
+                        }
+                        double Dphi_dx = (fabs(DeltaX) > gx) ? (p_e - p_w) / DeltaX : 0.0;
+                        double Dphi_dy = (fabs(DeltaY) > gy) ? (p_n - p_s) / DeltaY : 0.0;
+                        double Dphi_dz = (fabs(DeltaZ) > gz) ? (p_f - p_b) / DeltaZ : 0.0;
                         double Dt = sqrt(Dphi_dx * Dphi_dx + Dphi_dy * Dphi_dy + Dphi_dz * Dphi_dz);
                         if (Dt < tol) Dt = tol;
                         double D = p_p / Dt;

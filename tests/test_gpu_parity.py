@@ -68,6 +68,27 @@ def test_ib_tables_bit_exact(geo_pair):
             assert list(a.image) == list(b.image) and list(a.control) == list(b.control)
 
 
+@pytest.mark.parametrize("guard", [abi.REINIT_SCALED, abi.REINIT_REFERENCE])
+def test_reinit_fine_grid_bit_exact(guard):
+    """128x256x16 of the multi-bump topography (2 dy < 0.01): both
+    stencil-width guards of the near-interface rule, SDF bit-exact."""
+    from turbins_amd import Solver
+    c = W.case("turbidity1024", nx=128, ny=256, nz=16)
+    cfg = W.config_for(c, reinit_guard=guard)
+    g, o = Solver(cfg), oracle.Oracle(cfg)
+    h = W.topography(c)
+    g.setup_geometry(h)
+    o.setup_geometry(h)
+    assert g.reinit_iterations() == o.reinit_iterations()
+    a, b = g.download("SDF_C"), o.download("SDF_C")
+    assert np.array_equal(a, b), np.max(np.abs(a - b))
+    if guard == abi.REINIT_SCALED:
+        for q in range(4):
+            assert np.array_equal(g.flags(q), o.flags(q))
+    g.close()
+    o.close()
+
+
 def _random_state(c, seed=11):
     NT = (c["nx"] + 1) * (c["ny"] + 1) * (c["nz"] + 1)
     f = {}

@@ -295,6 +295,44 @@ def test_flat_bottom_reinit_is_a_fixed_point_and_layer_is_analytic():
     assert np.all(fl[:-1, 2:-1, :-1] == abi.FLUID)
 
 
+def _reinit_sdf(ny, guard):
+    c = W.case("turbidity1024", nx=128, ny=ny, nz=16)
+    cfg = W.config_for(c)
+    cfg.reinit_guard = guard
+    o = oracle.Oracle(cfg)
+    h = W.topography(c)
+    o.setup_geometry(h)
+    sh = o.shape
+    sdf = o.download("SDF_C").reshape(sh)[:-1, :-1, :-1]
+    dy = c["ly"] / ny
+    vert = o.grid("yc")[None, :-1, None] - (h[:-1, None, :-1] + 2.001 * dy)
+    return sdf, vert, dy
+
+
+def test_reinit_guard_scaled_equals_reference_on_coarse_grids():
+    """With every spacing >= 0.01/1.5 (ny = 128 over a unit height) the
+    scaled stencil-width guard is the reference's absolute 0.01 exactly."""
+    a, _, _ = _reinit_sdf(128, abi.REINIT_SCALED)
+    b, _, _ = _reinit_sdf(128, abi.REINIT_REFERENCE)
+    assert np.array_equal(a, b)
+
+
+def test_reinit_guard_fine_grid():
+    """At ny = 256 (2 dy < 0.01) the absolute guard drops the interior y
+    difference next to a near-flat bottom, D = phi/0.01 overshoots and the
+    march diverges from the bottom wall (Levelset.c:282,426-456 as written);
+    the scaled guard keeps a distance function: the sign of y - h everywhere,
+    and within a fraction of a cell of it at the interface."""
+    sdf, vert, dy = _reinit_sdf(256, abi.REINIT_REFERENCE)
+    assert np.abs(sdf).max() > 1e3
+    sdf, vert, dy = _reinit_sdf(256, abi.REINIT_SCALED)
+    assert np.abs(sdf).max() < 1.0
+    far = np.abs(vert) > 1e-9
+    assert np.array_equal(np.sign(sdf[far]), np.sign(vert[far]))
+    band = np.abs(vert) < 2 * dy
+    assert np.abs(sdf - vert)[band].max() < 0.5 * dy
+
+
 @pytest.mark.parametrize("name", ["bump_small", "turbidity_small"])
 def test_classification_brute_force(name):
     """IMMERSED <=> sdf < 0 with a sign change to a (clamped) 6-neighbour;

@@ -6,6 +6,7 @@ OUTSIDE, SOLID, FLUID, IMMERSED, BOUNDARY, BUFFER_SOLID = 1, 2, 3, 5, 7, 11
 SALINITY, PARTICLE = 1, 2
 POIS_FP64_PCG, POIS_MIXED_IR, POIS_LEARNED_IR = 0, 1, 2
 HELM_GS, HELM_BICGSTAB = 0, 1
+REINIT_SCALED, REINIT_REFERENCE = 0, 1
 
 KERNELS = ["ENO_MOM", "ENO_CONC", "HELM_OP", "BICG_UPD", "POIS_OP", "RBGS0", "RBGS0_F32",
            "PCG_UPD", "PROJECT", "RHS", "NET", "CG32_OP", "CG32_UPD", "HELM_GS"]
@@ -38,7 +39,8 @@ class Config(C.Structure):
         ("pois_mode", C.c_int32),
         ("bottom_offset", C.c_int32),
         ("mg_pre", C.c_int32), ("mg_post", C.c_int32),
-        ("reserved", C.c_int32 * 8),
+        ("reinit_guard", C.c_int32),
+        ("reserved", C.c_int32 * 7),
     ]
 
 

@@ -72,7 +72,8 @@ double tb_elapsed_ms(tb_ctx *c) {
 }
 
 extern "C" tb_ctx *tb_create(const tb_config *cfg) {
-    if (!cfg || cfg->nx < 4 || cfg->ny < 4 || cfg->nz < 4 || cfg->eno_order < 1 || cfg->eno_order > 3)
+    if (!cfg || cfg->nx < 4 || cfg->ny < 4 || cfg->nz < 4 || cfg->eno_order < 1 || cfg->eno_order > 3 ||
+        (cfg->reinit_guard != TB_REINIT_SCALED && cfg->reinit_guard != TB_REINIT_REFERENCE))
         return nullptr;
     tb_ctx *c = new tb_ctx();
     c->cfg = *cfg;

@@ -45,7 +45,7 @@ __global__ void k_ls_deriv(Dims d, GridDev g, const double *__restrict__ p, doub
 // Levelset_do_Godunov (Levelset.c:267-494), Russo-Smereka interface fix
 __global__ void k_ls_godunov(Dims d, GridDev g, const double *__restrict__ p0, double *__restrict__ p,
                              const double *__restrict__ Dx, const double *__restrict__ Dy,
-                             const double *__restrict__ Dz, double dt, Tiles t) {
+                             const double *__restrict__ Dz, double dt, int scaled_guard, Tiles t) {
     int i, j, k;
     if (!tile_point(t, i, j, k)) return;   // box is [0,NI) x [0,NJ) x [0,NK)
     int NI = d.NX - 1, NJ = d.NY - 1, NK = d.NZ - 1;
@@ -80,9 +80,17 @@ __global__ void k_ls_godunov(Dims d, GridDev g, const double *__restrict__ p0, d
     if (((p_p * p_w) < 0.0) || ((p_p * p_e) < 0.0) || ((p_p * p_n) < 0.0) || ((p_p * p_s) < 0.0) ||
         ((p_p * p_b) < 0.0) || ((p_p * p_f) < 0.0)) {
         const double tol = 0.01;
-        double Dphi_dx = (fabs(DeltaX) > tol) ? (p_e - p_w) / DeltaX : 0.0;
-        double Dphi_dy = (fabs(DeltaY) > tol) ? (p_n - p_s) / DeltaY : 0.0;
-        double Dphi_dz = (fabs(DeltaZ) > tol) ? (p_f - p_b) / DeltaZ : 0.0;
+        // scaled guard: min(tol, 1.5 d_c) keeps interior central differences
+        // on fine grids; equal to the absolute tol when every d_c >= tol/1.5
+        double gx = tol, gy = tol, gz = tol;
+        if (scaled_guard) {
+            gx = dmin(tol, 1.5 * g.dxc[i]);
+            gy = dmin(tol, 1.5 * g.dyc[j]);
+            gz = dmin(tol, 1.5 * g.dzc[k]);
+        }
+        double Dphi_dx = (fabs(DeltaX) > gx) ? (p_e - p_w) / DeltaX : 0.0;
+        double Dphi_dy = (fabs(DeltaY) > gy) ? (p_n - p_s) / DeltaY : 0.0;
+        double Dphi_dz = (fabs(DeltaZ) > gz) ? (p_f - p_b) / DeltaZ : 0.0;
         double Dt = sqrt(Dphi_dx * Dphi_dx + Dphi_dy * Dphi_dy + Dphi_dz * Dphi_dz);
         if (Dt < tol) Dt = tol;
         double D = p_p / Dt;
@@ -528,7 +536,7 @@ int tb_geometry_run(tb_ctx *c, const double *h_xz) {
             hipLaunchKernelGGL(k_ls_deriv, tiles_grid(all), tiles_block(), 0, c->s, d, c->g, sdf_c, Dx, Dy, Dz,
                                1.0 / c->dEta, 1.0 / c->dXi, 1.0 / c->dPhi, all);
             hipLaunchKernelGGL(k_ls_godunov, tiles_grid(inner), tiles_block(), 0, c->s, d, c->g, p0, sdf_c, Dx, Dy,
-                               Dz, dt, inner);
+                               Dz, dt, c->cfg.reinit_guard == TB_REINIT_SCALED ? 1 : 0, inner);
             hipLaunchKernelGGL(k_ls_norm, tiles_grid(all), tiles_block(), 0, c->s, d, c->g, sdf_c, pold, part, all);
             hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, c->s, part, all.per_xcd * NXCD, c->scal + S_TMP0);
             TB_LAUNCH_CHECK(c);
