@@ -205,9 +205,18 @@ int tb_set_rhs(tb_ctx *ctx, int q, double dt);
  * (Velocity.c:374-443, Conc.c:293-354) */
 int tb_helmholtz_apply(tb_ctx *ctx, int q, double dt, int f_in, int f_out);
 /* Velocity_solve (Velocity.c:306-324, Velocity.h:36) / Conc_solve
- * (Conc.c:840-859, Conc.h:29): BiCGStab, no preconditioner, previous field
- * as the initial guess (Solver.c:39-86, 269-322) */
+ * (Conc.c:840-859, Conc.h:29): previous field as the initial guess, stop at
+ * ||b - A x|| <= helm_rtol ||b|| on the true residual, as the reference's KSP
+ * (Solver.c:39-86, 269-322).  Method per tb_set_helmholtz_method: red-black
+ * Gauss-Seidel sweep pairs with BiCGStab taking over if they stall
+ * (TB_HELM_GS, default), or the reference's unpreconditioned BiCGStab
+ * (TB_HELM_BICGSTAB). */
 int tb_helmholtz_solve(tb_ctx *ctx, int q, double dt, tb_solve_stats *st);
+/* The smoother of TB_HELM_GS by itself (no reference counterpart; exposed
+ * for parity tests): f_x <- `pairs` red-black sweep pairs of
+ * A_q(dt) x = f_b (red = (i+j+k) even first; IMMERSED rows skipped, then
+ * one Jacobi update of the IMMERSED rows per pair), in place. */
+int tb_helmholtz_gs_sweeps(tb_ctx *ctx, int q, double dt, int f_b, int f_x, int pairs);
 /* Pressure_set_RHS (Pressure.c:171-276, Pressure.h:9) */
 int tb_poisson_rhs(tb_ctx *ctx, double dt);
 /* y = A_p x, rows of Pressure_form_LHS_matrix (Pressure.c:691-936) */

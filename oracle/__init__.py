@@ -45,6 +45,7 @@ def lib():
         L.ora_eno_conv.argtypes = [P, C.c_int]
         L.ora_set_rhs.argtypes = [P, C.c_int, C.c_double]
         L.ora_helmholtz_apply.argtypes = [P, C.c_int, C.c_double, P, P]
+        L.ora_helmholtz_gs_sweeps.argtypes = [P, C.c_int, C.c_double, P, P, C.c_int]
         L.ora_helmholtz_solve.argtypes = [P, C.c_int, C.c_double, C.c_double, C.c_int,
                                           C.POINTER(abi.SolveStats)]
         L.ora_poisson_rhs.argtypes = [P, C.c_double]
@@ -153,6 +154,12 @@ class Oracle:
         y = np.zeros(self.NT)
         self.L.ora_helmholtz_apply(self.h, q, dt, _ptr(x), _ptr(y))
         return y
+
+    def helmholtz_gs_sweeps(self, q, dt, b, x, pairs):
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.array(x, dtype=np.float64)
+        self.L.ora_helmholtz_gs_sweeps(self.h, q, dt, _ptr(b), _ptr(x), pairs)
+        return x
 
     def helmholtz_solve(self, q, dt, rtol=1e-14, maxit=500):
         st = abi.SolveStats()

@@ -502,6 +502,52 @@ int ora_helmholtz_apply(ora_ctx *o, int q, double dt, const double *x, double *y
     return 0;
 }
 
+/* Gauss-Seidel value of row r of the assembled rows: (b_r - sum_{c != r}
+ * a_rc x_c) / a_rr, off-diagonal terms in ascending column order */
+static double gs_row(const ora_mat *m, size_t r, const double *b, const double *x) {
+    double s = 0.0, dg = 1.0;
+    for (int e = 0; e < m->cnt[r]; e++) {
+        int64_t c = m->col[r * 9 + e];
+        double v = m->val[r * 9 + e];
+        if ((size_t)c == r) dg = v;
+        else s += v * x[c];
+    }
+    return (b[r] - s) / dg;
+}
+
+/* Red-black Gauss-Seidel sweep pairs on the assembled Helmholtz rows (the
+ * smoother of the device's Helmholtz solve; the reference solves with KSP):
+ * red = (i + j + k) even first, each colour in place, IMMERSED rows skipped
+ * by both colours and then updated once per pair by Jacobi. */
+int ora_helmholtz_gs_sweeps(ora_ctx *o, int q, double dt, const double *b, double *x, int pairs) {
+    if (q < 0 || q > 3 || pairs < 0) return -1;
+    ensure_helm(o, q, dt);
+    const ora_mat *m = &o->helm[q];
+    const uint8_t *st = o->st[q];
+    double *tmp = dalloc(o->nib[q] > 0 ? (size_t)o->nib[q] : 1);
+    for (int p = 0; p < pairs; p++) {
+        for (int color = 0; color < 2; color++) {
+#pragma omp parallel for schedule(static)
+            for (int k = 0; k < o->NZ; k++)
+                for (int j = 0; j < o->NY; j++)
+                    for (int i = (j + k + color) & 1; i < o->NX; i += 2) {
+                        size_t id = ora_idx(o, i, j, k);
+                        if (st[id] != TB_IMMERSED) x[id] = gs_row(m, id, b, x);
+                    }
+        }
+        for (int n = 0; n < o->nib[q]; n++) {
+            const tb_ib_node *nd = &o->ib[q][n];
+            tmp[n] = gs_row(m, ora_idx(o, nd->i, nd->j, nd->k), b, x);
+        }
+        for (int n = 0; n < o->nib[q]; n++) {
+            const tb_ib_node *nd = &o->ib[q][n];
+            x[ora_idx(o, nd->i, nd->j, nd->k)] = tmp[n];
+        }
+    }
+    free(tmp);
+    return 0;
+}
+
 static double dot(const double *a, const double *b, size_t n) {
     double s = 0.0;
 #pragma omp parallel for reduction(+ : s) schedule(static)

@@ -49,6 +49,7 @@ void ora_gauss_jordan(int n, const double *a, double *a_inv);
 int ora_eno_conv(ora_ctx *o, int q);
 int ora_set_rhs(ora_ctx *o, int q, double dt);
 int ora_helmholtz_apply(ora_ctx *o, int q, double dt, const double *x, double *y);
+int ora_helmholtz_gs_sweeps(ora_ctx *o, int q, double dt, const double *b, double *x, int pairs);
 int ora_helmholtz_matrix(ora_ctx *o, int q, double dt, int32_t *cnt, int64_t *col, double *val);
 int ora_poisson_matrix(ora_ctx *o, int32_t *cnt, int64_t *col, double *val);
 int ora_helmholtz_solve(ora_ctx *o, int q, double dt, double rtol, int maxit,

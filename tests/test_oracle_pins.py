@@ -225,6 +225,29 @@ def test_helmholtz_solve_matches_scipy(q):
     assert np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref) < 1e-9, st.res
 
 
+@pytest.mark.parametrize("q", [0, 1, 2, 3])
+def test_helmholtz_gs_sweeps_converge_to_the_solution(q):
+    """The red-black sweep pairs (with the Jacobi update of the IB rows) are a
+    convergent splitting of the assembled Helmholtz rows: the direct solution
+    is their fixed point and repeated pairs reach it from a random start."""
+    c = W.case("turbidity_small", nx=16, ny=16, nz=8)
+    o = oracle.Oracle(W.config_for(c))
+    o.setup_geometry(W.topography(c))
+    dt = 1e-3
+    rng = np.random.default_rng(3 + q)
+    b = rng.standard_normal(o.NT)
+    A = o.helmholtz_matrix(q, dt).tocsc()
+    x_ref = sla.spsolve(A, b)
+    one = o.helmholtz_gs_sweeps(q, dt, b, x_ref, 1)
+    assert np.max(np.abs(one - x_ref)) <= 1e-13 * np.max(np.abs(x_ref))
+    x = o.helmholtz_gs_sweeps(q, dt, b, rng.standard_normal(o.NT), 30)
+    assert np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref) < 1e-12
+    # and one pair of an unconverged state is not the identity
+    x3 = o.helmholtz_gs_sweeps(q, dt, b, np.zeros(o.NT), 3)
+    x4 = o.helmholtz_gs_sweeps(q, dt, b, x3, 1)
+    assert np.linalg.norm(x4 - x_ref) < 0.2 * np.linalg.norm(x3 - x_ref)
+
+
 def test_helmholtz_rows_on_constant_field():
     """Interior FLUID rows: (1/dt) I - (1/Re) Laplacian applied to a constant
     gives const/dt (no-slip wall rows differ, IB rows give ap_solid*(1+sum)."""

@@ -51,6 +51,7 @@ def lib():
     L.tb_eno_conv.argtypes = [P, C.c_int]
     L.tb_set_rhs.argtypes = [P, C.c_int, C.c_double]
     L.tb_helmholtz_apply.argtypes = [P, C.c_int, C.c_double, C.c_int, C.c_int]
+    L.tb_helmholtz_gs_sweeps.argtypes = [P, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int]
     L.tb_helmholtz_solve.argtypes = [P, C.c_int, C.c_double, C.POINTER(abi.SolveStats)]
     L.tb_poisson_rhs.argtypes = [P, C.c_double]
     L.tb_poisson_apply.argtypes = [P, C.c_int, C.c_int]
@@ -185,6 +186,15 @@ class Solver:
     def helmholtz_apply(self, q, dt, x):
         self.upload("SCRATCH0", x)
         self._ok(self.L.tb_helmholtz_apply(self.h, q, dt, abi.F["SCRATCH0"], abi.F["SCRATCH1"]), "tb_helmholtz_apply")
+        return self.download("SCRATCH1")
+
+    def helmholtz_gs_sweeps(self, q, dt, b, x, pairs):
+        """`pairs` red-black Gauss-Seidel sweep pairs of A_q(dt) x = b from x
+        (the smoother of the Helmholtz solve), through SCRATCH0/SCRATCH1."""
+        self.upload("SCRATCH0", b)
+        self.upload("SCRATCH1", x)
+        self._ok(self.L.tb_helmholtz_gs_sweeps(self.h, q, dt, abi.F["SCRATCH0"], abi.F["SCRATCH1"], pairs),
+                 "tb_helmholtz_gs_sweeps")
         return self.download("SCRATCH1")
 
     def helmholtz_solve(self, q, dt):

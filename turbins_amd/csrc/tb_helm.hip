@@ -241,109 +241,304 @@ static int helm_apply(tb_ctx *c, const HelmOp &H, const double *x, double *y, bo
 // residual of the unscaled operator, and hands over to BiCGStab if the
 // sweeps stop contracting.
 
-struct HelmRow {
-    double ab, as, aw, ae, an, af, ap;   // ap: diagonal without 1/dt, walls applied
-    bool hb, hs, hw, he, hn, hf;
+// Per-axis parts of a row, with the terms and wall rules of k_helm_km
+// (Velocity.c:449-1471, Conc.c:390-730): d is the axis's share of the
+// diagonal with its wall rules folded in, am / ap the minus / plus neighbour
+// coefficients (zero where the neighbour is absent).  The diagonal of the
+// Gauss-Seidel update is (d_x + d_y) + d_z + 1/dt; it differs from the
+// assembled row's only in rounding (the rows sum the same terms in another
+// order), so a sweep's fixed point solves the assembled system to ~1e-16.
+struct AxisGS {
+    double d, am, ap;
 };
 
-// the coefficients of row (i, j, k), with the same terms and wall rules as
-// k_helm_km (Velocity.c:449-1471, Conc.c:390-730)
-__device__ inline void helm_row(const HelmOp &H, int i, int j, int k, HelmRow &R) {
-    const Dims &d = H.d;
-    const GridDev &g = H.g;
-    const int NX = d.NX, NY = d.NY, NZ = d.NZ;
-    const int q = H.q;
-    const double Cx = H.Cx, Cy = H.Cy, Cz = H.Cz;
-    const int im1 = i > 0 ? i - 1 : 0, ip1 = i + 1 < NX ? i + 1 : NX - 1;
-    const int jm1 = j > 0 ? j - 1 : 0, jp1 = j + 1 < NY ? j + 1 : NY - 1;
-    const int km1 = k > 0 ? k - 1 : 0, kp1 = k + 1 < NZ ? k + 1 : NZ - 1;
-    double mpx, mex, mwx, mpy, mny, msy, mpz, mbz, mfz;
-    if (q == TB_Q_U) {
-        mpx = g.mxu[i]; mex = g.mxc[i]; mwx = g.mxc[im1]; mpy = g.myc[j]; mny = g.myv[jp1]; msy = g.myv[j];
-    } else if (q == TB_Q_V) {
-        mpx = g.mxc[i]; mex = g.mxu[ip1]; mwx = g.mxu[i]; mpy = g.myv[j]; mny = g.myc[j]; msy = g.myc[jm1];
-    } else {
-        mpx = g.mxc[i]; mex = g.mxu[ip1]; mwx = g.mxu[i]; mpy = g.myc[j]; mny = g.myv[jp1]; msy = g.myv[j];
-    }
-    if (q == TB_Q_W) { mpz = g.mzw[k]; mbz = g.mzc[km1]; mfz = g.mzc[k]; }
-    else { mpz = g.mzc[k]; mbz = g.mzw[k]; mfz = g.mzw[kp1]; }
-    const double axy = -Cx * mpx * (mex + mwx) - Cy * mpy * (mny + msy);
-    double ap = axy - Cz * mpz * (mbz + mfz);
-    R.aw = Cx * mpx * mwx; R.ae = Cx * mpx * mex;
-    R.an = Cy * mpy * mny; R.as = Cy * mpy * msy;
-    R.ab = Cz * mpz * mbz; R.af = Cz * mpz * mfz;
-    int rs, rw, re, rn;
-    const int *W = H.wall;
-    if (q == TB_Q_U) {
-        R.hs = j != 0;       rs = W[2] ? 1 : 2;
-        R.hw = i - 1 >= 1;   rw = 0;
-        R.he = i + 1 <= NX - 2; re = 0;
-        R.hn = j != NY - 2;  rn = W[3] ? 1 : 2;
-    } else if (q == TB_Q_V) {
-        R.hs = j - 1 >= 1;   rs = 0;
-        R.hw = i != 0;       rw = W[0] ? 1 : 2;
-        R.he = i < NX - 2;   re = (i == NX - 2) ? (W[1] ? 1 : 2) : 0;
-        R.hn = j + 1 <= NY - 2; rn = 0;
-    } else if (q == TB_Q_W) {
-        R.hs = j != 0;       rs = W[2] ? 1 : 2;
-        R.hw = i != 0;       rw = W[0] ? 1 : 2;
-        R.he = i < NX - 2;   re = (i == NX - 2) ? (W[1] ? 1 : 2) : 0;
-        R.hn = j != NY - 2;  rn = W[3] ? 1 : 2;
-    } else {
-        R.hs = j > 0;        rs = 2;
-        R.hw = i > 0;        rw = 2;
-        R.he = i < NX - 2;   re = 2;
-        R.hn = j < NY - 2;   rn = 2;
-    }
-    double cn = R.an;
-    if (q == TB_Q_C && H.particle) {
-        double bb = H.robin_b_scale * mny;
-        cn = R.an * (bb - H.vs_abs) / (bb + H.vs_abs);
-    }
-    const int rb = (q == TB_Q_W) ? 0 : (q == TB_Q_C ? 2 : (W[4] ? 1 : 2));
-    const int rf = (q == TB_Q_W) ? 0 : (q == TB_Q_C ? 2 : (W[5] ? 1 : 2));
-    if (q == TB_Q_W) { R.hb = k - 1 >= 1; R.hf = k + 1 <= NZ - 2; }
-    else if (q == TB_Q_C) { R.hb = k > 0; R.hf = k < NZ - 2; }
-    else { R.hb = k != 0; R.hf = k != NZ - 2; }
-    if (!R.hb) ap = wallmod(ap, R.ab, rb);
-    if (!R.hs) ap = wallmod(ap, R.as, rs);
-    if (!R.hw) ap = wallmod(ap, R.aw, rw);
-    if (!R.he) ap = wallmod(ap, R.ae, re);
-    if (!R.hn) ap = (q == TB_Q_C) ? ap + cn : wallmod(ap, R.an, rn);
-    if (!R.hf) ap = wallmod(ap, R.af, rf);
-    R.ap = ap;
+__device__ inline AxisGS axis_gs(double d, double am, bool hm, int rm, double ap, double wp, bool hp, int rp) {
+    AxisGS a;
+    if (!hm) d = wallmod(d, am, rm);
+    if (!hp) d = wallmod(d, wp, rp);
+    a.d = d;
+    a.am = hm ? am : 0.0;
+    a.ap = hp ? ap : 0.0;
+    return a;
 }
 
-// one colour of a red-black sweep over the non-IMMERSED rows; threads tile
-// (ceil(NX/2), NY, NZ) and take i = 2 ih + ((j + k + colour) & 1)
-__global__ void __launch_bounds__(256) k_helm_gs(HelmOp H, double *__restrict__ x, const double *__restrict__ b,
-                                                 int color, Tiles t) {
-    int ih, j, k;
-    if (!tile_point(t, ih, j, k)) return;
-    const Dims &d = H.d;
-    const int i = 2 * ih + ((j + k + color) & 1);
-    if (i >= d.NX) return;
-    const long id = gidx(d, i, j, k);
-    // neighbour loads first, clamped to the point itself at the box faces
-    const long os = (j > 0) ? d.px : 0, on = (j + 1 < d.NY) ? d.px : 0;
-    const long ow = (i > 0) ? 1 : 0, oe = (i + 1 < d.NX) ? 1 : 0;
-    const long ob = (k > 0) ? d.plane : 0, of = (k + 1 < d.NZ) ? d.plane : 0;
-    const double xs = x[id - os], xw = x[id - ow], xe = x[id + oe], xn = x[id + on];
-    const double xb = x[id - ob], xf = x[id + of];
-    const double bi = b[id];
-    const int s = H.st[id];
-    if (s == TB_IMMERSED) return;
-    HelmRow R;
-    helm_row(H, i, j, k, R);
+__device__ inline AxisGS axis_x(const HelmOp &H, int i) {
+    const GridDev &g = H.g;
+    const int NX = H.d.NX, q = H.q;
+    const int *W = H.wall;
+    const int im1 = i > 0 ? i - 1 : 0, ip1 = i + 1 < NX ? i + 1 : NX - 1;
+    double mpx, mex, mwx;
+    if (q == TB_Q_U) { mpx = g.mxu[i]; mex = g.mxc[i]; mwx = g.mxc[im1]; }
+    else { mpx = g.mxc[i]; mex = g.mxu[ip1]; mwx = g.mxu[i]; }
+    const double Cx = H.Cx;
+    const double t = -Cx * mpx * (mex + mwx), aw = Cx * mpx * mwx, ae = Cx * mpx * mex;
+    if (q == TB_Q_U) return axis_gs(t, aw, i - 1 >= 1, 0, ae, ae, i + 1 <= NX - 2, 0);
+    if (q == TB_Q_C) return axis_gs(t, aw, i > 0, 2, ae, ae, i < NX - 2, 2);
+    return axis_gs(t, aw, i != 0, W[0] ? 1 : 2, ae, ae, i < NX - 2, (i == NX - 2) ? (W[1] ? 1 : 2) : 0);
+}
+
+__device__ inline AxisGS axis_y(const HelmOp &H, int j) {
+    const GridDev &g = H.g;
+    const int NY = H.d.NY, q = H.q;
+    const int *W = H.wall;
+    const int jm1 = j > 0 ? j - 1 : 0, jp1 = j + 1 < NY ? j + 1 : NY - 1;
+    double mpy, mny, msy;
+    if (q == TB_Q_V) { mpy = g.myv[j]; mny = g.myc[j]; msy = g.myc[jm1]; }
+    else { mpy = g.myc[j]; mny = g.myv[jp1]; msy = g.myv[j]; }
+    const double Cy = H.Cy;
+    const double t = -(Cy * mpy * (mny + msy)), as = Cy * mpy * msy, an = Cy * mpy * mny;
+    double wn = an;   // top-wall coefficient: the Robin form for settling particles (Conc.c:560-585)
+    if (q == TB_Q_C && H.particle) {
+        const double bb = H.robin_b_scale * mny;
+        wn = an * (bb - H.vs_abs) / (bb + H.vs_abs);
+    }
+    if (q == TB_Q_V) return axis_gs(t, as, j - 1 >= 1, 0, an, wn, j + 1 <= NY - 2, 0);
+    if (q == TB_Q_C) return axis_gs(t, as, j > 0, 2, an, wn, j < NY - 2, 2);
+    return axis_gs(t, as, j != 0, W[2] ? 1 : 2, an, wn, j != NY - 2, W[3] ? 1 : 2);
+}
+
+__device__ inline AxisGS axis_z(const HelmOp &H, int k) {
+    const GridDev &g = H.g;
+    const int NZ = H.d.NZ, q = H.q;
+    const int *W = H.wall;
+    const int km1 = k > 0 ? k - 1 : 0, kp1 = k + 1 < NZ ? k + 1 : NZ - 1;
+    double mpz, mbz, mfz;
+    if (q == TB_Q_W) { mpz = g.mzw[k]; mbz = g.mzc[km1]; mfz = g.mzc[k]; }
+    else { mpz = g.mzc[k]; mbz = g.mzw[k]; mfz = g.mzw[kp1]; }
+    const double Cz = H.Cz;
+    const double t = -(Cz * mpz * (mbz + mfz)), ab = Cz * mpz * mbz, af = Cz * mpz * mfz;
+    if (q == TB_Q_W) return axis_gs(t, ab, k - 1 >= 1, 0, af, af, k + 1 <= NZ - 2, 0);
+    if (q == TB_Q_C) return axis_gs(t, ab, k > 0, 2, af, af, k < NZ - 2, 2);
+    return axis_gs(t, ab, k != 0, W[4] ? 1 : 2, af, af, k != NZ - 2, W[5] ? 1 : 2);
+}
+
+// Gauss-Seidel value of a row of status s from its own current value xp and
+// its six neighbour values (clamped to the point itself at the box faces,
+// where the coefficient is zero): IMMERSED rows keep xp, other non-fluid
+// rows take b / (ap_solid + 1/dt)
+__device__ inline double gs_value(const HelmOp &H, const AxisGS &X, const AxisGS &Y, const AxisGS &Z, int s,
+                                  double bi, double xp, double xb, double xs, double xw, double xe, double xn,
+                                  double xf) {
     const bool fluid = (s == TB_FLUID) || (H.q != TB_Q_C && s == TB_BUFFER_SOLID);
-    double o = 0.0;
-    o = R.hb ? o + R.ab * xb : o;
-    o = R.hs ? o + R.as * xs : o;
-    o = R.hw ? o + R.aw * xw : o;
-    o = R.he ? o + R.ae * xe : o;
-    o = R.hn ? o + R.an * xn : o;
-    o = R.hf ? o + R.af * xf : o;
-    x[id] = fluid ? (bi - o) / (R.ap + H.mod) : bi / (H.ap_solid + H.mod);
+    double o = Z.am * xb;
+    o = fma(Y.am, xs, o);
+    o = fma(X.am, xw, o);
+    o = fma(X.ap, xe, o);
+    o = fma(Y.ap, xn, o);
+    o = fma(Z.ap, xf, o);
+    const double num = fluid ? bi - o : bi;
+    const double den = fluid ? ((X.d + Y.d) + Z.d) + H.mod : H.ap_solid + H.mod;
+    const double v = num / den;
+    return s == TB_IMMERSED ? xp : v;
+}
+
+// One red-black sweep pair over the non-IMMERSED rows in a single pass:
+// xn = black(red(xo)), red = (i + j + k) even, updated first.  Every red
+// value reads only black neighbours of xo and every black value only red
+// neighbours of the red pass (a neighbour clamped to the point itself reads
+// the point's own current value), so the pair is the same iteration as two
+// in-place colour passes (equal to rounding: the diagonals are summed per
+// axis, see AxisGS), while xo and b are read from HBM once and xn written
+// once (the in-place form streams both arrays twice).
+//
+// A 256-thread block owns a 64 x 8 column tile and a chunk of GS_TK planes
+// and walks k upwards through two LDS rings of three planes: xo with a
+// two-point halo, and the red-pass values (black positions holding xo) with a
+// one-point halo.  Per plane m: red pass on plane m (tile + halo, recomputed
+// by the neighbouring tiles); black pass on plane m-1 (tile); store plane
+// m-1.  The chunk recomputes the red pass on one plane below and above.  The
+// xo plane, statuses and right-hand sides of the next plane are fetched into
+// registers one step ahead, so their latency overlaps the passes; row
+// coefficients come from per-axis tables built in LDS once per block.
+constexpr int GS_TX = 64, GS_TY = 8, GS_TK = 16;
+constexpr int GS_OX = GS_TX + 4, GS_OY = GS_TY + 4;   // xo ring: halo 2
+constexpr int GS_HX = GS_TX + 2, GS_HY = GS_TY + 2;   // red-pass ring: halo 1
+constexpr int GS_OZ = GS_TK + 4;                      // z table: k0-2 .. k0+GS_TK+1
+constexpr int GS_THREADS = 256;
+constexpr int GS_OLD_PER = (GS_OY * GS_OX + GS_THREADS - 1) / GS_THREADS;
+constexpr int GS_RED = GS_HY * (GS_HX / 2);           // red points of one halo plane
+constexpr int GS_RED_PER = (GS_RED + GS_THREADS - 1) / GS_THREADS;
+static_assert(GS_TX * GS_TY == 2 * GS_THREADS, "one black point per thread");
+static_assert(GS_TX % 2 == 0, "tile origins must be even");
+
+struct GsTiles {
+    int tx, ty, kc;
+    int n, per_xcd;
+};
+
+static GsTiles make_gs_tiles(const Dims &d) {
+    GsTiles t;
+    t.tx = (d.NX + GS_TX - 1) / GS_TX;
+    t.ty = (d.NY + GS_TY - 1) / GS_TY;
+    t.kc = (d.NZ + GS_TK - 1) / GS_TK;
+    t.n = t.tx * t.ty * t.kc;
+    t.per_xcd = (t.n + NXCD - 1) / NXCD;
+    return t;
+}
+
+// Neighbours are read without clamping at the box faces: every row whose
+// neighbour index leaves the box either has a zero coefficient there or is
+// not a FLUID row (the last plane of each axis is BOUNDARY), and the LDS
+// positions outside the box hold zeros or finite values of other planes.
+// Element offsets are 32-bit (the host checks the field size).
+__global__ void __launch_bounds__(GS_THREADS) k_helm_gs_pair(HelmOp H, const double *__restrict__ xo,
+                                                             double *__restrict__ xn, const double *__restrict__ b,
+                                                             GsTiles T) {
+    __shared__ double so[3][GS_OY * GS_OX];
+    __shared__ double sr[3][GS_HY * GS_HX];
+    __shared__ AxisGS cx[GS_OX], cy[GS_OY], cz[GS_OZ];
+    const Dims &d = H.d;
+    const int NX = d.NX, NY = d.NY, NZ = d.NZ, px = d.px;
+    const int plane = (int)d.plane;
+    const int bid = blockIdx.x;
+    const int lt = (bid % NXCD) * T.per_xcd + bid / NXCD;   // consecutive tiles share an XCD's L2
+    if (lt >= T.n) return;                                  // whole block
+    const int tx = lt % T.tx, rest = lt / T.tx;
+    const int i0 = tx * GS_TX, j0 = (rest % T.ty) * GS_TY;
+    const int k0 = (rest / T.ty) * GS_TK;
+    const int k1 = k0 + GS_TK < NZ ? k0 + GS_TK : NZ;
+    const int t = threadIdx.x;
+    const uint8_t *__restrict__ st = H.st;
+
+    for (int e = t; e < 3 * GS_OY * GS_OX; e += GS_THREADS) (&so[0][0])[e] = 0.0;
+    for (int e = t; e < 3 * GS_HY * GS_HX; e += GS_THREADS) (&sr[0][0])[e] = 0.0;
+    for (int e = t; e < GS_OX + GS_OY + GS_OZ; e += GS_THREADS) {
+        if (e < GS_OX) {
+            const int i = i0 - 2 + e;
+            if (i >= 0 && i < NX) cx[e] = axis_x(H, i);
+        } else if (e < GS_OX + GS_OY) {
+            const int j = j0 - 2 + (e - GS_OX);
+            if (j >= 0 && j < NY) cy[e - GS_OX] = axis_y(H, j);
+        } else {
+            const int k = k0 - 2 + (e - GS_OX - GS_OY);
+            if (k >= 0 && k < NZ) cz[e - GS_OX - GS_OY] = axis_z(H, k);
+        }
+    }
+    __syncthreads();   // zeros in place before the first planes are staged
+
+    // xo plane staging: in-plane offset of each of this thread's halo-2
+    // points (-1 outside the box); the LDS index is e itself
+    int ooff[GS_OLD_PER];
+#pragma unroll
+    for (int u = 0; u < GS_OLD_PER; u++) {
+        const int e = t + u * GS_THREADS;
+        const int r = e / GS_OX, cc = e - r * GS_OX;
+        const int i = i0 - 2 + cc, j = j0 - 2 + r;
+        ooff[u] = (e < GS_OY * GS_OX && i >= 0 && i < NX && j >= 0 && j < NY) ? j * px + i : -1;
+    }
+    double ro[GS_OLD_PER];
+    auto fetch_old = [&](int m) {
+#pragma unroll
+        for (int u = 0; u < GS_OLD_PER; u++) ro[u] = ooff[u] >= 0 ? xo[m * plane + ooff[u]] : 0.0;
+    };
+    auto put_old = [&](int m) {
+        double *S = so[m % 3];
+#pragma unroll
+        for (int u = 0; u < GS_OLD_PER; u++) {
+            const int e = t + u * GS_THREADS;
+            if (e < GS_OY * GS_OX) S[e] = ro[u];
+        }
+    };
+    // this thread's red points of the halo-1 plane: row, column pair, y
+    int rrow[GS_RED_PER], rc2[GS_RED_PER];
+#pragma unroll
+    for (int u = 0; u < GS_RED_PER; u++) {
+        const int e = t + u * GS_THREADS;
+        rrow[u] = e < GS_RED ? e / (GS_HX / 2) : 0;
+        rc2[u] = e < GS_RED ? e - rrow[u] * (GS_HX / 2) : -1;
+    }
+    // status (-1 outside the box) and b of the red points of plane m
+    int rs[GS_RED_PER];
+    double rb[GS_RED_PER];
+    auto fetch_red = [&](int m, int *s_, double *b_) {
+#pragma unroll
+        for (int u = 0; u < GS_RED_PER; u++) {
+            const int j = j0 - 1 + rrow[u];
+            const int i = i0 - 1 + 2 * rc2[u] + ((1 + j + m) & 1);   // i + j + m even (i0 is even)
+            s_[u] = -1;
+            b_[u] = 0.0;
+            if (rc2[u] >= 0 && i >= 0 && i < NX && j >= 0 && j < NY) {
+                const int id = m * plane + j * px + i;
+                s_[u] = st[id];
+                b_[u] = b[id];
+            }
+        }
+    };
+#pragma unroll
+    for (int u = 0; u < GS_RED_PER; u++) { rs[u] = -1; rb[u] = 0.0; }
+    // this thread's black point of plane k
+    const int br = t / (GS_TX / 2), bc2 = t % (GS_TX / 2);
+    const int bj = j0 + br;
+    int bs = -1;
+    double bbv = 0.0;
+    auto fetch_black = [&](int k, int &s_, double &b_) {
+        const int i = i0 + 2 * bc2 + ((1 + bj + k) & 1);
+        s_ = -1;
+        b_ = 0.0;
+        if (i < NX && bj < NY) {
+            const int id = k * plane + bj * px + i;
+            s_ = st[id];
+            b_ = b[id];
+        }
+    };
+
+    if (k0 >= 2) { fetch_old(k0 - 2); put_old(k0 - 2); }
+    if (k0 >= 1) { fetch_old(k0 - 1); put_old(k0 - 1); }
+    fetch_old(k0);
+    if (k0 >= 1) fetch_red(k0 - 1, rs, rb);
+    for (int m = k0 - 1; m <= k1; m++) {
+        if (m + 1 < NZ) put_old(m + 1);
+        if (m + 2 < NZ) fetch_old(m + 2);
+        int rs_n[GS_RED_PER];
+        double rb_n[GS_RED_PER];
+#pragma unroll
+        for (int u = 0; u < GS_RED_PER; u++) { rs_n[u] = -1; rb_n[u] = 0.0; }
+        if (m + 1 < NZ) fetch_red(m + 1, rs_n, rb_n);
+        int bs_n = -1;
+        double bb_n = 0.0;
+        if (m >= k0 && m < k1) fetch_black(m, bs_n, bb_n);
+        __syncthreads();
+        if (m >= 0 && m < NZ) {
+            // red pass on plane m over the tile + halo 1
+            const double *S = so[m % 3], *Sb = so[(m + 2) % 3], *Sf = so[(m + 1) % 3];
+            double *R = sr[m % 3];
+            const AxisGS Z = cz[m - k0 + 2];
+#pragma unroll
+            for (int u = 0; u < GS_RED_PER; u++) {
+                const int s = rs[u];
+                if (s < 0) continue;
+                const int r = rrow[u];
+                const int cc = 2 * rc2[u] + ((j0 + r + m) & 1);
+                const int o = (r + 1) * GS_OX + cc + 1;
+                R[r * GS_HX + cc] = gs_value(H, cx[cc + 1], cy[r + 1], Z, s, rb[u], S[o], Sb[o], S[o - GS_OX],
+                                             S[o - 1], S[o + 1], S[o + GS_OX], Sf[o]);
+            }
+        }
+        __syncthreads();
+        const int k = m - 1;
+        if (k >= k0 && k < k1 && bs >= 0) {
+            // black pass on plane k over the tile, into the black positions of sr
+            double *R = sr[k % 3];
+            const double *Rb = sr[(k + 2) % 3], *Rf = sr[(k + 1) % 3];
+            const int cc = 2 * bc2 + ((1 + bj + k) & 1);
+            const int h = (br + 1) * GS_HX + cc + 1;
+            R[h] = gs_value(H, cx[cc + 2], cy[br + 2], cz[k - k0 + 2], bs, bbv, so[k % 3][(br + 2) * GS_OX + cc + 2],
+                            Rb[h], R[h - GS_HX], R[h - 1], R[h + 1], R[h + GS_HX], Rf[h]);
+        }
+        __syncthreads();
+        if (k >= k0 && k < k1) {
+            const double *R = sr[k % 3];
+            for (int e = t; e < GS_TY * GS_TX; e += GS_THREADS) {
+                const int r = e / GS_TX, cc = e - r * GS_TX;
+                const int i = i0 + cc, j = j0 + r;
+                if (i < NX && j < NY) st_stream(&xn[k * plane + j * px + i], R[(r + 1) * GS_HX + cc + 1]);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < GS_RED_PER; u++) { rs[u] = rs_n[u]; rb[u] = rb_n[u]; }
+        bs = bs_n;
+        bbv = bb_n;
+    }
 }
 
 // IMMERSED rows, Jacobi over the IB table: new value of row a into tmp[a]
@@ -383,13 +578,51 @@ static int helm_residual(tb_ctx *c, const HelmOp &H, const double *x, const doub
     return 0;
 }
 
+// n sweep pairs, each followed by the Jacobi update of the IMMERSED rows,
+// alternating between the buffers cur and alt (the same size and pitch);
+// cur is swapped to the buffer that holds the result
+static int gs_pairs(tb_ctx *c, const HelmOp &H, double *&cur, double *&alt, const double *b, int n) {
+    const int q = H.q;
+    double *tmp = c->kv[4];
+    const GsTiles T = make_gs_tiles(c->d);
+    if (c->d.ntot >= (1L << 31)) {
+        c->err = "Helmholtz sweeps: field exceeds 2^31 elements";
+        return -1;
+    }
+    for (int sw = 0; sw < n; sw++) {
+        TB_PROF(c, TB_K_HELM_GS,
+                hipLaunchKernelGGL(k_helm_gs_pair, dim3(T.per_xcd * NXCD), dim3(GS_THREADS), 0, c->s, H,
+                                   (const double *)cur, alt, b, T));
+        double *w = cur;
+        cur = alt;
+        alt = w;
+        if (c->nib[q] > 0) {
+            const dim3 gi((c->nib[q] + 255) / 256);
+            hipLaunchKernelGGL(k_ib_gs, gi, dim3(256), 0, c->s, c->ib_row[q], c->ib_cnt[q], c->ib_col[q],
+                               c->ib_val[q], c->nib[q], (const double *)cur, b, tmp);
+            hipLaunchKernelGGL(k_ib_scatter, gi, dim3(256), 0, c->s, c->ib_row[q], c->nib[q], (const double *)tmp,
+                               cur);
+        }
+        TB_LAUNCH_CHECK(c);
+    }
+    return 0;
+}
+
+// copy the box points of src into dst (row padding untouched)
+static int copy_box(tb_ctx *c, double *dst, const double *src) {
+    const size_t pitch = sizeof(double) * (size_t)c->d.px;
+    TB_CHECK(c, hipMemcpy2DAsync(dst, pitch, src, pitch, sizeof(double) * (size_t)c->d.NX,
+                                 (size_t)c->d.NY * c->d.NZ, hipMemcpyDeviceToDevice, c->s));
+    return 0;
+}
+
 // returns 0 and sets ok when converged; ok = false when the sweeps stall
-// (contraction per sweep worse than 0.5) or run out of sweeps
+// (contraction per sweep worse than 0.5) or run out of sweeps; x holds the
+// last iterate either way
 static int helm_gs_solve(tb_ctx *c, const HelmOp &H, double *x, const double *b, double rtol, int maxit, int &its,
                          double &res0, double &res, double &bn, bool &ok) {
-    const int q = H.q;
-    double *ibuf = c->kv[5], *tmp = c->kv[4];
-    const Tiles half = make_tiles((c->d.NX + 1) / 2, c->d.NY, c->d.NZ);
+    double *ibuf = c->kv[5];
+    double *cur = x, *alt = c->kv[6];
     double rr, bb;
     if (helm_residual(c, H, x, b, ibuf, rr, bb)) return -1;
     bn = sqrt(bb);
@@ -400,22 +633,10 @@ static int helm_gs_solve(tb_ctx *c, const HelmOp &H, double *x, const double *b,
     int chunk = 2;
     while (!ok && its < maxit) {
         const int m = chunk < maxit - its ? chunk : maxit - its;
-        for (int sw = 0; sw < m; sw++) {
-            TB_PROF(c, TB_K_HELM_GS,
-                    hipLaunchKernelGGL(k_helm_gs, tiles_grid(half), tiles_block(), 0, c->s, H, x, b, 0, half);
-                    hipLaunchKernelGGL(k_helm_gs, tiles_grid(half), tiles_block(), 0, c->s, H, x, b, 1, half));
-            if (c->nib[q] > 0) {
-                const dim3 gi((c->nib[q] + 255) / 256);
-                hipLaunchKernelGGL(k_ib_gs, gi, dim3(256), 0, c->s, c->ib_row[q], c->ib_cnt[q], c->ib_col[q],
-                                   c->ib_val[q], c->nib[q], (const double *)x, b, tmp);
-                hipLaunchKernelGGL(k_ib_scatter, gi, dim3(256), 0, c->s, c->ib_row[q], c->nib[q],
-                                   (const double *)tmp, x);
-            }
-            TB_LAUNCH_CHECK(c);
-        }
+        if (gs_pairs(c, H, cur, alt, b, m)) return -1;
         its += m;
         const double prev = res;
-        if (helm_residual(c, H, x, b, ibuf, rr, bb)) return -1;
+        if (helm_residual(c, H, cur, b, ibuf, rr, bb)) return -1;
         res = sqrt(rr) / bn;
         if (!(res == res)) break;
         ok = res <= rtol;
@@ -427,6 +648,24 @@ static int helm_gs_solve(tb_ctx *c, const HelmOp &H, double *x, const double *b,
         const int need = (int)ceil(log(rtol / res) / log(rate));
         chunk = need < 1 ? 1 : (need > 3 ? 3 : need);
     }
+    if (cur != x && copy_box(c, x, cur)) return -1;
+    return 0;
+}
+
+// The smoother of tb_helmholtz_solve by itself: f_x <- `pairs` red-black
+// sweep pairs (each followed by the Jacobi update of the IMMERSED rows) of
+// A_q(dt) x = f_b, in place.
+extern "C" int tb_helmholtz_gs_sweeps(tb_ctx *c, int q, double dt, int f_b, int f_x, int pairs) {
+    if (!c || q < 0 || q > 3 || f_b < 0 || f_b >= TB_F_COUNT || f_x < 0 || f_x >= TB_F_COUNT || f_b == f_x ||
+        pairs < 0) {
+        if (c) c->err = "tb_helmholtz_gs_sweeps: bad argument";
+        return -1;
+    }
+    HelmOp H = make_op(c, q, dt);
+    double *cur = c->f[f_x], *alt = c->kv[6];
+    if (gs_pairs(c, H, cur, alt, c->f[f_b], pairs)) return -1;
+    if (cur != c->f[f_x] && copy_box(c, c->f[f_x], cur)) return -1;
+    TB_CHECK(c, hipStreamSynchronize(c->s));
     return 0;
 }
 
