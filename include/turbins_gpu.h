@@ -138,7 +138,8 @@ typedef struct tb_solve_stats {
     int32_t converged;          /* 1 if the target was met */
     int32_t fallback;           /* 1 if the plain FP64 path had to take over */
     int32_t vcycles;            /* V-cycles applied (Poisson) */
-    double res0;                /* initial true relative residual */
+    double res0;                /* initial true relative residual (NaN: not evaluated,
+                                   the Helmholtz Gauss-Seidel path) */
     double res;                 /* final true relative residual ||b-Ax||/||b|| */
     double ms;                  /* device time, milliseconds */
 } tb_solve_stats;

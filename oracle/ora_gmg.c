@@ -11,6 +11,7 @@
 #include "ora_internal.h"
 
 #define COARSE_MAX 1024
+#define COARSEST_CELLS 64   /* coarsening stops at this many cells */
 
 static size_t lev_n(const ora_level *L) { return (size_t)L->nx * L->ny * L->nz; }
 
@@ -51,7 +52,7 @@ int ora_gmg_build(ora_ctx *o) {
     int nl = 1;
     while (nl < 16) {
         L = &o->lev[nl - 1];
-        if (lev_n(L) <= COARSE_MAX / 2) break;
+        if (lev_n(L) <= COARSEST_CELLS) break;
         int cx, cy, cz;
         if (!choose(L, &cx, &cy, &cz)) break;
         L->cx = cx; L->cy = cy; L->cz = cz;

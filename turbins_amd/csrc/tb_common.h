@@ -170,6 +170,8 @@ struct Level {
     int cx, cy, cz;       // halved towards the next level
     double *x64, *b64, *r64;
     float *x32, *b32, *r32;
+    double *a64;          // second iterate buffer of levels smoothed by fused pairs, else null
+    float *a32;
 };
 
 // ---------------------------------------------------------------------------
@@ -220,9 +222,12 @@ struct tb_ctx {
     bool use_graphs = true;
     hipGraphExec_t vc_exec[2] = {nullptr, nullptr};
     int helm_method = TB_HELM_GS;    // Helmholtz solver
+    int helm_sweeps[4] = {0, 0, 0, 0};      // sweep pairs the last GS solve of u, v, w, c took
+    double helm_rate[4] = {0.0, 0.0, 0.0, 0.0};   // and its residual contraction per pair
     double ir_floor = 1e-5;      // smallest residual reduction asked of one inner FP32 solve
     int ir_maxit = 40;           // inner iterations per refinement pass
     int coarse_n = 0;
+    int small_l0[2] = {0, 0};   // first level of the one-workgroup cycle: FP32, FP64
     bool mg_built = false;
 
     // Poisson history for the learned guess

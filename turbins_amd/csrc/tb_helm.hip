@@ -617,36 +617,50 @@ static int copy_box(tb_ctx *c, double *dst, const double *src) {
 }
 
 // returns 0 and sets ok when converged; ok = false when the sweeps stall
-// (contraction per sweep worse than 0.5) or run out of sweeps; x holds the
-// last iterate either way
+// (contraction per pair worse than 0.5) or run out of sweeps; x holds the
+// last iterate either way.  The true residual costs about two thirds of a
+// sweep pair, so it is evaluated only where the solve may have converged:
+// first after as many pairs as the last solve of the same quantity needed
+// (the count is steady from step to step), then after the pairs predicted
+// from the observed contraction (at most 3 at a time: the contraction
+// improves as the sweeps go, a longer prediction overshoots).  The initial
+// residual is not evaluated (res0 = NaN).
 static int helm_gs_solve(tb_ctx *c, const HelmOp &H, double *x, const double *b, double rtol, int maxit, int &its,
                          double &res0, double &res, double &bn, bool &ok) {
+    const int q = H.q;
     double *ibuf = c->kv[5];
     double *cur = x, *alt = c->kv[6];
     double rr, bb;
-    if (helm_residual(c, H, x, b, ibuf, rr, bb)) return -1;
-    bn = sqrt(bb);
-    if (bn == 0.0) bn = 1.0;
-    res0 = res = sqrt(rr) / bn;
+    res0 = NAN;
+    res = NAN;
     its = 0;
-    ok = res <= rtol;
-    int chunk = 2;
+    ok = false;
+    int chunk = c->helm_sweeps[q] > 0 ? c->helm_sweeps[q] : 2;
+    double prev = NAN, rate = c->helm_rate[q];
     while (!ok && its < maxit) {
         const int m = chunk < maxit - its ? chunk : maxit - its;
         if (gs_pairs(c, H, cur, alt, b, m)) return -1;
         its += m;
-        const double prev = res;
         if (helm_residual(c, H, cur, b, ibuf, rr, bb)) return -1;
+        bn = sqrt(bb);
+        if (bn == 0.0) bn = 1.0;
         res = sqrt(rr) / bn;
         if (!(res == res)) break;
         ok = res <= rtol;
-        const double rate = pow(res / prev, 1.0 / m);
+        if (ok) break;
+        if (prev == prev) rate = pow(res / prev, 1.0 / m);
+        prev = res;
+        if (!(rate > 0.0)) rate = 0.05;   // no estimate yet
         if (!(rate < 0.5)) break;
-        // sweeps still needed at the last observed contraction, at most 3
-        // before the next check (the contraction improves as the sweeps go:
-        // a longer prediction overshoots the tolerance by orders of magnitude)
         const int need = (int)ceil(log(rtol / res) / log(rate));
         chunk = need < 1 ? 1 : (need > 3 ? 3 : need);
+    }
+    if (ok) {
+        // one pair fewer next time when this residual shows it would have
+        // sufficed at the last contraction
+        const double r1 = (rate > 0.0 && rate < 0.5) ? rate : 0.05;
+        c->helm_sweeps[q] = (its > 1 && res / r1 <= rtol) ? its - 1 : its;
+        if (prev == prev) c->helm_rate[q] = rate;
     }
     if (cur != x && copy_box(c, x, cur)) return -1;
     return 0;

@@ -167,8 +167,10 @@ extern "C" int tb_poisson_apply(tb_ctx *c, int f_in, int f_out) {
 // Geometric multigrid on cell-centred levels.  Level 0 is the interior of
 // the padded field layout; coarser levels are compact.  Coarsening halves a
 // direction when its spacing is at most half the largest one (semi-coarsening
-// the strongly coupled directions first), stops at <= 512 cells, and the
-// coarsest level is solved with a dense pseudo-inverse (A + J/n)^-1.
+// the strongly coupled directions first), stops at <= COARSEST_CELLS cells,
+// and the coarsest level is solved with a dense pseudo-inverse (A + J/n)^-1.
+
+constexpr long COARSEST_CELLS = 64;
 
 template <typename T>
 struct LV {
@@ -195,6 +197,21 @@ __global__ void __launch_bounds__(256) k_zero(LV<T> L, T *__restrict__ x, Tiles 
 
 // red-black Gauss-Seidel update of cell (i, j, k); zero_init: the other
 // colour still holds the zero initial guess, so no neighbour is read
+// the Gauss-Seidel value of cell (i, j, k) from b and its six neighbour
+// values (values of absent neighbours are not used)
+template <typename T>
+__device__ inline T rb_value(const LV<T> &L, int i, int j, int k, T bv, T xw, T xe, T xs, T xn, T xb, T xf) {
+    const bool hw = i > 0, he = i < L.nx - 1, hs = j > 0, hn = j < L.ny - 1, hb = k > 0, hf = k < L.nz - 1;
+    T s = bv, dg = (T)0;
+    s = hw ? s + L.cx * xw : s; dg = hw ? dg + L.cx : dg;
+    s = he ? s + L.cx * xe : s; dg = he ? dg + L.cx : dg;
+    s = hs ? s + L.cy * xs : s; dg = hs ? dg + L.cy : dg;
+    s = hn ? s + L.cy * xn : s; dg = hn ? dg + L.cy : dg;
+    s = hb ? s + L.cz * xb : s; dg = hb ? dg + L.cz : dg;
+    s = hf ? s + L.cz * xf : s; dg = hf ? dg + L.cz : dg;
+    return s / dg;
+}
+
 template <typename T>
 __device__ inline void rbgs_cell(const LV<T> &L, T *__restrict__ x, const T *__restrict__ b, int i, int j, int k,
                                  bool zero_init) {
@@ -208,14 +225,7 @@ __device__ inline void rbgs_cell(const LV<T> &L, T *__restrict__ x, const T *__r
         xs = x[id - (hs ? L.px : 0)]; xn = x[id + (hn ? L.px : 0)];
         xb = x[id - (hb ? pl : 0)]; xf = x[id + (hf ? pl : 0)];
     }
-    T s = b[id], dg = (T)0;
-    s = hw ? s + L.cx * xw : s; dg = hw ? dg + L.cx : dg;
-    s = he ? s + L.cx * xe : s; dg = he ? dg + L.cx : dg;
-    s = hs ? s + L.cy * xs : s; dg = hs ? dg + L.cy : dg;
-    s = hn ? s + L.cy * xn : s; dg = hn ? dg + L.cy : dg;
-    s = hb ? s + L.cz * xb : s; dg = hb ? dg + L.cz : dg;
-    s = hf ? s + L.cz * xf : s; dg = hf ? dg + L.cz : dg;
-    x[id] = s / dg;
+    x[id] = rb_value(L, i, j, k, b[id], xw, xe, xs, xn, xb, xf);
 }
 
 template <typename T>
@@ -265,10 +275,11 @@ __device__ inline void restrict_cell(const LV<T> &F, const LV<T> &C, int cx, int
     bc[lidx(C, I, J, K)] = s;
 }
 
-// prolongation: (3/4, 1/4) per halved axis, reflecting; x_f += P x_c
+// prolongation: (3/4, 1/4) per halved axis, reflecting; x_f = x_in + P x_c
+// (x_in may be x_f itself)
 template <typename T>
 __device__ inline void prolong_cell(const LV<T> &F, const LV<T> &C, int cx, int cy, int cz, const T *__restrict__ xc,
-                                    T *__restrict__ xf, int i, int j, int k) {
+                                    const T *xin, T *xf, int i, int j, int k) {
     int K0, K1, J0, J1, I0, I1;
     T wk1, wj1, wi1;
     if (cz) { K0 = k >> 1; K1 = clampi((k & 1) ? K0 + 1 : K0 - 1, 0, C.nz - 1); wk1 = (T)0.25; } else { K0 = K1 = k; wk1 = (T)0; }
@@ -279,7 +290,8 @@ __device__ inline void prolong_cell(const LV<T> &F, const LV<T> &C, int cx, int 
                  wj1 * (wi0 * xc[lidx(C, I0, J1, K0)] + wi1 * xc[lidx(C, I1, J1, K0)])) +
           wk1 * (wj0 * (wi0 * xc[lidx(C, I0, J0, K1)] + wi1 * xc[lidx(C, I1, J0, K1)]) +
                  wj1 * (wi0 * xc[lidx(C, I0, J1, K1)] + wi1 * xc[lidx(C, I1, J1, K1)]));
-    xf[lidx(F, i, j, k)] += s;
+    const long id = lidx(F, i, j, k);
+    xf[id] = xin[id] + s;
 }
 
 // coarsest level: x = Minv (b - mean b), then x -= mean x (whole block;
@@ -340,10 +352,175 @@ __global__ void __launch_bounds__(256) k_restrict(LV<T> F, LV<T> C, int cx, int 
 
 template <typename T>
 __global__ void __launch_bounds__(256) k_prolong(LV<T> F, LV<T> C, int cx, int cy, int cz, const T *__restrict__ xc,
-                                                 T *__restrict__ xf, Tiles t) {
+                                                 const T *xin, T *xf, Tiles t) {
     int i, j, k;
     if (!tile_point(t, i, j, k)) return;
-    prolong_cell(F, C, cx, cy, cz, xc, xf, i, j, k);
+    prolong_cell(F, C, cx, cy, cz, xc, xin, xf, i, j, k);
+}
+
+// One red-black sweep pair on a large level in a single pass, xn =
+// second(first(xo)), first colour = (i + j + k) % 2 == F (the pre-smoother
+// sweeps colour 0 first, the post-smoother colour 1).  The first colour reads
+// only second-colour neighbours of xo and the second only first-colour
+// values of this pair, so the result equals two half sweeps bit for bit
+// (same expression, rb_value) while xo and b cross HBM once.  ZERO: xo is
+// the zero initial guess and is not read.  The block layout is the one of
+// k_helm_gs_pair: a 64 x 8 column tile walking PAIR_TK planes through LDS
+// rings of xo (halo 2) and first-colour values (halo 1), the next plane's
+// xo and b fetched into registers one plane ahead.
+constexpr int PAIR_TX = 64, PAIR_TY = 8, PAIR_TK = 16;
+constexpr int PAIR_OX = PAIR_TX + 4, PAIR_OY = PAIR_TY + 4;
+constexpr int PAIR_HX = PAIR_TX + 2, PAIR_HY = PAIR_TY + 2;
+constexpr int PAIR_THREADS = 256;
+constexpr int PAIR_OLD_PER = (PAIR_OY * PAIR_OX + PAIR_THREADS - 1) / PAIR_THREADS;
+constexpr int PAIR_RED = PAIR_HY * (PAIR_HX / 2);
+constexpr int PAIR_RED_PER = (PAIR_RED + PAIR_THREADS - 1) / PAIR_THREADS;
+static_assert(PAIR_TX * PAIR_TY == 2 * PAIR_THREADS, "one second-colour cell per thread");
+// FP64 levels at least this large use the fused pair (smaller ones have too
+// few tiles to hide the plane-by-plane chain of a block)
+constexpr long PAIR_LEVEL_CELLS = 8L << 20;
+
+struct PairTiles {
+    int tx, ty, kc;
+    int n, per_xcd;
+};
+
+static PairTiles make_pair_tiles(int nx, int ny, int nz) {
+    PairTiles t;
+    t.tx = (nx + PAIR_TX - 1) / PAIR_TX;
+    t.ty = (ny + PAIR_TY - 1) / PAIR_TY;
+    t.kc = (nz + PAIR_TK - 1) / PAIR_TK;
+    t.n = t.tx * t.ty * t.kc;
+    t.per_xcd = (t.n + NXCD - 1) / NXCD;
+    return t;
+}
+
+template <typename T, int F, bool ZERO>
+__global__ void __launch_bounds__(PAIR_THREADS) k_rbgs_pair(LV<T> L, const T *__restrict__ xo, T *__restrict__ xn,
+                                                            const T *__restrict__ b, PairTiles P) {
+    __shared__ T so[3][PAIR_OY * PAIR_OX];
+    __shared__ T sr[3][PAIR_HY * PAIR_HX];
+    const int nx = L.nx, ny = L.ny, nz = L.nz, px = L.px;
+    const int plane = L.px * L.py;
+    const int bid = blockIdx.x;
+    const int lt = (bid % NXCD) * P.per_xcd + bid / NXCD;
+    if (lt >= P.n) return;                                  // whole block
+    const int tx = lt % P.tx, rest = lt / P.tx;
+    const int i0 = tx * PAIR_TX, j0 = (rest % P.ty) * PAIR_TY;
+    const int k0 = (rest / P.ty) * PAIR_TK;
+    const int k1 = k0 + PAIR_TK < nz ? k0 + PAIR_TK : nz;
+    const int t = threadIdx.x;
+
+    for (int e = t; e < 3 * PAIR_OY * PAIR_OX; e += PAIR_THREADS) (&so[0][0])[e] = (T)0;
+    for (int e = t; e < 3 * PAIR_HY * PAIR_HX; e += PAIR_THREADS) (&sr[0][0])[e] = (T)0;
+    __syncthreads();
+
+    int ooff[PAIR_OLD_PER];
+#pragma unroll
+    for (int u = 0; u < PAIR_OLD_PER; u++) {
+        const int e = t + u * PAIR_THREADS;
+        const int r = e / PAIR_OX, cc = e - r * PAIR_OX;
+        const int i = i0 - 2 + cc, j = j0 - 2 + r;
+        ooff[u] = (e < PAIR_OY * PAIR_OX && i >= 0 && i < nx && j >= 0 && j < ny) ? j * px + i : -1;
+    }
+    T ro[PAIR_OLD_PER];
+    auto fetch_old = [&](int m) {
+#pragma unroll
+        for (int u = 0; u < PAIR_OLD_PER; u++) ro[u] = ooff[u] >= 0 ? xo[m * plane + ooff[u]] : (T)0;
+    };
+    auto put_old = [&](int m) {
+        T *S = so[m % 3];
+#pragma unroll
+        for (int u = 0; u < PAIR_OLD_PER; u++) {
+            const int e = t + u * PAIR_THREADS;
+            if (e < PAIR_OY * PAIR_OX) S[e] = ro[u];
+        }
+    };
+    int rrow[PAIR_RED_PER], rc2[PAIR_RED_PER];
+#pragma unroll
+    for (int u = 0; u < PAIR_RED_PER; u++) {
+        const int e = t + u * PAIR_THREADS;
+        rrow[u] = e < PAIR_RED ? e / (PAIR_HX / 2) : 0;
+        rc2[u] = e < PAIR_RED ? e - rrow[u] * (PAIR_HX / 2) : -1;
+    }
+    // b of this thread's first-colour cells of plane m (valid: inside the box)
+    T rb[PAIR_RED_PER];
+    bool rv[PAIR_RED_PER];
+    auto fetch_red = [&](int m, T *b_, bool *v_) {
+#pragma unroll
+        for (int u = 0; u < PAIR_RED_PER; u++) {
+            const int j = j0 - 1 + rrow[u];
+            const int i = i0 - 1 + 2 * rc2[u] + ((F + 1 + j + m) & 1);
+            v_[u] = rc2[u] >= 0 && i >= 0 && i < nx && j >= 0 && j < ny;
+            b_[u] = v_[u] ? b[m * plane + j * px + i] : (T)0;
+        }
+    };
+#pragma unroll
+    for (int u = 0; u < PAIR_RED_PER; u++) { rb[u] = (T)0; rv[u] = false; }
+    const int br = t / (PAIR_TX / 2), bc2 = t % (PAIR_TX / 2);
+    const int bj = j0 + br;
+    T bb = (T)0;
+    auto fetch_black = [&](int k) -> T {
+        const int i = i0 + 2 * bc2 + ((1 + F + bj + k) & 1);
+        return (i < nx && bj < ny) ? b[k * plane + bj * px + i] : (T)0;
+    };
+
+    if (!ZERO) {
+        if (k0 >= 2) { fetch_old(k0 - 2); put_old(k0 - 2); }
+        if (k0 >= 1) { fetch_old(k0 - 1); put_old(k0 - 1); }
+        fetch_old(k0);
+    }
+    if (k0 >= 1) fetch_red(k0 - 1, rb, rv);
+    for (int m = k0 - 1; m <= k1; m++) {
+        if (!ZERO) {
+            if (m + 1 < nz) put_old(m + 1);
+            if (m + 2 < nz) fetch_old(m + 2);
+        }
+        T rb_n[PAIR_RED_PER];
+        bool rv_n[PAIR_RED_PER];
+#pragma unroll
+        for (int u = 0; u < PAIR_RED_PER; u++) { rb_n[u] = (T)0; rv_n[u] = false; }
+        if (m + 1 < nz) fetch_red(m + 1, rb_n, rv_n);
+        const T bb_n = (m >= k0 && m < k1) ? fetch_black(m) : (T)0;
+        __syncthreads();
+        if (m >= 0 && m < nz) {
+            const T *S = so[m % 3], *Sb = so[(m + 2) % 3], *Sf = so[(m + 1) % 3];
+            T *R = sr[m % 3];
+#pragma unroll
+            for (int u = 0; u < PAIR_RED_PER; u++) {
+                if (!rv[u]) continue;
+                const int r = rrow[u];
+                const int j = j0 - 1 + r;
+                const int cc = 2 * rc2[u] + ((F + 1 + j + m) & 1);
+                const int i = i0 - 1 + cc;
+                const int o = (r + 1) * PAIR_OX + cc + 1;
+                R[r * PAIR_HX + cc] = rb_value(L, i, j, m, rb[u], S[o - 1], S[o + 1], S[o - PAIR_OX], S[o + PAIR_OX],
+                                               Sb[o], Sf[o]);
+            }
+        }
+        __syncthreads();
+        const int k = m - 1;
+        const int cc = 2 * bc2 + ((1 + F + bj + k) & 1);
+        const int i = i0 + cc;
+        if (k >= k0 && k < k1 && i < nx && bj < ny) {
+            T *R = sr[k % 3];
+            const T *Rb = sr[(k + 2) % 3], *Rf = sr[(k + 1) % 3];
+            const int h = (br + 1) * PAIR_HX + cc + 1;
+            R[h] = rb_value(L, i, bj, k, bb, R[h - 1], R[h + 1], R[h - PAIR_HX], R[h + PAIR_HX], Rb[h], Rf[h]);
+        }
+        __syncthreads();
+        if (k >= k0 && k < k1) {
+            const T *R = sr[k % 3];
+            for (int e = t; e < PAIR_TY * PAIR_TX; e += PAIR_THREADS) {
+                const int r = e / PAIR_TX, c2 = e - r * PAIR_TX;
+                const int ii = i0 + c2, jj = j0 + r;
+                if (ii < nx && jj < ny) xn[k * plane + jj * px + ii] = R[(r + 1) * PAIR_HX + c2 + 1];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < PAIR_RED_PER; u++) { rb[u] = rb_n[u]; rv[u] = rv_n[u]; }
+        bb = bb_n;
+    }
 }
 
 template <typename T>
@@ -354,87 +531,130 @@ __global__ void k_coarse(const T *__restrict__ Minv, int n, const T *__restrict_
     coarse_solve(Minv, n, b, x, sh, bb, &mean_s);
 }
 
-// The small levels of a V-cycle (at most SMALL_LEVEL_CELLS cells each) run
-// in one workgroup of 1024 threads, one barrier between phases: their
-// launches (~5 us each, 12 per level) would otherwise cost more than their
-// work.  Larger levels are latency-bound in a single workgroup (a
-// 32768-cell level made the fused kernel slower than the launches).
-constexpr long SMALL_LEVEL_CELLS = 4096;
+// The small levels of a V-cycle run in one workgroup of 1024 threads with
+// every level resident in LDS (compact layouts), one barrier between phases:
+// their launches (~3-5 us each, 11 per level) and L2 round trips would
+// otherwise cost far more than their work.  Level l0 is read from (b) and
+// written to (x) global memory once; one residual buffer serves all levels.
+// The first level run this way is the largest one that has at most
+// SMALL_LEVEL_CELLS cells and whose levels fit SMALL_LDS_BYTES.
+constexpr long SMALL_LEVEL_CELLS = 8192;
+constexpr int SMALL_LDS_BYTES = 156 * 1024;
 constexpr int MAXLEV = 16;
 
 template <typename T>
 struct SmallCycle {
     int l0, nl, pre, post, coarse_n;
-    LV<T> lv[MAXLEV];
-    T *x[MAXLEV], *b[MAXLEV], *r[MAXLEV];
+    LV<T> lv[MAXLEV];          // compact layouts of the LDS copies
+    LV<T> g0;                  // layout of level l0 in global memory
+    T *gx0;                    // level l0 in global memory: x out
+    const T *gb0;              //                            b in
+    int xo[MAXLEV], bo[MAXLEV], ro;   // LDS offsets in elements of T
     int cx[MAXLEV], cy[MAXLEV], cz[MAXLEV];
     const T *Minv;
 };
 
+// LDS bytes of a small cycle starting at level l0: coarse-solve scratch
+// (1024 + 32 doubles), then x and b of every level and one residual buffer
+template <typename T>
+static int small_lds_bytes(const tb_ctx *c, int l0) {
+    long e = 0;
+    for (int m = l0; m < c->nlev; m++) e += 2L * c->lev[m].nx * c->lev[m].ny * c->lev[m].nz;
+    e += (long)c->lev[l0].nx * c->lev[l0].ny * c->lev[l0].nz;
+    return (int)(sizeof(double) * (1024 + 32) + sizeof(T) * e);
+}
+
+// Calls f(i, j, k) for this thread's cells of an nx x ny x nz box, cell q =
+// threadIdx.x + s * blockDim.x in x-fastest order; the position advances by
+// carries instead of a division per cell.
+template <typename F>
+__device__ inline void block_cells(int nx, int ny, int nz, F f) {
+    const int n = nx * ny * nz;
+    int q = threadIdx.x;
+    int i = q % nx, r = q / nx;
+    int j = r % ny, k = r / ny;
+    const int di = blockDim.x % nx, dr = blockDim.x / nx;
+    const int dj = dr % ny, dk = dr / ny;
+    for (; q < n; q += blockDim.x) {
+        f(i, j, k);
+        i += di;
+        j += dj;
+        k += dk;
+        if (i >= nx) { i -= nx; j++; }
+        if (j >= ny) { j -= ny; k++; }
+    }
+}
+
 template <typename T>
 __device__ inline void block_sweep(const LV<T> &L, T *x, const T *b, int color, bool zero_init) {
-    const int hx = (L.nx + 1) / 2;
-    const int n = hx * L.ny * L.nz;
-    for (int q = threadIdx.x; q < n; q += blockDim.x) {
-        const int ih = q % hx;
-        const int rr = q / hx;
-        const int j = rr % L.ny, k = rr / L.ny;
+    block_cells((L.nx + 1) / 2, L.ny, L.nz, [&](int ih, int j, int k) {
         const int i = 2 * ih + ((j + k + color) & 1);
         if (i < L.nx) rbgs_cell(L, x, b, i, j, k, zero_init);
-    }
+    });
     __syncthreads();
 }
 
 template <typename T>
 __global__ void __launch_bounds__(1024) k_small_cycle(SmallCycle<T> A) {
-    __shared__ double sh[16];
-    __shared__ double bb[1024];
-    __shared__ double mean_s;
+    extern __shared__ double smem[];
+    double *bb = smem, *sh = smem + 1024, *mean_s = smem + 1024 + 16;
+    T *S = reinterpret_cast<T *>(smem + 1024 + 32);
     const int last = A.nl - 1;
-    for (int l = A.l0; l < last; l++) {
-        const LV<T> &L = A.lv[l];
-        const int n = L.nx * L.ny * L.nz;
-        if (A.pre > 0) {
-            block_sweep(L, A.x[l], A.b[l], 0, true);
-            block_sweep(L, A.x[l], A.b[l], 1, false);
-            for (int s = 1; s < A.pre; s++) {
-                block_sweep(L, A.x[l], A.b[l], 0, false);
-                block_sweep(L, A.x[l], A.b[l], 1, false);
-            }
-        } else {
-            for (int q = threadIdx.x; q < n; q += blockDim.x) {
-                const int i = q % L.nx, rr = q / L.nx;
-                A.x[l][lidx(L, i, rr % L.ny, rr / L.ny)] = (T)0;
-            }
-            __syncthreads();
-        }
-        for (int q = threadIdx.x; q < n; q += blockDim.x) {
-            const int i = q % L.nx, rr = q / L.nx;
-            resid_cell(L, A.x[l], A.b[l], A.r[l], i, rr % L.ny, rr / L.ny);
-        }
-        __syncthreads();
-        const LV<T> &C = A.lv[l + 1];
-        const int nc = C.nx * C.ny * C.nz;
-        for (int q = threadIdx.x; q < nc; q += blockDim.x) {
-            const int I = q % C.nx, rr = q / C.nx;
-            restrict_cell(L, C, A.cx[l], A.cy[l], A.cz[l], A.r[l], A.b[l + 1], I, rr % C.ny, rr / C.ny);
-        }
+    {
+        const LV<T> &L = A.lv[A.l0];
+        T *b0 = S + A.bo[A.l0];
+        block_cells(L.nx, L.ny, L.nz, [&](int i, int j, int k) {
+            b0[(k * L.ny + j) * L.nx + i] = A.gb0[lidx(A.g0, i, j, k)];
+        });
         __syncthreads();
     }
-    coarse_solve(A.Minv, A.coarse_n, A.b[last], A.x[last], sh, bb, &mean_s);
+    T *r = S + A.ro;
+    for (int l = A.l0; l < last; l++) {
+        const LV<T> &L = A.lv[l];
+        T *x = S + A.xo[l];
+        const T *b = S + A.bo[l];
+        const int n = L.nx * L.ny * L.nz;
+        if (A.pre > 0) {
+            block_sweep(L, x, b, 0, true);
+            block_sweep(L, x, b, 1, false);
+            for (int s = 1; s < A.pre; s++) {
+                block_sweep(L, x, b, 0, false);
+                block_sweep(L, x, b, 1, false);
+            }
+        } else {
+            for (int q = threadIdx.x; q < n; q += blockDim.x) x[q] = (T)0;
+            __syncthreads();
+        }
+        block_cells(L.nx, L.ny, L.nz, [&](int i, int j, int k) { resid_cell(L, (const T *)x, b, r, i, j, k); });
+        __syncthreads();
+        const LV<T> &C = A.lv[l + 1];
+        T *bc = S + A.bo[l + 1];
+        block_cells(C.nx, C.ny, C.nz, [&](int I, int J, int K) {
+            restrict_cell(L, C, A.cx[l], A.cy[l], A.cz[l], (const T *)r, bc, I, J, K);
+        });
+        __syncthreads();
+    }
+    coarse_solve(A.Minv, A.coarse_n, (const T *)(S + A.bo[last]), S + A.xo[last], sh, bb, mean_s);
     for (int l = last - 1; l >= A.l0; l--) {
         const LV<T> &L = A.lv[l];
         const LV<T> &C = A.lv[l + 1];
-        const int n = L.nx * L.ny * L.nz;
-        for (int q = threadIdx.x; q < n; q += blockDim.x) {
-            const int i = q % L.nx, rr = q / L.nx;
-            prolong_cell(L, C, A.cx[l], A.cy[l], A.cz[l], A.x[l + 1], A.x[l], i, rr % L.ny, rr / L.ny);
-        }
+        T *x = S + A.xo[l];
+        const T *b = S + A.bo[l];
+        block_cells(L.nx, L.ny, L.nz, [&](int i, int j, int k) {
+            prolong_cell(L, C, A.cx[l], A.cy[l], A.cz[l], (const T *)(S + A.xo[l + 1]), (const T *)x, x, i, j, k);
+        });
         __syncthreads();
         for (int s = 0; s < A.post; s++) {
-            block_sweep(L, A.x[l], A.b[l], 1, false);
-            block_sweep(L, A.x[l], A.b[l], 0, false);
+            block_sweep(L, x, b, 1, false);
+            block_sweep(L, x, b, 0, false);
         }
+    }
+    {
+        const LV<T> &L = A.lv[A.l0];
+        const T *x0 = S + A.xo[A.l0];
+        block_cells(L.nx, L.ny, L.nz, [&](int i, int j, int k) {
+            A.gx0[lidx(A.g0, i, j, k)] = x0[(k * L.ny + j) * L.nx + i];
+        });
     }
 }
 
@@ -489,8 +709,9 @@ void tb_mg_free(tb_ctx *c) {
         Level &L = c->lev[l];
         if (l > 0) { hipFree(L.x64); hipFree(L.b64); hipFree(L.r64); }
         hipFree(L.x32); hipFree(L.b32); hipFree(L.r32);
-        L.x64 = L.b64 = L.r64 = nullptr;
-        L.x32 = L.b32 = L.r32 = nullptr;
+        hipFree(L.a64); hipFree(L.a32);
+        L.x64 = L.b64 = L.r64 = L.a64 = nullptr;
+        L.x32 = L.b32 = L.r32 = L.a32 = nullptr;
     }
     hipFree(c->coarse_inv64); hipFree(c->coarse_inv32);
     c->coarse_inv64 = nullptr; c->coarse_inv32 = nullptr;
@@ -513,7 +734,7 @@ int tb_mg_build(tb_ctx *c) {
     int nl = 1;
     while (nl < 16) {
         Level &L = c->lev[nl - 1];
-        if ((long)L.nx * L.ny * L.nz <= 512) break;
+        if ((long)L.nx * L.ny * L.nz <= COARSEST_CELLS) break;
         int cx, cy, cz;
         if (!choose(L, cx, cy, cz)) break;
         L.cx = cx; L.cy = cy; L.cz = cz;
@@ -539,6 +760,13 @@ int tb_mg_build(tb_ctx *c) {
             TB_CHECK(c, hipMemsetAsync(L.r64, 0, sizeof(double) * L.ntot, c->s));
             c->bytes += 3 * sizeof(double) * L.ntot;
         }
+        // FP64 only: the FP32 arrays of a level stay resident in the 256 MB
+        // Infinity Cache, where two half sweeps cost no more than a pair
+        if ((long)L.nx * L.ny * L.nz >= PAIR_LEVEL_CELLS && L.ntot < (1L << 31)) {
+            TB_CHECK(c, hipMalloc((void **)&L.a64, sizeof(double) * L.ntot));
+            TB_CHECK(c, hipMemsetAsync(L.a64, 0, sizeof(double) * L.ntot, c->s));
+            c->bytes += sizeof(double) * L.ntot;
+        }
         TB_CHECK(c, hipMalloc((void **)&L.x32, sizeof(float) * L.ntot));
         TB_CHECK(c, hipMalloc((void **)&L.b32, sizeof(float) * L.ntot));
         TB_CHECK(c, hipMalloc((void **)&L.r32, sizeof(float) * L.ntot));
@@ -553,6 +781,23 @@ int tb_mg_build(tb_ctx *c) {
         TB_CHECK(c, hipMemsetAsync(c->cg32[a], 0, sizeof(float) * L0.ntot, c->s));
         c->bytes += sizeof(float) * L0.ntot;
     }
+    // first level of the LDS-resident small cycle, per precision
+    for (int a = 0; a < 2; a++) {
+        int l0 = nl - 1;
+        while (l0 > 0) {
+            const Level &P = c->lev[l0 - 1];
+            const int bytes = a ? small_lds_bytes<double>(c, l0 - 1) : small_lds_bytes<float>(c, l0 - 1);
+            if ((long)P.nx * P.ny * P.nz > SMALL_LEVEL_CELLS || bytes > SMALL_LDS_BYTES) break;
+            l0--;
+        }
+        c->small_l0[a] = l0;
+        const int bytes = a ? small_lds_bytes<double>(c, l0) : small_lds_bytes<float>(c, l0);
+        if (bytes > SMALL_LDS_BYTES) { c->err = "multigrid: coarsest levels exceed the LDS budget"; return -1; }
+    }
+    TB_CHECK(c, hipFuncSetAttribute((const void *)k_small_cycle<float>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    SMALL_LDS_BYTES));
+    TB_CHECK(c, hipFuncSetAttribute((const void *)k_small_cycle<double>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    SMALL_LDS_BYTES));
     // dense inverse of (A_c + J/n) on the coarsest level (Gauss-Jordan,
     // partial pivoting), FP64 on the host
     Level &Lc = c->lev[nl - 1];
@@ -603,15 +848,15 @@ int tb_mg_build(tb_ctx *c) {
 
 template <typename T>
 struct MGBuf {
-    T *x, *b, *r;
+    T *x, *b, *r, *a;   // a: second iterate buffer of the fused-pair levels, else null
 };
 
 template <typename T>
 static MGBuf<T> bufs(tb_ctx *c, int l);
 template <>
-MGBuf<double> bufs<double>(tb_ctx *c, int l) { return {c->lev[l].x64, c->lev[l].b64, c->lev[l].r64}; }
+MGBuf<double> bufs<double>(tb_ctx *c, int l) { return {c->lev[l].x64, c->lev[l].b64, c->lev[l].r64, c->lev[l].a64}; }
 template <>
-MGBuf<float> bufs<float>(tb_ctx *c, int l) { return {c->lev[l].x32, c->lev[l].b32, c->lev[l].r32}; }
+MGBuf<float> bufs<float>(tb_ctx *c, int l) { return {c->lev[l].x32, c->lev[l].b32, c->lev[l].r32, c->lev[l].a32}; }
 
 template <typename T>
 static const T *coarse_inv(tb_ctx *c);
@@ -625,7 +870,7 @@ static int vcycle_level(tb_ctx *c, int l) {
     const Level &L = c->lev[l];
     MGBuf<T> B = bufs<T>(c, l);
     LV<T> lv = lv_of<T>(L);
-    if ((long)L.nx * L.ny * L.nz <= SMALL_LEVEL_CELLS || l == c->nlev - 1) {
+    if (l == c->small_l0[sizeof(T) == 8 ? 1 : 0]) {
         SmallCycle<T> A;
         A.l0 = l;
         A.nl = c->nlev;
@@ -633,19 +878,71 @@ static int vcycle_level(tb_ctx *c, int l) {
         A.post = c->cfg.mg_post;
         A.coarse_n = c->coarse_n;
         A.Minv = coarse_inv<T>(c);
+        A.g0 = lv;
+        A.gx0 = B.x;
+        A.gb0 = B.b;
+        int off = 0;
         for (int m = l; m < c->nlev; m++) {
-            MGBuf<T> Bm = bufs<T>(c, m);
-            A.lv[m] = lv_of<T>(c->lev[m]);
-            A.x[m] = Bm.x; A.b[m] = Bm.b; A.r[m] = Bm.r;
-            A.cx[m] = c->lev[m].cx; A.cy[m] = c->lev[m].cy; A.cz[m] = c->lev[m].cz;
+            const Level &Lm = c->lev[m];
+            LV<T> v = lv_of<T>(Lm);
+            v.px = Lm.nx;
+            v.py = Lm.ny;
+            A.lv[m] = v;
+            A.xo[m] = off; off += Lm.nx * Lm.ny * Lm.nz;
+            A.bo[m] = off; off += Lm.nx * Lm.ny * Lm.nz;
+            A.cx[m] = Lm.cx; A.cy[m] = Lm.cy; A.cz[m] = Lm.cz;
         }
-        hipLaunchKernelGGL(k_small_cycle<T>, dim3(1), dim3(1024), 0, c->s, A);
+        A.ro = off;
+        hipLaunchKernelGGL(k_small_cycle<T>, dim3(1), dim3(1024), small_lds_bytes<T>(c, l), c->s, A);
         TB_LAUNCH_CHECK(c);
         return 0;
     }
     Tiles all = make_tiles(L.nx, L.ny, L.nz);
     Tiles half = make_tiles((L.nx + 1) / 2, L.ny, L.nz);
     const int kid = (l == 0) ? (sizeof(T) == 8 ? TB_K_RBGS0 : TB_K_RBGS0_F32) : -1;
+    const int pre = c->cfg.mg_pre, post = c->cfg.mg_post;
+    // large levels: fused sweep pairs alternating between x and the level's
+    // second buffer; cur is the buffer holding the current iterate
+    const bool pairs = B.a != nullptr;
+    T *cur = B.x;
+    if (pairs) {
+        const PairTiles P = make_pair_tiles(L.nx, L.ny, L.nz);
+        const dim3 grid(P.per_xcd * NXCD), blk(PAIR_THREADS);
+        auto pair = [&](int first, bool zero) {
+            T *dst = zero ? B.x : (cur == B.x ? B.a : B.x);
+            if (c->prof_on && kid >= 0) tb_prof_begin(c, kid);
+            if (zero) hipLaunchKernelGGL((k_rbgs_pair<T, 0, true>), grid, blk, 0, c->s, lv, (const T *)cur, dst, (const T *)B.b, P);
+            else if (first == 0) hipLaunchKernelGGL((k_rbgs_pair<T, 0, false>), grid, blk, 0, c->s, lv, (const T *)cur, dst, (const T *)B.b, P);
+            else hipLaunchKernelGGL((k_rbgs_pair<T, 1, false>), grid, blk, 0, c->s, lv, (const T *)cur, dst, (const T *)B.b, P);
+            if (c->prof_on && kid >= 0) tb_prof_end(c, kid);
+            cur = dst;
+        };
+        if (pre > 0) {
+            pair(0, true);
+            for (int s = 1; s < pre; s++) pair(0, false);
+        } else {
+            hipLaunchKernelGGL(k_zero<T>, tiles_grid(all), tiles_block(), 0, c->s, lv, B.x, all);
+        }
+        hipLaunchKernelGGL(k_mg_resid<T>, tiles_grid(all), tiles_block(), 0, c->s, lv, (const T *)cur, (const T *)B.b,
+                           B.r, all);
+        const Level &Cl = c->lev[l + 1];
+        MGBuf<T> CB = bufs<T>(c, l + 1);
+        LV<T> cv = lv_of<T>(Cl);
+        Tiles call = make_tiles(Cl.nx, Cl.ny, Cl.nz);
+        hipLaunchKernelGGL(k_restrict<T>, tiles_grid(call), tiles_block(), 0, c->s, lv, cv, L.cx, L.cy, L.cz,
+                           (const T *)B.r, CB.b, call);
+        TB_LAUNCH_CHECK(c);
+        if (vcycle_level<T>(c, l + 1)) return -1;
+        // the post pairs alternate buffers, so the prolonged iterate goes to
+        // the buffer from which they end in x
+        T *dst = (post % 2 == 0) ? B.x : B.a;
+        hipLaunchKernelGGL(k_prolong<T>, tiles_grid(all), tiles_block(), 0, c->s, lv, cv, L.cx, L.cy, L.cz,
+                           (const T *)CB.x, (const T *)cur, dst, all);
+        cur = dst;
+        for (int s = 0; s < post; s++) pair(1, false);
+        TB_LAUNCH_CHECK(c);
+        return 0;
+    }
     auto smooth = [&](int color, int zero_init) {
         if (c->prof_on && kid >= 0) tb_prof_begin(c, kid);
         hipLaunchKernelGGL(k_rbgs<T>, tiles_grid(half), tiles_block(), 0, c->s, lv, B.x, (const T *)B.b, color,
@@ -654,10 +951,10 @@ static int vcycle_level(tb_ctx *c, int l) {
     };
     // zero initial guess: the first red half sweep reads no neighbour and
     // the black one only red cells, so x needs no clearing pass
-    if (c->cfg.mg_pre > 0) {
+    if (pre > 0) {
         smooth(0, 1);
         smooth(1, 0);
-        for (int s = 1; s < c->cfg.mg_pre; s++) {
+        for (int s = 1; s < pre; s++) {
             smooth(0, 0);
             smooth(1, 0);
         }
@@ -675,8 +972,8 @@ static int vcycle_level(tb_ctx *c, int l) {
     TB_LAUNCH_CHECK(c);
     if (vcycle_level<T>(c, l + 1)) return -1;
     hipLaunchKernelGGL(k_prolong<T>, tiles_grid(all), tiles_block(), 0, c->s, lv, cv, L.cx, L.cy, L.cz,
-                       (const T *)CB.x, B.x, all);
-    for (int s = 0; s < c->cfg.mg_post; s++) {
+                       (const T *)CB.x, (const T *)B.x, B.x, all);
+    for (int s = 0; s < post; s++) {
         smooth(1, 0);
         smooth(0, 0);
     }
