@@ -17,20 +17,22 @@ static size_t lev_n(const ora_level *L) { return (size_t)L->nx * L->ny * L->nz; 
 
 /* Coarsening rule: halve a direction when its spacing is at most half the
  * largest spacing (keeps coupling ratios <= 4 so point smoothing works);
- * if no direction qualifies, halve the finest coarsenable one. */
+ * if no direction qualifies, halve every coarsenable one. */
 static int choose(const ora_level *L, int *cx, int *cy, int *cz) {
     int n[3] = {L->nx, L->ny, L->nz};
     double h[3] = {L->hx, L->hy, L->hz};
     double hmax = h[0];
     if (h[1] > hmax) hmax = h[1];
     if (h[2] > hmax) hmax = h[2];
-    int c[3] = {0, 0, 0}, any = 0, best = -1;
+    int c[3] = {0, 0, 0}, any = 0;
     for (int d = 0; d < 3; d++) {
         int can = (n[d] % 2 == 0) && (n[d] >= 4);
         if (can && 2.0 * h[d] <= hmax * (1.0 + 1e-9)) { c[d] = 1; any = 1; }
-        if (can && (best < 0 || h[d] < h[best])) best = d;
     }
-    if (!any && best >= 0) { c[best] = 1; any = 1; }
+    /* spacings within a factor 2 of each other: full coarsening */
+    if (!any)
+        for (int d = 0; d < 3; d++)
+            if ((n[d] % 2 == 0) && (n[d] >= 4)) { c[d] = 1; any = 1; }
     *cx = c[0]; *cy = c[1]; *cz = c[2];
     return any;
 }

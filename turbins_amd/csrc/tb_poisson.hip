@@ -167,7 +167,8 @@ extern "C" int tb_poisson_apply(tb_ctx *c, int f_in, int f_out) {
 // Geometric multigrid on cell-centred levels.  Level 0 is the interior of
 // the padded field layout; coarser levels are compact.  Coarsening halves a
 // direction when its spacing is at most half the largest one (semi-coarsening
-// the strongly coupled directions first), stops at <= COARSEST_CELLS cells,
+// the strongly coupled directions first; all directions when the spacings
+// are within a factor 2), stops at <= COARSEST_CELLS cells,
 // and the coarsest level is solved with a dense pseudo-inverse (A + J/n)^-1.
 
 constexpr long COARSEST_CELLS = 64;
@@ -348,6 +349,163 @@ __global__ void __launch_bounds__(256) k_restrict(LV<T> F, LV<T> C, int cx, int 
     int I, J, K;
     if (!tile_point(t, I, J, K)) return;
     restrict_cell(F, C, cx, cy, cz, rf, bc, I, J, K);
+}
+
+// Residual and restriction in one pass for the coarsening pattern of the
+// large levels, y and z halved and x kept (cx, cy, cz) = (0, 1, 1): a thread
+// owns coarse column (I, J) = fine column i = I, rows 2J-1 .. 2J+2, and walks
+// its coarse planes upwards.  Each fine plane's residual is formed once, on
+// the four rows, from x planes held in registers (rows 2J-2 .. 2J+3), summed
+// over y with the restriction weights, and the z weights are applied over a
+// window of four such plane sums; the residual array never reaches HBM.  The
+// same weights and wall reflection as restrict_cell, summed y first.
+constexpr int RR_KC = 8;   // coarse planes per block
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_resid_restrict_011(LV<T> F, LV<T> C, const T *__restrict__ x,
+                                                            const T *__restrict__ b, T *__restrict__ bc, KTiles t) {
+    const int bid = blockIdx.x;
+    const int lt = (bid % NXCD) * (int)t.per_xcd + bid / NXCD;
+    if (lt >= t.n) return;
+    const int I = (lt % t.tx) * TILE_X + threadIdx.x;
+    const int rest = lt / t.tx;
+    const int J = (rest % t.ty) * TILE_Y + threadIdx.y;
+    const int K0 = (rest / t.ty) * RR_KC;
+    const int K1 = K0 + RR_KC < C.nz ? K0 + RR_KC : C.nz;
+    // every lane of a wave stays to the end: the x neighbours are shuffled
+    const bool act = I < C.nx && J < C.ny;
+    const int lane = threadIdx.x;   // blockDim.x == 64: one wave per row of the block
+    const int i = act ? I : C.nx - 1;
+    const int Jc = act ? J : C.ny - 1;
+    const int nx = F.nx, ny = F.ny, nz = F.nz;
+    const int pl = F.px * F.py;   // 32-bit offsets: the host checks the level size
+    const T W0 = (T)0.125, W1 = (T)0.375;
+    // register rows r6 = 0..5 hold fine rows 2J-2 .. 2J+3 (clamped)
+    int roff[6];
+#pragma unroll
+    for (int r6 = 0; r6 < 6; r6++) roff[r6] = clampi(2 * Jc - 2 + r6, 0, ny - 1) * F.px + i;
+    // planes kc-1 .. kc+2 of x and kc, kc+1 of b in registers: the loads of
+    // a plane are issued two steps before its residual needs them
+    T xm[6], xc[6], xp[6], xq[6], bc0[4], bc1[4];
+    int kc = 0;   // fine plane held in xc
+    auto load_plane = [&](int kf, T *dst) {
+        if (kf < 0 || kf >= nz) return;
+#pragma unroll
+        for (int r6 = 0; r6 < 6; r6++) dst[r6] = x[kf * pl + roff[r6]];
+    };
+    auto load_b = [&](int kf, T *dst) {
+        if (kf < 0 || kf >= nz) return;
+#pragma unroll
+        for (int tj = 0; tj < 4; tj++) dst[tj] = b[kf * pl + roff[1 + tj]];
+    };
+    auto step = [&]() {
+#pragma unroll
+        for (int r6 = 0; r6 < 6; r6++) { xm[r6] = xc[r6]; xc[r6] = xp[r6]; xp[r6] = xq[r6]; }
+#pragma unroll
+        for (int tj = 0; tj < 4; tj++) bc0[tj] = bc1[tj];
+        kc++;
+        load_plane(kc + 2, xq);
+        load_b(kc + 1, bc1);
+    };
+    // y-restricted residual of fine plane kc (absent planes are not read)
+    auto plane_sum = [&]() -> T {
+        const bool hb = kc > 0, hf = kc < nz - 1;
+        T sum = (T)0;
+#pragma unroll
+        for (int tj = 0; tj < 4; tj++) {
+            const int jf = clampi(2 * Jc - 1 + tj, 0, ny - 1);
+            const int r6 = jf - (2 * Jc - 2);
+            const int id = kc * pl + jf * F.px + i;
+            const bool hw = i > 0, he = i < nx - 1, hs = jf > 0, hn = jf < ny - 1;
+            const T xcc = xc[r6];
+            // x neighbours from the adjacent lanes (a wave is 64 consecutive
+            // i); the wave's edge lanes load theirs
+            T xw = __shfl_up(xcc, 1, 64), xe = __shfl_down(xcc, 1, 64);
+            if (lane == 0) xw = hw ? x[id - 1] : xcc;
+            if (lane == 63) xe = he ? x[id + 1] : xcc;
+            xw = hw ? xw : xcc;
+            xe = he ? xe : xcc;
+            const T xs = hs ? xc[r6 - 1] : xcc, xn = hn ? xc[r6 + 1] : xcc;
+            const T xb = hb ? xm[r6] : xcc, xf = hf ? xp[r6] : xcc;
+            const T sr = F.cx * (xcc - xw) + F.cx * (xcc - xe) + F.cy * (xcc - xs) + F.cy * (xcc - xn) +
+                         F.cz * (xcc - xb) + F.cz * (xcc - xf);
+            sum += ((tj == 0 || tj == 3) ? W0 : W1) * (bc0[tj] - sr);   // bc0[tj]: b of row jf
+        }
+        return sum;
+    };
+    kc = 2 * K0 - 1 < 0 ? 0 : 2 * K0 - 1;
+    load_plane(kc - 1, xm);
+    load_plane(kc, xc);
+    load_plane(kc + 1, xp);
+    load_plane(kc + 2, xq);
+    load_b(kc, bc0);
+    load_b(kc + 1, bc1);
+    T Rm1 = plane_sum();
+    if (kc < 2 * K0) step();
+    T R0 = plane_sum();
+    for (int K = K0; K < K1; K++) {
+        step();
+        const T R1 = plane_sum();
+        T R2 = R1;
+        if (2 * K + 2 < nz) {
+            step();
+            R2 = plane_sum();
+        }
+        if (act) bc[lidx(C, I, J, K)] = W0 * Rm1 + W1 * R0 + W1 * R1 + W0 * R2;
+        Rm1 = R1;
+        R0 = R2;
+    }
+}
+
+// Prolongation for the (0, 1, 1) pattern: a thread owns fine column (i, j)
+// and walks PR_KC fine planes, keeping the coarse values of rows J0, J1 of
+// the coarse planes it straddles in registers (one coarse load per two fine
+// planes instead of four per cell).  Same expression as prolong_cell (its x
+// weights are 1 and 0 here), so the result is bit-identical.
+constexpr int PR_KC = 8;
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_prolong_011(LV<T> F, LV<T> C, const T *__restrict__ xc, const T *xin,
+                                                     T *xf, KTiles t) {
+    int i, j, k0, k1;
+    bool act;
+    {
+        const int bid = blockIdx.x;
+        const int lt = (bid % NXCD) * (int)t.per_xcd + bid / NXCD;
+        if (lt >= t.n) return;
+        i = (lt % t.tx) * TILE_X + threadIdx.x;
+        const int rest = lt / t.tx;
+        j = (rest % t.ty) * TILE_Y + threadIdx.y;
+        k0 = (rest / t.ty) * PR_KC;
+        k1 = k0 + PR_KC < F.nz ? k0 + PR_KC : F.nz;
+        act = i < F.nx && j < F.ny;
+    }
+    if (!act) return;
+    const int J0 = j >> 1, J1 = clampi((j & 1) ? J0 + 1 : J0 - 1, 0, C.ny - 1);
+    const T wj1 = (T)0.25, wj0 = (T)1 - wj1, wk1 = (T)0.25, wk0 = (T)1 - wk1;
+    const int cpl = C.px * C.py, fpl = F.px * F.py;
+    const int c0 = J0 * C.px + i, c1 = J1 * C.px + i;
+    // coarse planes Kc-1, Kc, Kc+1 (clamped) of rows J0 (a) and J1 (b)
+    int Kc = k0 >> 1;
+    T am, a0, ap, bm, b0, bp;
+    {
+        const int Km = Kc > 0 ? Kc - 1 : 0, Kp = Kc + 1 < C.nz ? Kc + 1 : C.nz - 1;
+        am = xc[Km * cpl + c0]; a0 = xc[Kc * cpl + c0]; ap = xc[Kp * cpl + c0];
+        bm = xc[Km * cpl + c1]; b0 = xc[Kc * cpl + c1]; bp = xc[Kp * cpl + c1];
+    }
+    for (int k = k0; k < k1; k++) {
+        if ((k >> 1) != Kc) {
+            Kc++;
+            am = a0; a0 = ap; bm = b0; b0 = bp;
+            const int Kp = Kc + 1 < C.nz ? Kc + 1 : C.nz - 1;
+            ap = xc[Kp * cpl + c0];
+            bp = xc[Kp * cpl + c1];
+        }
+        const T a1 = (k & 1) ? ap : am, b1 = (k & 1) ? bp : bm;
+        const T s = wk0 * (wj0 * a0 + wj1 * b0) + wk1 * (wj0 * a1 + wj1 * b1);
+        const int id = k * fpl + j * F.px + i;
+        xf[id] = xin[id] + s;
+    }
 }
 
 template <typename T>
@@ -675,13 +833,15 @@ static int choose(const Level &L, int &cx, int &cy, int &cz) {
     double hmax = h[0];
     if (h[1] > hmax) hmax = h[1];
     if (h[2] > hmax) hmax = h[2];
-    int cc[3] = {0, 0, 0}, any = 0, best = -1;
+    int cc[3] = {0, 0, 0}, any = 0;
     for (int a = 0; a < 3; a++) {
         int can = (n[a] % 2 == 0) && (n[a] >= 4);
         if (can && 2.0 * h[a] <= hmax * (1.0 + 1e-9)) { cc[a] = 1; any = 1; }
-        if (can && (best < 0 || h[a] < h[best])) best = a;
     }
-    if (!any && best >= 0) { cc[best] = 1; any = 1; }
+    // spacings within a factor 2 of each other: full coarsening
+    if (!any)
+        for (int a = 0; a < 3; a++)
+            if ((n[a] % 2 == 0) && (n[a] >= 4)) { cc[a] = 1; any = 1; }
     cx = cc[0]; cy = cc[1]; cz = cc[2];
     return any;
 }
@@ -865,6 +1025,67 @@ const double *coarse_inv<double>(tb_ctx *c) { return c->coarse_inv64; }
 template <>
 const float *coarse_inv<float>(tb_ctx *c) { return c->coarse_inv32; }
 
+// the column-walking transfer kernels serve large levels coarsened in y and
+// z (on small ones their plane-by-plane chains cost more than two launches)
+constexpr long FUSED_TRANSFER_CELLS = 1L << 20;
+static bool transfer_011(const Level &L) {
+    return L.cx == 0 && L.cy == 1 && L.cz == 1 && L.ntot < (1L << 31) &&
+           (long)L.nx * L.ny * L.nz >= FUSED_TRANSFER_CELLS;
+}
+
+// x_f = x_in + P x_c from level l+1 to level l
+template <typename T>
+static void prolong(tb_ctx *c, int l, const T *xin, T *xf) {
+    const Level &L = c->lev[l];
+    LV<T> lv = lv_of<T>(L);
+    const Level &Cl = c->lev[l + 1];
+    MGBuf<T> CB = bufs<T>(c, l + 1);
+    LV<T> cv = lv_of<T>(Cl);
+    if (transfer_011(L)) {
+        KTiles t;
+        t.nx = L.nx; t.ny = L.ny; t.nz = L.nz;
+        t.tx = (L.nx + TILE_X - 1) / TILE_X;
+        t.ty = (L.ny + TILE_Y - 1) / TILE_Y;
+        t.kc = (L.nz + PR_KC - 1) / PR_KC;
+        t.n = (long)t.tx * t.ty * t.kc;
+        t.per_xcd = (t.n + NXCD - 1) / NXCD;
+        hipLaunchKernelGGL(k_prolong_011<T>, ktiles_grid(t), tiles_block(), 0, c->s, lv, cv, (const T *)CB.x, xin, xf,
+                           t);
+        return;
+    }
+    Tiles all = make_tiles(L.nx, L.ny, L.nz);
+    hipLaunchKernelGGL(k_prolong<T>, tiles_grid(all), tiles_block(), 0, c->s, lv, cv, L.cx, L.cy, L.cz,
+                       (const T *)CB.x, xin, xf, all);
+}
+
+// b of level l+1 from the residual of x on level l (fused for (0, 1, 1))
+template <typename T>
+static void resid_restrict(tb_ctx *c, int l, const T *x) {
+    const Level &L = c->lev[l];
+    MGBuf<T> B = bufs<T>(c, l);
+    LV<T> lv = lv_of<T>(L);
+    const Level &Cl = c->lev[l + 1];
+    MGBuf<T> CB = bufs<T>(c, l + 1);
+    LV<T> cv = lv_of<T>(Cl);
+    if (transfer_011(L)) {
+        KTiles t;
+        t.nx = Cl.nx; t.ny = Cl.ny; t.nz = Cl.nz;
+        t.tx = (Cl.nx + TILE_X - 1) / TILE_X;
+        t.ty = (Cl.ny + TILE_Y - 1) / TILE_Y;
+        t.kc = (Cl.nz + RR_KC - 1) / RR_KC;
+        t.n = (long)t.tx * t.ty * t.kc;
+        t.per_xcd = (t.n + NXCD - 1) / NXCD;
+        hipLaunchKernelGGL(k_resid_restrict_011<T>, ktiles_grid(t), tiles_block(), 0, c->s, lv, cv, x,
+                           (const T *)B.b, CB.b, t);
+        return;
+    }
+    Tiles all = make_tiles(L.nx, L.ny, L.nz);
+    hipLaunchKernelGGL(k_mg_resid<T>, tiles_grid(all), tiles_block(), 0, c->s, lv, x, (const T *)B.b, B.r, all);
+    Tiles call = make_tiles(Cl.nx, Cl.ny, Cl.nz);
+    hipLaunchKernelGGL(k_restrict<T>, tiles_grid(call), tiles_block(), 0, c->s, lv, cv, L.cx, L.cy, L.cz,
+                       (const T *)B.r, CB.b, call);
+}
+
 template <typename T>
 static int vcycle_level(tb_ctx *c, int l) {
     const Level &L = c->lev[l];
@@ -923,21 +1144,16 @@ static int vcycle_level(tb_ctx *c, int l) {
         } else {
             hipLaunchKernelGGL(k_zero<T>, tiles_grid(all), tiles_block(), 0, c->s, lv, B.x, all);
         }
-        hipLaunchKernelGGL(k_mg_resid<T>, tiles_grid(all), tiles_block(), 0, c->s, lv, (const T *)cur, (const T *)B.b,
-                           B.r, all);
+        resid_restrict<T>(c, l, (const T *)cur);
         const Level &Cl = c->lev[l + 1];
         MGBuf<T> CB = bufs<T>(c, l + 1);
         LV<T> cv = lv_of<T>(Cl);
-        Tiles call = make_tiles(Cl.nx, Cl.ny, Cl.nz);
-        hipLaunchKernelGGL(k_restrict<T>, tiles_grid(call), tiles_block(), 0, c->s, lv, cv, L.cx, L.cy, L.cz,
-                           (const T *)B.r, CB.b, call);
         TB_LAUNCH_CHECK(c);
         if (vcycle_level<T>(c, l + 1)) return -1;
         // the post pairs alternate buffers, so the prolonged iterate goes to
         // the buffer from which they end in x
         T *dst = (post % 2 == 0) ? B.x : B.a;
-        hipLaunchKernelGGL(k_prolong<T>, tiles_grid(all), tiles_block(), 0, c->s, lv, cv, L.cx, L.cy, L.cz,
-                           (const T *)CB.x, (const T *)cur, dst, all);
+        prolong<T>(c, l, (const T *)cur, dst);
         cur = dst;
         for (int s = 0; s < post; s++) pair(1, false);
         TB_LAUNCH_CHECK(c);
@@ -961,18 +1177,13 @@ static int vcycle_level(tb_ctx *c, int l) {
     } else {
         hipLaunchKernelGGL(k_zero<T>, tiles_grid(all), tiles_block(), 0, c->s, lv, B.x, all);
     }
-    hipLaunchKernelGGL(k_mg_resid<T>, tiles_grid(all), tiles_block(), 0, c->s, lv, (const T *)B.x, (const T *)B.b,
-                       B.r, all);
+    resid_restrict<T>(c, l, (const T *)B.x);
     const Level &Cl = c->lev[l + 1];
     MGBuf<T> CB = bufs<T>(c, l + 1);
     LV<T> cv = lv_of<T>(Cl);
-    Tiles call = make_tiles(Cl.nx, Cl.ny, Cl.nz);
-    hipLaunchKernelGGL(k_restrict<T>, tiles_grid(call), tiles_block(), 0, c->s, lv, cv, L.cx, L.cy, L.cz,
-                       (const T *)B.r, CB.b, call);
     TB_LAUNCH_CHECK(c);
     if (vcycle_level<T>(c, l + 1)) return -1;
-    hipLaunchKernelGGL(k_prolong<T>, tiles_grid(all), tiles_block(), 0, c->s, lv, cv, L.cx, L.cy, L.cz,
-                       (const T *)CB.x, (const T *)B.x, B.x, all);
+    prolong<T>(c, l, (const T *)B.x, B.x);
     for (int s = 0; s < post; s++) {
         smooth(1, 0);
         smooth(0, 0);
