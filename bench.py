@@ -47,7 +47,7 @@ WEIGHTS = os.path.join(ROOT, "turbins_amd", "weights", "poisson_guess.f32")
 # HBM bytes per instrumented launch measured with rocprofv3 FETCH_SIZE /
 # WRITE_SIZE on this workload, scaled by a plain copy of known size
 # (profiles/hbm_traffic_bump512x256.md); only valid for the kernels as built
-MEASURED_TRAFFIC = {"HELM_GS": 1.0233e9}
+MEASURED_TRAFFIC = {"HELM_GS": 520.1e6}
 
 
 def log(*a):
@@ -117,11 +117,11 @@ def roofline(prof, stats, c):
             # BiCGStab iteration also reads rhat (8 B/point)
             total = 17 * NT * n + 8 * NT * helm_its
         elif name == "HELM_GS":
-            # one red + one black launch: each reads x (8 B) and writes half of
-            # x (4 B), reads half of b (4 B) and of the status bytes (0.5 B)
-            total = 33 * NT * n
+            # one fused red+black sweep pair: x, b and the status byte read
+            # once, the new x written once
+            total = 25 * NT * n
         elif name == "RBGS0":
-            total = 16 * ncell * n      # x read, half of b read, half of x written
+            total = 24 * ncell * n      # FP64 fused pair: x, b read, x written
         elif name == "RBGS0_F32":
             total = 8 * ncell * n
         else:
