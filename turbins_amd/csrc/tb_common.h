@@ -75,6 +75,23 @@ __device__ inline bool tile_point(const Tiles &t, int &i, int &j, int &k) {
     return i < t.nx && j < t.ny;
 }
 
+// Same tiles, numbered with k fastest: the blocks an XCD runs back to back
+// cover one 64 x 4 column plane after plane, so a stencil reaching several
+// planes in z finds them in that XCD's L2 (the x-fastest order above revisits
+// a column only after a whole plane of tiles).
+__device__ inline bool tile_point_kfast(const Tiles &t, int &i, int &j, int &k) {
+    long b = blockIdx.x;
+    long lt = (b % NXCD) * t.per_xcd + b / NXCD;
+    if (lt >= t.n) return false;
+    k = (int)(lt % t.nz);
+    long r = lt / t.nz;
+    int tx = (int)(r % t.tx);
+    int ty = (int)(r / t.tx);
+    i = tx * TILE_X + threadIdx.x;
+    j = ty * TILE_Y + threadIdx.y;
+    return i < t.nx && j < t.ny;
+}
+
 // Column tiles for k-marching stencils: a 64 x 4 block owns a tile of
 // columns and walks KCHUNK planes upwards, keeping the k-1, k, k+1 values of
 // its column in registers, so each plane is read from HBM once.

@@ -173,7 +173,7 @@ struct MomArgs {
 template <int R>
 __global__ void __launch_bounds__(256) k_eno_momentum(MomArgs a, Tiles t) {
     int i, j, k;
-    if (!tile_point(t, i, j, k)) return;
+    if (!tile_point_kfast(t, i, j, k)) return;
     const Dims &d = a.d;
     const GridDev &g = a.g;
     long id = gidx(d, i, j, k);
@@ -236,7 +236,7 @@ struct ConcArgs {
 template <int R>
 __global__ void __launch_bounds__(256) k_eno_conc(ConcArgs a, Tiles t) {
     int i, j, k;
-    if (!tile_point(t, i, j, k)) return;
+    if (!tile_point_kfast(t, i, j, k)) return;
     const Dims &d = a.d;
     const GridDev &g = a.g;
     long id = gidx(d, i, j, k);
