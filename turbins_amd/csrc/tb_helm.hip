@@ -341,7 +341,7 @@ __device__ inline double gs_value(const HelmOp &H, const AxisGS &X, const AxisGS
 // axis, see AxisGS), while xo and b are read from HBM once and xn written
 // once (the in-place form streams both arrays twice).
 //
-// A 256-thread block owns a 64 x 8 column tile and a chunk of GS_TK planes
+// A 512-thread block owns a 64 x 16 column tile and a chunk of GS_TK planes
 // and walks k upwards through two LDS rings of three planes: xo with a
 // two-point halo, and the red-pass values (black positions holding xo) with a
 // one-point halo.  Per plane m: red pass on plane m (tile + halo, recomputed
@@ -350,11 +350,11 @@ __device__ inline double gs_value(const HelmOp &H, const AxisGS &X, const AxisGS
 // xo plane, statuses and right-hand sides of the next plane are fetched into
 // registers one step ahead, so their latency overlaps the passes; row
 // coefficients come from per-axis tables built in LDS once per block.
-constexpr int GS_TX = 64, GS_TY = 8, GS_TK = 16;
+constexpr int GS_TX = 64, GS_TY = 16, GS_TK = 16;
 constexpr int GS_OX = GS_TX + 4, GS_OY = GS_TY + 4;   // xo ring: halo 2
 constexpr int GS_HX = GS_TX + 2, GS_HY = GS_TY + 2;   // red-pass ring: halo 1
 constexpr int GS_OZ = GS_TK + 4;                      // z table: k0-2 .. k0+GS_TK+1
-constexpr int GS_THREADS = 256;
+constexpr int GS_THREADS = 512;
 constexpr int GS_OLD_PER = (GS_OY * GS_OX + GS_THREADS - 1) / GS_THREADS;
 constexpr int GS_RED = GS_HY * (GS_HX / 2);           // red points of one halo plane
 constexpr int GS_RED_PER = (GS_RED + GS_THREADS - 1) / GS_THREADS;
@@ -616,9 +616,10 @@ static int copy_box(tb_ctx *c, double *dst, const double *src) {
     return 0;
 }
 
-// returns 0 and sets ok when converged; ok = false when the sweeps stall
-// (contraction per pair worse than 0.5) or run out of sweeps; x holds the
-// last iterate either way.  The true residual costs about two thirds of a
+// returns 0 and sets ok when converged (the field of quantity q then holds
+// the solution, possibly in the buffer that was the scratch vector); ok =
+// false when the sweeps stall (contraction per pair worse than 0.5) or run
+// out of sweeps, x then holds the last iterate.  The true residual costs about two thirds of a
 // sweep pair, so it is evaluated only where the solve may have converged:
 // first after as many pairs as the last solve of the same quantity needed
 // (the count is steady from step to step), then after the pairs predicted
@@ -662,7 +663,16 @@ static int helm_gs_solve(tb_ctx *c, const HelmOp &H, double *x, const double *b,
         c->helm_sweeps[q] = (its > 1 && res / r1 <= rtol) ? its - 1 : its;
         if (prev == prev) c->helm_rate[q] = rate;
     }
-    if (cur != x && copy_box(c, x, cur)) return -1;
+    if (cur != x) {
+        if (ok && x == c->f[TB_F_U + q] && alt == x) {
+            // the field and the scratch vector trade buffers (no kernel or
+            // graph keeps either pointer between calls)
+            c->f[TB_F_U + q] = cur;
+            c->kv[6] = x;
+        } else if (copy_box(c, x, cur)) {
+            return -1;   // BiCGStab continues from x
+        }
+    }
     return 0;
 }
 
