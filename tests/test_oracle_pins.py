@@ -297,6 +297,17 @@ def test_vcycle_symmetric_and_contracting():
     assert z1 @ r1 > 0
 
 
+def test_gmg_hierarchy_rule():
+    """Semi-coarsening of the directions at most half as coarse as the
+    coarsest one, full coarsening once the spacings are within a factor 2,
+    down to <= 64 cells (512x256x128 over 16x1x1: h = 1/32, 1/256, 1/128)."""
+    c = W.case("bump512x256")
+    o = oracle.Oracle(W.config_for(c))
+    dims = [tuple(d) for d in o.gmg_levels()]
+    assert dims == [(512, 256, 128), (512, 128, 64), (512, 64, 32), (512, 32, 32), (256, 16, 16), (128, 8, 8),
+                    (64, 4, 4), (32, 2, 2), (16, 2, 2)]
+
+
 # ---------------------------------------------------------------------------
 # geometry: classification and signed distance
 

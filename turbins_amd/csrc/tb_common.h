@@ -224,6 +224,8 @@ struct tb_ctx {
 
     // Krylov work vectors (full padded layout)
     double *kv[8] = {};
+    double *eno_w = nullptr;     // interpolation weights of the momentum transposes (tb_eno.hip)
+    int eno_wmax = 0;
     double *red = nullptr;       // partial sums [RED_SLOTS][RED_MAX_BLOCKS]
     double *scal = nullptr;      // device scalars
     double *hscal = nullptr;     // pinned host mirror of scal
@@ -250,7 +252,7 @@ struct tb_ctx {
     // Poisson history for the learned guess
     double *phi_hist[3] = {};    // last three Poisson solutions, newest first
     int n_hist = 0;
-    double *b_hist[2] = {};      // their right-hand sides (learned mode only)
+    double *b_hist[3] = {};      // their right-hand sides (learned mode only)
     int nb_hist = 0;
     tb_net *net = nullptr;       // attached learned guess (not owned)
     tb_net *net_record = nullptr;    // snapshot sink after every solve (not owned)

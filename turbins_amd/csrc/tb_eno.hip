@@ -12,6 +12,7 @@
 // centres (Velocity_cell_center, Velocity.c:1476-1602) are interpolated in
 // registers.  All arithmetic keeps the reference's order (no contraction),
 // so results are bit-identical to the line-by-line evaluation.
+#include <algorithm>
 #include <cmath>
 
 #include "tb_common.h"
@@ -35,6 +36,21 @@ __device__ inline double interp4d(double q0, double q1, double q2, double q3, do
     double Wp2 = 1.0 - Wp0;
     return Wp0 * q_bottom + Wp2 * q_top;
 }
+
+// interp4d with its two weights given: Wa = |(x1 - X10) / (X11 - X10)| and
+// Wb = |(x2 - X20) / (X22 - X20)| depend on one grid index each, so they come
+// from tables built once with the same operations (1/(X11 - X10) first, then
+// the product), which keeps the result bit-identical
+__device__ inline double interp4w(double q0, double q1, double q2, double q3, double Wa, double Wb) {
+    double Wp1 = 1.0 - Wa;
+    double q_top = Wa * q2 + Wp1 * q3;
+    double q_bottom = Wa * q0 + Wp1 * q1;
+    double Wp2 = 1.0 - Wb;
+    return Wb * q_bottom + Wp2 * q_top;
+}
+
+// the twelve weight tables of the transposes, one per (interpolation, axis)
+enum { W_VU_X, W_VU_Y, W_WU_X, W_WU_Z, W_UV_X, W_UV_Y, W_WV_Z, W_WV_Y, W_UW_X, W_UW_Z, W_VW_Z, W_VW_Y, W_COUNT };
 
 // One line direction of one quantity: interior span and wall ghost rules.
 struct LineSpec {
@@ -164,6 +180,8 @@ struct QLines {
 struct MomArgs {
     Dims d;
     GridDev g;
+    const double *W;       // W_COUNT tables of Wmax entries
+    int Wmax;
     const double *u, *v, *w;
     const uint8_t *su, *sv, *sw;
     double *cu, *cv, *cw;
@@ -178,14 +196,16 @@ __global__ void __launch_bounds__(256) k_eno_momentum(MomArgs a, Tiles t) {
     const GridDev &g = a.g;
     long id = gidx(d, i, j, k);
     const double *u = a.u, *v = a.v, *w = a.w;
+    const double *W = a.W;
+    const int wm = a.Wmax;
     // u node
     {
         double r = 0.0;
         if (a.su[id] != TB_BOUNDARY) {
-            double vu = interp4d(v[id - 1], v[id], v[id + d.px - 1], v[id + d.px], g.xv[i - 1], g.xv[i], g.yv[j],
-                                 g.yv[j + 1], g.xu[i], g.yu[j]);
-            double wu = interp4d(w[id - 1], w[id], w[id + d.plane - 1], w[id + d.plane], g.xw[i - 1], g.xw[i],
-                                 g.zw[k], g.zw[k + 1], g.xu[i], g.zu[k]);
+            double vu = interp4w(v[id - 1], v[id], v[id + d.px - 1], v[id + d.px], W[W_VU_X * wm + i],
+                                 W[W_VU_Y * wm + j]);
+            double wu = interp4w(w[id - 1], w[id], w[id + d.plane - 1], w[id + d.plane], W[W_WU_X * wm + i],
+                                 W[W_WU_Z * wm + k]);
             r = eno_line_point<R>(a.L[0].ax[0], u, id, i, u[id]);
             r += eno_line_point<R>(a.L[0].ax[1], u, id, j, vu);
             r += eno_line_point<R>(a.L[0].ax[2], u, id, k, wu);
@@ -196,10 +216,10 @@ __global__ void __launch_bounds__(256) k_eno_momentum(MomArgs a, Tiles t) {
     {
         double r = 0.0;
         if (a.sv[id] != TB_BOUNDARY) {
-            double uv = interp4d(u[id - d.px], u[id - d.px + 1], u[id], u[id + 1], g.xu[i], g.xu[i + 1], g.yu[j - 1],
-                                 g.yu[j], g.xv[i], g.yv[j]);
-            double wv = interp4d(w[id - d.px], w[id + d.plane - d.px], w[id], w[id + d.plane], g.zw[k], g.zw[k + 1],
-                                 g.yw[j - 1], g.yw[j], g.zv[k], g.yv[j]);
+            double uv = interp4w(u[id - d.px], u[id - d.px + 1], u[id], u[id + 1], W[W_UV_X * wm + i],
+                                 W[W_UV_Y * wm + j]);
+            double wv = interp4w(w[id - d.px], w[id + d.plane - d.px], w[id], w[id + d.plane], W[W_WV_Z * wm + k],
+                                 W[W_WV_Y * wm + j]);
             r = eno_line_point<R>(a.L[1].ax[0], v, id, i, uv);
             r += eno_line_point<R>(a.L[1].ax[1], v, id, j, v[id]);
             r += eno_line_point<R>(a.L[1].ax[2], v, id, k, wv);
@@ -210,10 +230,10 @@ __global__ void __launch_bounds__(256) k_eno_momentum(MomArgs a, Tiles t) {
     {
         double r = 0.0;
         if (a.sw[id] != TB_BOUNDARY) {
-            double uw = interp4d(u[id - d.plane], u[id - d.plane + 1], u[id], u[id + 1], g.xu[i], g.xu[i + 1],
-                                 g.zu[k - 1], g.zu[k], g.xw[i], g.zw[k]);
-            double vw = interp4d(v[id - d.plane], v[id], v[id - d.plane + d.px], v[id + d.px], g.zv[k - 1], g.zv[k],
-                                 g.yv[j], g.yv[j + 1], g.zw[k], g.yw[j]);
+            double uw = interp4w(u[id - d.plane], u[id - d.plane + 1], u[id], u[id + 1], W[W_UW_X * wm + i],
+                                 W[W_UW_Z * wm + k]);
+            double vw = interp4w(v[id - d.plane], v[id], v[id - d.plane + d.px], v[id + d.px], W[W_VW_Z * wm + k],
+                                 W[W_VW_Y * wm + j]);
             r = eno_line_point<R>(a.L[2].ax[0], w, id, i, uw);
             r += eno_line_point<R>(a.L[2].ax[1], w, id, j, vw);
             r += eno_line_point<R>(a.L[2].ax[2], w, id, k, w[id]);
@@ -296,9 +316,52 @@ static QLines make_lines(tb_ctx *c, int q) {
     return Q;
 }
 
+// weight Wa = |(x1 - X10) * (1 / (X11 - X10))| of interp4d, 0 where an
+// index leaves the grid (those nodes are BOUNDARY and never interpolated)
+static double iw(const std::vector<double> &x1, int i1, const std::vector<double> &X, int i0, int i11) {
+    const int n = (int)X.size();
+    if (i1 < 0 || i1 >= (int)x1.size() || i0 < 0 || i0 >= n || i11 < 0 || i11 >= n) return 0.0;
+    const double aD = 1.0 / (X[i11] - X[i0]);
+    return fabs((x1[i1] - X[i0]) * aD);
+}
+
+static int eno_weights(tb_ctx *c) {
+    if (c->eno_w) return 0;
+    const auto &xu = c->hgrid[3], &yu = c->hgrid[4], &zu = c->hgrid[5];
+    const auto &xv = c->hgrid[6], &yv = c->hgrid[7], &zv = c->hgrid[8];
+    const auto &xw = c->hgrid[9], &yw = c->hgrid[10], &zw = c->hgrid[11];
+    const int wm = std::max(c->d.NX, std::max(c->d.NY, c->d.NZ));
+    std::vector<double> W((size_t)W_COUNT * wm, 0.0);
+    for (int i = 0; i < c->d.NX; i++) {
+        W[W_VU_X * wm + i] = iw(xu, i, xv, i - 1, i);
+        W[W_WU_X * wm + i] = iw(xu, i, xw, i - 1, i);
+        W[W_UV_X * wm + i] = iw(xv, i, xu, i, i + 1);
+        W[W_UW_X * wm + i] = iw(xw, i, xu, i, i + 1);
+    }
+    for (int j = 0; j < c->d.NY; j++) {
+        W[W_VU_Y * wm + j] = iw(yu, j, yv, j, j + 1);
+        W[W_UV_Y * wm + j] = iw(yv, j, yu, j - 1, j);
+        W[W_WV_Y * wm + j] = iw(yv, j, yw, j - 1, j);
+        W[W_VW_Y * wm + j] = iw(yw, j, yv, j, j + 1);
+    }
+    for (int k = 0; k < c->d.NZ; k++) {
+        W[W_WU_Z * wm + k] = iw(zu, k, zw, k, k + 1);
+        W[W_WV_Z * wm + k] = iw(zv, k, zw, k, k + 1);
+        W[W_UW_Z * wm + k] = iw(zw, k, zu, k - 1, k);
+        W[W_VW_Z * wm + k] = iw(zw, k, zv, k - 1, k);
+    }
+    TB_CHECK(c, hipMalloc((void **)&c->eno_w, sizeof(double) * W.size()));
+    TB_CHECK(c, hipMemcpy(c->eno_w, W.data(), sizeof(double) * W.size(), hipMemcpyHostToDevice));
+    c->eno_wmax = wm;
+    return 0;
+}
+
 int tb_eno_momentum(tb_ctx *c) {
+    if (eno_weights(c)) return -1;
     MomArgs a;
     a.d = c->d; a.g = c->g;
+    a.W = c->eno_w;
+    a.Wmax = c->eno_wmax;
     a.u = c->f[TB_F_U]; a.v = c->f[TB_F_V]; a.w = c->f[TB_F_W];
     a.su = c->st[0]; a.sv = c->st[1]; a.sw = c->st[2];
     a.cu = c->f[TB_F_CONV_U]; a.cv = c->f[TB_F_CONV_V]; a.cw = c->f[TB_F_CONV_W];

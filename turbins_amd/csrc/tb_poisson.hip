@@ -1537,7 +1537,7 @@ extern "C" int tb_set_mixed_inner(tb_ctx *c, double reduction_floor, int maxit) 
 
 static int mixed_ir(tb_ctx *c, const PoisCtx &P, double *x, const double *b, double bn, double rtol, int maxit,
                     int &its, int &vcy, double &res, bool &stalled) {
-    double *r = c->kv[0];
+    double *r = nullptr;   // only the FP32 copy of the residual and its norms are needed
     Level &L0 = c->lev[0];
     double rr, bb;
     stalled = false;
@@ -1657,9 +1657,14 @@ static int learned_guess(tb_ctx *c, const PoisCtx &P, double *x, const double *b
         TB_LAUNCH_CHECK(c);
         use_d = 1;
     }
+    // without the network the third direction is phi_{n-3}, whose image
+    // A phi_{n-3} is its stored right-hand side
+    const bool hist3 = !use_d && c->n_hist >= 3 && c->nb_hist >= 3;
+    const double *d3 = hist3 ? c->phi_hist[2] : dd, *q3 = hist3 ? c->b_hist[2] : q;
+    if (hist3) use_d = 1;
     hipLaunchKernelGGL(k_subspace_dots, dim3(PB), dim3(PT), 0, c->s, n, (const double *)c->phi_hist[0],
-                       (const double *)c->phi_hist[1], (const double *)dd, (const double *)c->b_hist[0],
-                       (const double *)c->b_hist[1], (const double *)q, b, use_d, c->red);
+                       (const double *)c->phi_hist[1], d3, (const double *)c->b_hist[0],
+                       (const double *)c->b_hist[1], q3, b, use_d, c->red);
     hipLaunchKernelGGL(k_sum_cols, dim3(1), dim3(64), 0, c->s, (const double *)c->red, PB, 12, c->scal + S_SUB);
     TB_LAUNCH_CHECK(c);
     if (read_s(c, S_SUB, 12)) return -1;
@@ -1672,7 +1677,7 @@ static int learned_guess(tb_ctx *c, const PoisCtx &P, double *x, const double *b
         a[0] = a[1] = a[2] = 0.0;
     }
     if (ndir == 0) return 0;
-    if (ndir == 3) {
+    if (ndir == 3 && !hist3) {
         // energies J = -a.r/2 of the Galerkin guesses with and without d
         double a2[3] = {0.0, 0.0, 0.0};
         if (small_solve(2, M, r, a2)) {
@@ -1682,12 +1687,11 @@ static int learned_guess(tb_ctx *c, const PoisCtx &P, double *x, const double *b
         }
     }
     hipLaunchKernelGGL(k_combine3, dim3(PB), dim3(PT), 0, c->s, n, x, (const double *)c->phi_hist[0],
-                       (const double *)c->phi_hist[1], (const double *)dd, a[0], ndir > 1 ? a[1] : 0.0,
-                       ndir > 2 ? a[2] : 0.0);
+                       (const double *)c->phi_hist[1], d3, a[0], ndir > 1 ? a[1] : 0.0, ndir > 2 ? a[2] : 0.0);
     TB_LAUNCH_CHECK(c);
     double rr, bb;
     if (shift_corner(c, P, x)) return -1;
-    if (residual(c, P, false, x, b, c->kv[0], nullptr, rr, bb)) return -1;
+    if (residual(c, P, false, x, b, nullptr, nullptr, rr, bb)) return -1;   // norms only
     resg = sqrt(rr) / bn;
     return 0;
 }
@@ -1743,7 +1747,7 @@ extern "C" int tb_poisson_solve(tb_ctx *c, tb_solve_stats *st) {
     TB_CHECK(c, hipEventRecord(c->ev0, c->s));
     double rr, bb;
     if (shift_corner(c, P, x)) return -1;
-    if (residual(c, P, false, x, b, c->kv[0], nullptr, rr, bb)) return -1;
+    if (residual(c, P, false, x, b, nullptr, nullptr, rr, bb)) return -1;   // norms only
     double bn = sqrt(bb);
     tb_solve_stats loc;
     memset(&loc, 0, sizeof(loc));
@@ -1794,9 +1798,12 @@ extern "C" int tb_poisson_solve(tb_ctx *c, tb_solve_stats *st) {
     TB_CHECK(c, hipMemcpyAsync(c->phi_hist[0], x, sizeof(double) * c->d.ntot, hipMemcpyDeviceToDevice, c->s));
     if (c->n_hist < 3) c->n_hist++;
     if (c->cfg.pois_mode == TB_POIS_LEARNED_IR) {
-        std::swap(c->b_hist[0], c->b_hist[1]);
+        double *oldest = c->b_hist[2];
+        c->b_hist[2] = c->b_hist[1];
+        c->b_hist[1] = c->b_hist[0];
+        c->b_hist[0] = oldest;
         TB_CHECK(c, hipMemcpyAsync(c->b_hist[0], b, sizeof(double) * c->d.ntot, hipMemcpyDeviceToDevice, c->s));
-        if (c->nb_hist < 2) c->nb_hist++;
+        if (c->nb_hist < 3) c->nb_hist++;
     } else {
         c->nb_hist = 0;
     }
