@@ -255,6 +255,11 @@ int tb_set_helmholtz_method(tb_ctx *ctx, int method);
 /* mixed Poisson mode: smallest residual reduction one FP32 inner solve is
  * asked for (default 1e-5) and its iteration cap (default 40) */
 int tb_set_mixed_inner(tb_ctx *ctx, double reduction_floor, int maxit);
+/* learned mode: number of previous solutions (1..6, default 6) whose
+ * A-norm-optimal combination, with the network's direction when one is
+ * attached and its gate open, is the initial guess (the reference starts
+ * from the previous pressure alone, Solver.c:256) */
+int tb_set_guess_history(tb_ctx *ctx, int h);
 /* multigrid V-cycles replayed from HIP graphs (default 1) or launched
  * kernel by kernel (0); results are identical */
 int tb_set_graphs(tb_ctx *ctx, int on);

@@ -64,6 +64,7 @@ def lib():
     L.tb_set_poisson_mode.argtypes = [P, C.c_int, C.c_double]
     L.tb_set_helmholtz_rtol.argtypes = [P, C.c_double]
     L.tb_set_mixed_inner.argtypes = [P, C.c_double, C.c_int]
+    L.tb_set_guess_history.argtypes = [P, C.c_int]
     L.tb_set_helmholtz_method.argtypes = [P, C.c_int]
     L.tb_set_graphs.argtypes = [P, C.c_int]
     L.tb_profile_enable.argtypes = [P, C.c_int]
@@ -249,6 +250,9 @@ class Solver:
 
     def set_mixed_inner(self, reduction_floor, maxit=40):
         return self._ok(self.L.tb_set_mixed_inner(self.h, reduction_floor, maxit), "tb_set_mixed_inner")
+
+    def set_guess_history(self, h):
+        return self._ok(self.L.tb_set_guess_history(self.h, h), "tb_set_guess_history")
 
     def profile(self, on=True):
         return self._ok(self.L.tb_profile_enable(self.h, 1 if on else 0), "tb_profile_enable")

@@ -179,6 +179,11 @@ __device__ inline void block_sum(double (&v)[NV], double *sh /* [NV*16] */) {
 
 // ---------------------------------------------------------------------------
 // GMG level description (cell-centred arrays)
+// history depth of the Galerkin initial guess, and its directions with the
+// network's
+constexpr int GUESS_HIST = 6;
+constexpr int GUESS_DIRS = GUESS_HIST + 1;
+
 struct Level {
     int nx, ny, nz;       // cells
     int px;               // row pitch (elements)
@@ -250,9 +255,10 @@ struct tb_ctx {
     bool mg_built = false;
 
     // Poisson history for the learned guess
-    double *phi_hist[3] = {};    // last three Poisson solutions, newest first
+    double *phi_hist[GUESS_HIST] = {};   // last Poisson solutions, newest first
     int n_hist = 0;
-    double *b_hist[3] = {};      // their right-hand sides (learned mode only)
+    double *b_hist[GUESS_HIST] = {};     // their right-hand sides (learned mode only)
+    int guess_hist = GUESS_HIST; // solutions in the Galerkin initial guess (tb_set_guess_history)
     int nb_hist = 0;
     tb_net *net = nullptr;       // attached learned guess (not owned)
     tb_net *net_record = nullptr;    // snapshot sink after every solve (not owned)
@@ -289,7 +295,7 @@ int tb_prof_flush(tb_ctx *c);
 // device scalar slots (ctx->scal)
 enum {
     S_RHO = 0, S_ALPHA, S_OMEGA, S_BETA, S_RZ, S_PQ, S_RR, S_TS, S_TT, S_RHV,
-    S_BB, S_MEAN, S_CORNER, S_TMP0, S_TMP1, S_TMP2, S_SUB = 16, S_COUNT = 32
+    S_BB, S_MEAN, S_CORNER, S_TMP0, S_TMP1, S_TMP2, S_SUB = 16, S_COUNT = 16 + GUESS_DIRS * GUESS_DIRS + GUESS_DIRS
 };
 
 // error helpers -------------------------------------------------------------

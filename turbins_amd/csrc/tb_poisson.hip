@@ -1569,35 +1569,43 @@ static int mixed_ir(tb_ctx *c, const PoisCtx &P, double *x, const double *b, dou
 
 // ---------------------------------------------------------------------------
 // Initial guess of the learned mode: the A-norm-optimal combination
-//   x0 = a0 phi_{n-1} + a1 phi_{n-2} + a2 d
-// of the last two solutions and the network's correction d (when a network
-// is attached), i.e. the Galerkin solution of A x = b on that subspace.
-// A phi_{n-k} = b_{n-k} holds to the solve tolerance, so only d needs an
-// operator application.  Returns the true relative residual of x0 (pinned
-// operator) in resg and the number of directions used in ndir (0: none
-// usable; the caller then restores phi_{n-1}).
+//   x0 = sum_k a_k phi_{n-k} (k = 1..H) + a_d d
+// of the last H solutions and the network's correction d (when a network is
+// attached and its gate is open), i.e. the Galerkin solution of A x = b on
+// that subspace.  A phi_{n-k} = b_{n-k} holds to the solve tolerance, so only
+// d needs an operator application.  Returns the true relative residual of
+// x0 (pinned operator) in resg and the number of directions used in ndir
+// (0: none usable; the caller then restores phi_{n-1}).
 
-// the 12 sums g_i . A g_j (i, j < 3) and g_i . b
-__global__ void __launch_bounds__(PT) k_subspace_dots(long n, const double *__restrict__ h0, const double *__restrict__ h1,
-                                                      const double *__restrict__ dd, const double *__restrict__ b0,
-                                                      const double *__restrict__ b1, const double *__restrict__ q,
-                                                      const double *__restrict__ b, int use_d, double *part) {
-    __shared__ double sh[12 * 16];
-    double a[12];
+struct Dirs {
+    const double *g[GUESS_DIRS];    // directions
+    const double *ag[GUESS_DIRS];   // their images A g
+};
+
+// the M*M sums g_i . A g_j and the M sums g_i . b, per block
+template <int M>
+__global__ void __launch_bounds__(PT) k_subspace_dots(long n, Dirs D, const double *__restrict__ b, double *part) {
+    constexpr int NS = M * M + M;
+    __shared__ double sh[NS * 16];
+    double a[NS];
 #pragma unroll
-    for (int k = 0; k < 12; k++) a[k] = 0.0;
+    for (int k = 0; k < NS; k++) a[k] = 0.0;
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-        const double g0 = h0[i], g1 = h1[i], g2 = use_d ? dd[i] : 0.0;
-        const double A0 = b0[i], A1 = b1[i], A2 = use_d ? q[i] : 0.0, bi = b[i];
-        a[0] += g0 * A0; a[1] += g0 * A1; a[2] += g0 * A2;
-        a[3] += g1 * A0; a[4] += g1 * A1; a[5] += g1 * A2;
-        a[6] += g2 * A0; a[7] += g2 * A1; a[8] += g2 * A2;
-        a[9] += g0 * bi; a[10] += g1 * bi; a[11] += g2 * bi;
+        double g[M], A[M];
+#pragma unroll
+        for (int u = 0; u < M; u++) { g[u] = D.g[u][i]; A[u] = D.ag[u][i]; }
+        const double bi = b[i];
+#pragma unroll
+        for (int u = 0; u < M; u++) {
+#pragma unroll
+            for (int v = 0; v < M; v++) a[u * M + v] += g[u] * A[v];
+            a[M * M + u] += g[u] * bi;
+        }
     }
-    block_sum<12>(a, sh);
+    block_sum<NS>(a, sh);
     if (threadIdx.x == 0)
 #pragma unroll
-        for (int k = 0; k < 12; k++) part[(long)blockIdx.x * 12 + k] = a[k];
+        for (int k = 0; k < NS; k++) part[(long)blockIdx.x * NS + k] = a[k];
 }
 
 // out[k] = sum over blocks of part[blk * ncol + k], fixed order
@@ -1609,16 +1617,22 @@ __global__ void k_sum_cols(const double *part, int nblk, int ncol, double *out) 
     out[k] = s;
 }
 
-__global__ void k_combine3(long n, double *__restrict__ x, const double *__restrict__ h0, const double *__restrict__ h1,
-                           const double *__restrict__ dd, double a0, double a1, double a2) {
-    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
-        x[i] = a0 * h0[i] + a1 * h1[i] + (a2 != 0.0 ? a2 * dd[i] : 0.0);
+struct Coefs {
+    double a[GUESS_DIRS];
+};
+
+__global__ void k_combine(long n, double *__restrict__ x, Dirs D, int m, Coefs C) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        double s = 0.0;
+        for (int u = 0; u < m; u++) s += C.a[u] * D.g[u][i];
+        x[i] = s;
+    }
 }
 
 // solve the m x m system M a = r by Gaussian elimination with partial
 // pivoting; false when a pivot falls below 1e-12 of the largest entry
-static bool small_solve(int m, double M[3][3], double r[3], double a[3]) {
-    double A[3][4];
+static bool small_solve(int m, const double M[GUESS_DIRS][GUESS_DIRS], const double *r, double *a) {
+    double A[GUESS_DIRS][GUESS_DIRS + 1];
     double mx = 0.0;
     for (int i = 0; i < m; i++) {
         for (int j = 0; j < m; j++) { A[i][j] = M[i][j]; mx = std::max(mx, std::fabs(M[i][j])); }
@@ -1641,12 +1655,23 @@ static bool small_solve(int m, double M[3][3], double r[3], double a[3]) {
     return true;
 }
 
+template <int M>
+static void launch_dots(tb_ctx *c, long n, const Dirs &D, const double *b) {
+    hipLaunchKernelGGL(k_subspace_dots<M>, dim3(PB), dim3(PT), 0, c->s, n, D, b, c->red);
+}
+
 static int learned_guess(tb_ctx *c, const PoisCtx &P, double *x, const double *b, double bn, double &resg, int &ndir) {
     const long n = c->d.ntot;
     double *dd = c->kv[5], *q = c->kv[6];
     ndir = 0;
-    int use_d = 0;
     c->net_used = false;
+    int H = c->guess_hist;
+    if (H > c->n_hist) H = c->n_hist;
+    if (H > c->nb_hist) H = c->nb_hist;
+    Dirs D;
+    for (int k = 0; k < H; k++) { D.g[k] = c->phi_hist[k]; D.ag[k] = c->b_hist[k]; }
+    int m = H;
+    bool use_d = false;
     if (c->net && c->net_open) {
         TB_CHECK(c, hipMemsetAsync(dd, 0, sizeof(double) * n, c->s));
         TB_CHECK(c, hipEventRecord(c->net_ev[0], c->s));
@@ -1655,44 +1680,66 @@ static int learned_guess(tb_ctx *c, const PoisCtx &P, double *x, const double *b
         hipLaunchKernelGGL(k_pois_km<0>, ktiles_grid(P.kt), tiles_block(), 0, c->s, P.Pn, (const double *)dd,
                            (const double *)nullptr, q, (float *)nullptr, c->red, P.kt);
         TB_LAUNCH_CHECK(c);
-        use_d = 1;
+        D.g[m] = dd;
+        D.ag[m] = q;
+        m++;
+        use_d = true;
     }
-    // without the network the third direction is phi_{n-3}, whose image
-    // A phi_{n-3} is its stored right-hand side
-    const bool hist3 = !use_d && c->n_hist >= 3 && c->nb_hist >= 3;
-    const double *d3 = hist3 ? c->phi_hist[2] : dd, *q3 = hist3 ? c->b_hist[2] : q;
-    if (hist3) use_d = 1;
-    hipLaunchKernelGGL(k_subspace_dots, dim3(PB), dim3(PT), 0, c->s, n, (const double *)c->phi_hist[0],
-                       (const double *)c->phi_hist[1], d3, (const double *)c->b_hist[0],
-                       (const double *)c->b_hist[1], q3, b, use_d, c->red);
-    hipLaunchKernelGGL(k_sum_cols, dim3(1), dim3(64), 0, c->s, (const double *)c->red, PB, 12, c->scal + S_SUB);
+    if (m < 1) return 0;
+    switch (m) {
+    case 1: launch_dots<1>(c, n, D, b); break;
+    case 2: launch_dots<2>(c, n, D, b); break;
+    case 3: launch_dots<3>(c, n, D, b); break;
+    case 4: launch_dots<4>(c, n, D, b); break;
+    case 5: launch_dots<5>(c, n, D, b); break;
+    case 6: launch_dots<6>(c, n, D, b); break;
+    default: launch_dots<7>(c, n, D, b); break;
+    }
+    const int ns = m * m + m;
+    hipLaunchKernelGGL(k_sum_cols, dim3(1), dim3(64), 0, c->s, (const double *)c->red, PB, ns, c->scal + S_SUB);
     TB_LAUNCH_CHECK(c);
-    if (read_s(c, S_SUB, 12)) return -1;
+    if (read_s(c, S_SUB, ns)) return -1;
     const double *h = c->hscal + S_SUB;
-    double M[3][3], r[3] = {h[9], h[10], h[11]}, a[3] = {0.0, 0.0, 0.0};
-    for (int i = 0; i < 3; i++)
-        for (int j = 0; j < 3; j++) M[i][j] = 0.5 * (h[3 * i + j] + h[3 * j + i]);
-    for (int m = use_d ? 3 : 2; m >= 1; m--) {
-        if (small_solve(m, M, r, a)) { ndir = m; break; }
-        a[0] = a[1] = a[2] = 0.0;
+    double M[GUESS_DIRS][GUESS_DIRS], r[GUESS_DIRS], a[GUESS_DIRS];
+    for (int i = 0; i < m; i++) {
+        r[i] = h[m * m + i];
+        for (int j = 0; j < m; j++) M[i][j] = 0.5 * (h[i * m + j] + h[j * m + i]);
+    }
+    // the largest leading block that is well conditioned (the network's
+    // direction comes last and is the first to go)
+    for (int k = m; k >= 1; k--) {
+        if (small_solve(k, M, r, a)) { ndir = k; break; }
     }
     if (ndir == 0) return 0;
-    if (ndir == 3 && !hist3) {
+    if (use_d && ndir == m && m > 1) {
         // energies J = -a.r/2 of the Galerkin guesses with and without d
-        double a2[3] = {0.0, 0.0, 0.0};
-        if (small_solve(2, M, r, a2)) {
-            c->net_J2 = -0.5 * (a2[0] * r[0] + a2[1] * r[1]);
-            c->net_J3 = -0.5 * (a[0] * r[0] + a[1] * r[1] + a[2] * r[2]);
+        double a2[GUESS_DIRS];
+        if (small_solve(m - 1, M, r, a2)) {
+            double j2 = 0.0, j3 = 0.0;
+            for (int i = 0; i < m - 1; i++) j2 += a2[i] * r[i];
+            for (int i = 0; i < m; i++) j3 += a[i] * r[i];
+            c->net_J2 = -0.5 * j2;
+            c->net_J3 = -0.5 * j3;
             c->net_used = true;
         }
     }
-    hipLaunchKernelGGL(k_combine3, dim3(PB), dim3(PT), 0, c->s, n, x, (const double *)c->phi_hist[0],
-                       (const double *)c->phi_hist[1], d3, a[0], ndir > 1 ? a[1] : 0.0, ndir > 2 ? a[2] : 0.0);
+    Coefs C;
+    for (int k = 0; k < GUESS_DIRS; k++) C.a[k] = k < ndir ? a[k] : 0.0;
+    hipLaunchKernelGGL(k_combine, dim3(PB), dim3(PT), 0, c->s, n, x, D, ndir, C);
     TB_LAUNCH_CHECK(c);
     double rr, bb;
     if (shift_corner(c, P, x)) return -1;
     if (residual(c, P, false, x, b, nullptr, nullptr, rr, bb)) return -1;   // norms only
     resg = sqrt(rr) / bn;
+    return 0;
+}
+
+extern "C" int tb_set_guess_history(tb_ctx *c, int h) {
+    if (!c || h < 1 || h > GUESS_HIST) {
+        if (c) c->err = "tb_set_guess_history: h must be 1..6";
+        return -1;
+    }
+    c->guess_hist = h;
     return 0;
 }
 
@@ -1788,22 +1835,21 @@ extern "C" int tb_poisson_solve(tb_ctx *c, tb_solve_stats *st) {
             if (judge_network(c, x, b, loc.res0, res, its, tb_elapsed_ms(c))) return -1;
         }
     }
-    // history of solutions for the learned guess
+    // history of solutions (and, in learned mode, right-hand sides) for the
+    // initial guess, newest first: the pointers rotate, one copy per solve
     {
-        double *oldest = c->phi_hist[2];
-        c->phi_hist[2] = c->phi_hist[1];
-        c->phi_hist[1] = c->phi_hist[0];
+        double *oldest = c->phi_hist[GUESS_HIST - 1];
+        for (int k = GUESS_HIST - 1; k > 0; k--) c->phi_hist[k] = c->phi_hist[k - 1];
         c->phi_hist[0] = oldest;
     }
     TB_CHECK(c, hipMemcpyAsync(c->phi_hist[0], x, sizeof(double) * c->d.ntot, hipMemcpyDeviceToDevice, c->s));
-    if (c->n_hist < 3) c->n_hist++;
+    if (c->n_hist < GUESS_HIST) c->n_hist++;
     if (c->cfg.pois_mode == TB_POIS_LEARNED_IR) {
-        double *oldest = c->b_hist[2];
-        c->b_hist[2] = c->b_hist[1];
-        c->b_hist[1] = c->b_hist[0];
+        double *oldest = c->b_hist[GUESS_HIST - 1];
+        for (int k = GUESS_HIST - 1; k > 0; k--) c->b_hist[k] = c->b_hist[k - 1];
         c->b_hist[0] = oldest;
         TB_CHECK(c, hipMemcpyAsync(c->b_hist[0], b, sizeof(double) * c->d.ntot, hipMemcpyDeviceToDevice, c->s));
-        if (c->nb_hist < 3) c->nb_hist++;
+        if (c->nb_hist < GUESS_HIST) c->nb_hist++;
     } else {
         c->nb_hist = 0;
     }
