@@ -209,6 +209,15 @@ def main():
     head_sps = args.steps / t_head
     prof, st_prof = profile_path(g, c, max(2, min(args.steps, 4)))
     gate = g.net_gate() if learned else None
+    if gate is not None and gate.get("net_ms_per_call", 0) > 0:
+        # MFMA work of one inference: 2 * 27 * Cin * Cout flops per output
+        # voxel of every convolution (the head's 16 -> 1 is vector code)
+        n0 = c["nx"] * c["ny"] * c["nz"]
+        flops = 54 * (16 * 16 + 48 * 16) * n0 + 54 * (16 * 32 + 32 * 32 + 96 * 32) * n0 / 8 \
+            + 54 * (32 * 64 + 64 * 64) * n0 / 64
+        tf = flops / (gate["net_ms_per_call"] * 1e-3) / 1e12
+        gate["mfma_tflops"] = round(tf, 1)
+        gate["mfma_frac_of_bf16_dense_peak"] = round(tf / 2500.0, 4)
     mixed_err = mixed_vs_fp64(g, c, mode)
 
     def pois_summary(stats):
