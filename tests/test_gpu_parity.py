@@ -12,9 +12,9 @@ pytestmark = pytest.mark.gpu
 import oracle  # noqa: E402  (checker only)
 
 
-def _pair(name, **over):
+def _pair(name, dims=None, **over):
     from turbins_amd import Solver
-    c = W.case(name)
+    c = W.case(name) if dims is None else W.case(name, nx=dims[0], ny=dims[1], nz=dims[2])
     cfg = W.config_for(c, **over)
     g = Solver(cfg)
     o = oracle.Oracle(cfg)
@@ -147,13 +147,16 @@ def test_rhs_and_operators():
     o.close()
 
 
-@pytest.mark.parametrize("name", ["bump_small", "turbidity_small"])
+@pytest.mark.parametrize("name,dims", [("bump_small", None), ("turbidity_small", None),
+                                       ("turbidity_small", (192, 64, 40))])
 @pytest.mark.parametrize("pairs", [1, 3])
-def test_helmholtz_gs_sweeps_match_oracle(name, pairs):
+def test_helmholtz_gs_sweeps_match_oracle(name, dims, pairs):
     """The fused single-pass red-black sweep pair against the oracle's two
     in-place colour passes on the assembled rows; 65x33x17 points leave a
-    one-column x tile, a one-row y tile and a one-plane k chunk."""
-    c, cfg, g, o = _pair(name)
+    one-column x tile, a one-row y tile and a one-plane k chunk, and
+    193x65x41 has tiles whose halo lies inside the box (the kernel's
+    unchecked path) beside edge tiles."""
+    c, cfg, g, o = _pair(name, dims)
     dt = 1e-3
     NT = o.NT
     for q in range(4):
