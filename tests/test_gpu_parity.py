@@ -12,9 +12,10 @@ pytestmark = pytest.mark.gpu
 import oracle  # noqa: E402  (checker only)
 
 
-def _pair(name, dims=None, **over):
+def _pair(name, dims=None, case_over=None, **over):
     from turbins_amd import Solver
-    c = W.case(name) if dims is None else W.case(name, nx=dims[0], ny=dims[1], nz=dims[2])
+    c = W.case(name, **(case_over or {})) if dims is None else \
+        W.case(name, nx=dims[0], ny=dims[1], nz=dims[2], **(case_over or {}))
     cfg = W.config_for(c, **over)
     g = Solver(cfg)
     o = oracle.Oracle(cfg)
@@ -102,11 +103,16 @@ def _random_state(c, seed=11):
 
 
 @pytest.mark.parametrize("order", [1, 2, 3])
-@pytest.mark.parametrize("name,walls", [("bump_small", (1, 1, 1, 1, 1, 1)),
-                                        ("bump_small", (1, 1, 0, 0, 0, 0)),
-                                        ("turbidity_small", (1, 1, 1, 1, 1, 1))])
-def test_eno_conv(order, name, walls):
-    c, cfg, g, o = _pair(name, eno_order=order, wall_noslip=walls)
+@pytest.mark.parametrize("name,walls,box", [("bump_small", (1, 1, 1, 1, 1, 1), None),
+                                            ("bump_small", (1, 1, 0, 0, 0, 0), None),
+                                            ("turbidity_small", (1, 1, 1, 1, 1, 1), None),
+                                            ("bump_small", (1, 1, 1, 1, 1, 1), (12.0, 1.0, 0.9))])
+def test_eno_conv(order, name, walls, box):
+    """Power-of-two spacings (16/64, 1/32, 1/16) take the kernel's uniform
+    window form inside the box; the 12 x 1 x 0.9 box has spacings 3/16 and
+    0.05625 along x and z, which keep the general form there."""
+    over = None if box is None else dict(lx=box[0], ly=box[1], lz=box[2])
+    c, cfg, g, o = _pair(name, case_over=over, eno_order=order, wall_noslip=walls)
     st = _random_state(c)
     for k, v in st.items():
         g.upload(k, v)
@@ -119,6 +125,8 @@ def test_eno_conv(order, name, walls):
         same = np.mean(a == b)
         print(name, order, nm, "bit-identical fraction", same, "max rel", _maxrel(a, b))
         assert _maxrel(a, b) <= 1e-12
+        # same operations in the same order (no contraction): bit-identical
+        assert same == 1.0, (nm, same)
     g.close()
     o.close()
 
@@ -154,8 +162,8 @@ def test_helmholtz_gs_sweeps_match_oracle(name, dims, pairs):
     """The fused single-pass red-black sweep pair against the oracle's two
     in-place colour passes on the assembled rows; 65x33x17 points leave a
     one-column x tile, a one-row y tile and a one-plane k chunk, and
-    193x65x41 has tiles whose halo lies inside the box (the kernel's
-    unchecked path) beside edge tiles."""
+    193x65x41 has tiles whose halo lies wholly inside the box beside
+    edge tiles."""
     c, cfg, g, o = _pair(name, dims)
     dt = 1e-3
     NT = o.NT

@@ -63,6 +63,9 @@ struct LineSpec {
     double hi_scale;
     const double *pos;     // coordinates of the quantity's nodes along the axis
     long stride;           // element stride of the axis in the padded layout
+    int uni;               // 1: every gap pos[m+1]-pos[m] of [lo, hi] is h, every pos[m+1]-pos[m-1] is 2h,
+                           //    h a power of two (pow2_uniform)
+    double h, rh;          // that h and its exact reciprocal
 };
 
 // Window of 2R+1 values and positions around axis position m0 (buffer
@@ -164,9 +167,41 @@ __device__ inline double eno_eval(const double (&w)[7], const double (&p)[7], do
     return vel * dQ;
 }
 
+// eno_eval<R> (R <= 2) for a window inside [lo, hi] of a uniform axis with a
+// power-of-two spacing h: a quotient by h or 2h is the product with its exact
+// reciprocal (both are the correctly rounded a/h), and the stencil term's
+// position factor (x_i - x_k) + (x_i - x_k1) evaluates to +h (V > 0) or -h,
+// so the result is bit-identical to eno_eval<R> on the loaded positions.
+template <int R>
+__device__ inline double eno_uniform(const LineSpec &L, const double *__restrict__ f, long base, double vel) {
+    double w[5];
+#pragma unroll
+    for (int o = -R; o <= R; o++) w[2 + o] = f[base + (long)o * L.stride];
+    const double rh = L.rh;
+    double D1[4];   // D1[m + 2] = (w[m+1] - w[m]) / h, m = -R..R-1
+#pragma unroll
+    for (int m = -R; m < R; m++) D1[m + 2] = (w[2 + m + 1] - w[2 + m]) * rh;
+    bool left = vel > 0.0;
+    double dQ = left ? D1[1] : D1[2];
+    if (R > 1) {
+        const double r2h = 0.5 * rh;
+        double D2[3];   // D2[m + 1] = (D1[m] - D1[m-1]) / 2h, m = -1..1
+#pragma unroll
+        for (int m = -1; m <= 1; m++) D2[m + 1] = (D1[m + 2] - D1[m + 1]) * r2h;
+        double d2k = left ? D2[0] : D2[1];
+        double d2k1 = left ? D2[1] : D2[2];
+        double c = (fabs(d2k) <= fabs(d2k1)) ? d2k : d2k1;
+        dQ += c * (left ? L.h : -L.h);
+    }
+    return vel * dQ;
+}
+
 template <int R>
 __device__ inline double eno_line_point(const LineSpec &L, const double *__restrict__ f, long base, int m0,
                                         double vel) {
+    if constexpr (R <= 2) {
+        if (L.uni && m0 - R >= L.lo && m0 + R <= L.hi) return eno_uniform<R>(L, f, base, vel);
+    }
     double w[7], p[7];
     load_window<R>(L, f, base, m0, w, p);
     return eno_eval<R>(w, p, vel);
@@ -274,6 +309,21 @@ __global__ void __launch_bounds__(256) k_eno_conc(ConcArgs a, Tiles t) {
     st_stream(&a.cc[id], r);
 }
 
+// True when every first gap fl(p[m+1] - p[m]) of [lo, hi] equals one power
+// of two h and every second gap fl(p[m+1] - p[m-1]) equals 2h, the exact
+// divisors of eno_eval<R <= 2> in a window inside [lo, hi].
+static bool pow2_uniform(const std::vector<double> &p, int lo, int hi, double &h) {
+    if (lo < 0 || hi >= (int)p.size() || hi - lo < 2) return false;
+    h = p[lo + 1] - p[lo];
+    int e;
+    if (!(h > 0.0) || !std::isfinite(h) || std::frexp(h, &e) != 0.5 || !std::isnormal(1.0 / h)) return false;
+    for (int m = lo; m < hi; m++)
+        if (p[m + 1] - p[m] != h) return false;
+    for (int m = lo + 1; m < hi; m++)
+        if (p[m + 1] - p[m - 1] != 2.0 * h) return false;
+    return true;
+}
+
 // Line specs per quantity (the ghost rules of Velocity.c:2124-2801 and
 // Conc.c:1021-1220; wall order x0,x1,y0,y1,z0,z1).
 static QLines make_lines(tb_ctx *c, int q) {
@@ -299,6 +349,10 @@ static QLines make_lines(tb_ctx *c, int q) {
         L.wall_hi = whi[ax];
         L.pos = posq[q][ax];
         L.stride = stride[ax];
+        const int hq = (q == TB_Q_C) ? ax : 3 + 3 * q + ax;   // host copy of posq[q][ax]
+        L.uni = pow2_uniform(c->hgrid[hq], L.lo, L.hi, L.h) ? 1 : 0;
+        L.rh = L.uni ? 1.0 / L.h : 0.0;
+        if (!L.uni) L.h = 0.0;
         if (q != TB_Q_C) {
             L.sgn_lo = c->cfg.wall_noslip[2 * ax] ? -1.0 : 1.0;
             L.sgn_hi = c->cfg.wall_noslip[2 * ax + 1] ? -1.0 : 1.0;
