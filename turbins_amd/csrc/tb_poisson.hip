@@ -1535,8 +1535,11 @@ extern "C" int tb_set_mixed_inner(tb_ctx *c, double reduction_floor, int maxit) 
     return 0;
 }
 
+// rr_known: x is already pinned (corner exactly 0) and lev[0].b32 holds its
+// FP32 residual, of squared norm *rr_known; the first shift and residual are
+// skipped.
 static int mixed_ir(tb_ctx *c, const PoisCtx &P, double *x, const double *b, double bn, double rtol, int maxit,
-                    int &its, int &vcy, double &res, bool &stalled) {
+                    int &its, int &vcy, double &res, bool &stalled, const double *rr_known) {
     double *r = nullptr;   // only the FP32 copy of the residual and its norms are needed
     Level &L0 = c->lev[0];
     double rr, bb;
@@ -1544,8 +1547,12 @@ static int mixed_ir(tb_ctx *c, const PoisCtx &P, double *x, const double *b, dou
     if (tb_mg_build(c)) return -1;
     const KTiles kt0 = make_ktiles(L0.nx, L0.ny, L0.nz);
     if (kt0.per_xcd * NXCD > RED_MAX_BLOCKS) { c->err = "grid too large for the reduction buffer"; return -1; }
-    if (shift_corner(c, P, x)) return -1;
-    if (residual(c, P, true, x, b, r, L0.b32, rr, bb)) return -1;
+    if (rr_known) {
+        rr = *rr_known;
+    } else {
+        if (shift_corner(c, P, x)) return -1;
+        if (residual(c, P, true, x, b, r, L0.b32, rr, bb)) return -1;
+    }
     res = sqrt(rr) / bn;
     while (res > rtol && its < maxit) {
         const double prev = res;
@@ -1608,13 +1615,24 @@ __global__ void __launch_bounds__(PT) k_subspace_dots(long n, Dirs D, const doub
         for (int k = 0; k < NS; k++) part[(long)blockIdx.x * NS + k] = a[k];
 }
 
-// out[k] = sum over blocks of part[blk * ncol + k], fixed order
-__global__ void k_sum_cols(const double *part, int nblk, int ncol, double *out) {
-    const int k = threadIdx.x;
-    if (k >= ncol) return;
+// out[k] = sum over blocks of part[blk * ncol + k]: one workgroup per
+// column, each thread a fixed stride of blocks, then a fixed-order tree in
+// LDS, so the sums do not depend on scheduling
+constexpr int SUM_COLS_THREADS = 256;
+__global__ void __launch_bounds__(SUM_COLS_THREADS) k_sum_cols(const double *part, int nblk, int ncol,
+                                                                double *out) {
+    __shared__ double sh[SUM_COLS_THREADS];
+    const int k = blockIdx.x;
     double s = 0.0;
-    for (int bI = 0; bI < nblk; bI++) s += part[(long)bI * ncol + k];
-    out[k] = s;
+    for (int bI = threadIdx.x; bI < nblk; bI += SUM_COLS_THREADS) s += part[(long)bI * ncol + k];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+#pragma unroll
+    for (int w = SUM_COLS_THREADS / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[k] = sh[0];
 }
 
 struct Coefs {
@@ -1660,7 +1678,10 @@ static void launch_dots(tb_ctx *c, long n, const Dirs &D, const double *b) {
     hipLaunchKernelGGL(k_subspace_dots<M>, dim3(PB), dim3(PT), 0, c->s, n, D, b, c->red);
 }
 
-static int learned_guess(tb_ctx *c, const PoisCtx &P, double *x, const double *b, double bn, double &resg, int &ndir) {
+// On return with ndir > 0, x is pinned and lev[0].b32 holds the FP32 copy of
+// its residual, whose squared norm is rrg.
+static int learned_guess(tb_ctx *c, const PoisCtx &P, double *x, const double *b, double bn, double &resg, double &rrg,
+                         int &ndir) {
     const long n = c->d.ntot;
     double *dd = c->kv[5], *q = c->kv[6];
     ndir = 0;
@@ -1696,7 +1717,8 @@ static int learned_guess(tb_ctx *c, const PoisCtx &P, double *x, const double *b
     default: launch_dots<7>(c, n, D, b); break;
     }
     const int ns = m * m + m;
-    hipLaunchKernelGGL(k_sum_cols, dim3(1), dim3(64), 0, c->s, (const double *)c->red, PB, ns, c->scal + S_SUB);
+    hipLaunchKernelGGL(k_sum_cols, dim3(ns), dim3(SUM_COLS_THREADS), 0, c->s, (const double *)c->red, PB, ns,
+                       c->scal + S_SUB);
     TB_LAUNCH_CHECK(c);
     if (read_s(c, S_SUB, ns)) return -1;
     const double *h = c->hscal + S_SUB;
@@ -1727,10 +1749,18 @@ static int learned_guess(tb_ctx *c, const PoisCtx &P, double *x, const double *b
     for (int k = 0; k < GUESS_DIRS; k++) C.a[k] = k < ndir ? a[k] : 0.0;
     hipLaunchKernelGGL(k_combine, dim3(PB), dim3(PT), 0, c->s, n, x, D, ndir, C);
     TB_LAUNCH_CHECK(c);
-    double rr, bb;
-    if (shift_corner(c, P, x)) return -1;
-    if (residual(c, P, false, x, b, nullptr, nullptr, rr, bb)) return -1;   // norms only
-    resg = sqrt(rr) / bn;
+    double bb;
+    // Every stored solution is pinned (each solve ends with shift_corner:
+    // corner exactly 0, half-cell planes 0), so a combination of them is
+    // too and the shift would subtract exactly 0; only the network's
+    // direction needs it.
+    if (use_d && ndir == m && shift_corner(c, P, x)) return -1;
+    // With x[corner] = 0 the pinned and the Neumann residual are the same
+    // vector (the rows differ only in the corner's own coefficient): one
+    // pass gives the guess's residual norm and the FP32 residual the mixed
+    // solve starts from.
+    if (residual(c, P, true, x, b, nullptr, c->lev[0].b32, rrg, bb)) return -1;
+    resg = sqrt(rrg) / bn;
     return 0;
 }
 
@@ -1806,12 +1836,15 @@ extern "C" int tb_poisson_solve(tb_ctx *c, tb_solve_stats *st) {
     } else {
         loc.res0 = sqrt(rr) / bn;
         int mode = c->cfg.pois_mode;
+        bool guess_resid = false;   // lev[0].b32 holds the residual of the accepted guess
+        double rr_guess = 0.0;
         if (mode == TB_POIS_LEARNED_IR && c->n_hist >= 2 && c->nb_hist >= 2) {
             double resg = 0.0;
             int ndir = 0;
-            if (learned_guess(c, P, x, b, bn, resg, ndir)) return -1;
+            if (learned_guess(c, P, x, b, bn, resg, rr_guess, ndir)) return -1;
             if (ndir > 0 && resg < loc.res0) {
                 loc.res0 = resg;
+                guess_resid = true;
             } else {
                 // keep the previous solution (still in phi_hist[0])
                 TB_CHECK(c, hipMemcpyAsync(x, c->phi_hist[0], sizeof(double) * c->d.ntot, hipMemcpyDeviceToDevice, c->s));
@@ -1822,7 +1855,8 @@ extern "C" int tb_poisson_solve(tb_ctx *c, tb_solve_stats *st) {
             if (pcg64(c, P, x, b, bn, rtol, maxit, its, vcy, res)) return -1;
         } else {
             bool stalled = false;
-            if (mixed_ir(c, P, x, b, bn, rtol, maxit, its, vcy, res, stalled)) return -1;
+            if (mixed_ir(c, P, x, b, bn, rtol, maxit, its, vcy, res, stalled, guess_resid ? &rr_guess : nullptr))
+                return -1;
             if (res > rtol) {
                 loc.fallback |= 1;   // plain FP64 GMG-PCG takes over
                 if (pcg64(c, P, x, b, bn, rtol, maxit, its, vcy, res)) return -1;
