@@ -257,6 +257,7 @@ struct tb_ctx {
     // Poisson history for the learned guess
     double *phi_hist[GUESS_HIST] = {};   // last Poisson solutions, newest first
     int n_hist = 0;
+    bool phi_pinned = false;   // the phi field is the output of the last Poisson solve (corner exactly 0)
     double *b_hist[GUESS_HIST] = {};     // their right-hand sides (learned mode only)
     int guess_hist = GUESS_HIST; // solutions in the Galerkin initial guess (tb_set_guess_history)
     int nb_hist = 0;
@@ -328,3 +329,8 @@ int tb_vcycle32(tb_ctx *c);
 int tb_net_predict(tb_ctx *c, double *out, int correction_only);
 int tb_net_record_one(tb_ctx *c);
 double tb_elapsed_ms(tb_ctx *c);
+
+// every entry point that writes a caller-chosen field reports it here
+inline void tb_field_written(tb_ctx *c, int f) {
+    if (f == TB_F_PHI) c->phi_pinned = false;
+}

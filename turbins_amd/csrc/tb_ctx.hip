@@ -176,6 +176,7 @@ extern "C" int tb_upload_field(tb_ctx *c, int f, const double *host) {
                                  sizeof(double) * d.NX, (size_t)d.NY * d.NZ, hipMemcpyHostToDevice, c->s));
     TB_CHECK(c, hipStreamSynchronize(c->s));
     if (f == TB_F_PHI) c->n_hist = c->nb_hist = 0;
+    tb_field_written(c, f);
     return 0;
 }
 

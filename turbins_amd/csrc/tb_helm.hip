@@ -685,6 +685,7 @@ extern "C" int tb_helmholtz_gs_sweeps(tb_ctx *c, int q, double dt, int f_b, int 
         if (c) c->err = "tb_helmholtz_gs_sweeps: bad argument";
         return -1;
     }
+    tb_field_written(c, f_x);
     HelmOp H = make_op(c, q, dt);
     double *cur = c->f[f_x], *alt = c->kv[6];
     if (gs_pairs(c, H, cur, alt, c->f[f_b], pairs)) return -1;
@@ -698,6 +699,7 @@ extern "C" int tb_helmholtz_apply(tb_ctx *c, int q, double dt, int f_in, int f_o
         if (c) c->err = "tb_helmholtz_apply: bad argument";
         return -1;
     }
+    tb_field_written(c, f_out);
     HelmOp H = make_op(c, q, dt);
     if (helm_apply(c, H, c->f[f_in], c->f[f_out], false, 0) < 0) return -1;
     TB_CHECK(c, hipStreamSynchronize(c->s));

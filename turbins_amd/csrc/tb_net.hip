@@ -1202,6 +1202,7 @@ extern "C" int tb_net_predict_field(tb_ctx *c, int f_out) {
     if (!c || f_out < 0 || f_out >= TB_F_COUNT) return -1;
     if (!c->net) { c->err = "no network attached"; return -1; }
     if (c->n_hist < 2) { c->err = "need two solved pressures"; return -1; }
+    tb_field_written(c, f_out);
     TB_CHECK(c, hipMemsetAsync(c->f[f_out], 0, sizeof(double) * c->d.ntot, c->s));
     if (tb_net_predict(c, c->f[f_out], 0)) return -1;
     TB_CHECK(c, hipStreamSynchronize(c->s));

@@ -155,6 +155,7 @@ extern "C" int tb_poisson_apply(tb_ctx *c, int f_in, int f_out) {
         if (c) c->err = "tb_poisson_apply: bad argument";
         return -1;
     }
+    tb_field_written(c, f_out);
     KTiles t = make_ktiles(c->d.NX, c->d.NY, c->d.NZ);
     hipLaunchKernelGGL(k_pois_km<0>, ktiles_grid(t), tiles_block(), 0, c->s, make_pois(c, 0), (const double *)c->f[f_in],
                        (const double *)nullptr, c->f[f_out], (float *)nullptr, c->red, t);
@@ -1343,15 +1344,21 @@ static int residual(tb_ctx *c, const PoisCtx &P, bool neumann, const double *x, 
 }
 
 // FP64 GMG-preconditioned CG from the current x; returns iterations
+// rr_known: x is already pinned and kv[0] holds its residual, of squared norm
+// *rr_known; the first shift and residual are skipped.
 static int pcg64(tb_ctx *c, const PoisCtx &P, double *x, const double *b, double bn, double rtol, int maxit,
-                 int &its, int &vcy, double &res) {
+                 int &its, int &vcy, double &res, const double *rr_known) {
     long n = c->d.ntot;
     double *r = c->kv[0], *z = c->kv[1], *p = c->kv[2], *q = c->kv[3];
     double rr, bb;
     // the V-cycle writes interior cells only; keep z's half-cell planes at 0
     TB_CHECK(c, hipMemsetAsync(z, 0, sizeof(double) * n, c->s));
-    if (shift_corner(c, P, x)) return -1;
-    if (residual(c, P, false, x, b, r, nullptr, rr, bb)) return -1;
+    if (rr_known) {
+        rr = *rr_known;
+    } else {
+        if (shift_corner(c, P, x)) return -1;
+        if (residual(c, P, false, x, b, r, nullptr, rr, bb)) return -1;
+    }
     res = sqrt(rr) / bn;
     while (res > rtol && its < maxit) {
         if (tb_vcycle64(c, r, z)) return -1;
@@ -1823,8 +1830,19 @@ extern "C" int tb_poisson_solve(tb_ctx *c, tb_solve_stats *st) {
     int maxit = c->cfg.pois_maxit;
     TB_CHECK(c, hipEventRecord(c->ev0, c->s));
     double rr, bb;
-    if (shift_corner(c, P, x)) return -1;
-    if (residual(c, P, false, x, b, nullptr, nullptr, rr, bb)) return -1;   // norms only
+    // the field left by the previous solve is pinned already (the shift
+    // would subtract exactly 0)
+    if (!c->phi_pinned && shift_corner(c, P, x)) return -1;
+    c->phi_pinned = false;
+    // this residual is also the first one of the method that follows: FP64
+    // PCG takes r (kv[0]), the mixed solve its FP32 copy (x is pinned, so the
+    // Neumann and the pinned residual are the same vector); the learned mode
+    // replaces x by its guess and needs the norms only
+    const int mode = c->cfg.pois_mode;
+    double *r_top = (mode == TB_POIS_FP64_PCG) ? c->kv[0] : nullptr;
+    float *r32_top = (mode == TB_POIS_MIXED_IR) ? c->lev[0].b32 : nullptr;
+    if (residual(c, P, mode == TB_POIS_MIXED_IR, x, b, r_top, r32_top, rr, bb)) return -1;
+    const double rr_top = rr;
     double bn = sqrt(bb);
     tb_solve_stats loc;
     memset(&loc, 0, sizeof(loc));
@@ -1835,7 +1853,6 @@ extern "C" int tb_poisson_solve(tb_ctx *c, tb_solve_stats *st) {
         loc.converged = 1;
     } else {
         loc.res0 = sqrt(rr) / bn;
-        int mode = c->cfg.pois_mode;
         bool guess_resid = false;   // lev[0].b32 holds the residual of the accepted guess
         double rr_guess = 0.0;
         if (mode == TB_POIS_LEARNED_IR && c->n_hist >= 2 && c->nb_hist >= 2) {
@@ -1852,14 +1869,14 @@ extern "C" int tb_poisson_solve(tb_ctx *c, tb_solve_stats *st) {
             }
         }
         if (mode == TB_POIS_FP64_PCG) {
-            if (pcg64(c, P, x, b, bn, rtol, maxit, its, vcy, res)) return -1;
+            if (pcg64(c, P, x, b, bn, rtol, maxit, its, vcy, res, &rr_top)) return -1;
         } else {
             bool stalled = false;
-            if (mixed_ir(c, P, x, b, bn, rtol, maxit, its, vcy, res, stalled, guess_resid ? &rr_guess : nullptr))
-                return -1;
+            const double *rr_known = guess_resid ? &rr_guess : (mode == TB_POIS_MIXED_IR ? &rr_top : nullptr);
+            if (mixed_ir(c, P, x, b, bn, rtol, maxit, its, vcy, res, stalled, rr_known)) return -1;
             if (res > rtol) {
                 loc.fallback |= 1;   // plain FP64 GMG-PCG takes over
-                if (pcg64(c, P, x, b, bn, rtol, maxit, its, vcy, res)) return -1;
+                if (pcg64(c, P, x, b, bn, rtol, maxit, its, vcy, res, nullptr)) return -1;
             }
         }
         loc.converged = res <= rtol;
@@ -1869,15 +1886,20 @@ extern "C" int tb_poisson_solve(tb_ctx *c, tb_solve_stats *st) {
             if (judge_network(c, x, b, loc.res0, res, its, tb_elapsed_ms(c))) return -1;
         }
     }
+    // every path above leaves x pinned (shift_corner last, or all zero)
+    c->phi_pinned = true;
     // history of solutions (and, in learned mode, right-hand sides) for the
-    // initial guess, newest first: the pointers rotate, one copy per solve
-    {
+    // initial guess and the network's inputs, newest first: the pointers
+    // rotate, one copy per solve; kept only where something reads it
+    if (c->cfg.pois_mode == TB_POIS_LEARNED_IR || c->net || c->net_record) {
         double *oldest = c->phi_hist[GUESS_HIST - 1];
         for (int k = GUESS_HIST - 1; k > 0; k--) c->phi_hist[k] = c->phi_hist[k - 1];
         c->phi_hist[0] = oldest;
+        TB_CHECK(c, hipMemcpyAsync(c->phi_hist[0], x, sizeof(double) * c->d.ntot, hipMemcpyDeviceToDevice, c->s));
+        if (c->n_hist < GUESS_HIST) c->n_hist++;
+    } else {
+        c->n_hist = 0;
     }
-    TB_CHECK(c, hipMemcpyAsync(c->phi_hist[0], x, sizeof(double) * c->d.ntot, hipMemcpyDeviceToDevice, c->s));
-    if (c->n_hist < GUESS_HIST) c->n_hist++;
     if (c->cfg.pois_mode == TB_POIS_LEARNED_IR) {
         double *oldest = c->b_hist[GUESS_HIST - 1];
         for (int k = GUESS_HIST - 1; k > 0; k--) c->b_hist[k] = c->b_hist[k - 1];
