@@ -342,11 +342,13 @@ __device__ inline double gs_value(const HelmOp &H, const AxisGS &X, const AxisGS
 // once (the in-place form streams both arrays twice).
 //
 // A 512-thread block owns a 64 x 16 column tile and a chunk of GS_TK planes
-// and walks k upwards through two LDS rings of three planes: xo with a
-// two-point halo, and the red-pass values (black positions holding xo) with a
-// one-point halo.  Per plane m: red pass on plane m (tile + halo, recomputed
-// by the neighbouring tiles); black pass on plane m-1 (tile); store plane
-// m-1.  The chunk recomputes the red pass on one plane below and above.  The
+// and walks k upwards through two LDS rings: four planes of xo with a
+// two-point halo, and three of the red-pass values (black positions holding
+// xo) with a one-point halo.  Per plane m, with two barriers: store plane m-2
+// (both of its passes are done) and red pass on plane m (tile + halo,
+// recomputed by the neighbouring tiles); black pass on plane m-1 (tile).
+// The fourth xo plane lets the next plane's staging overlap the black pass.
+// The chunk recomputes the red pass on one plane below and above.  The
 // xo plane, statuses and right-hand sides of the next plane are fetched into
 // registers one step ahead, so their latency overlaps the passes; row
 // coefficients come from per-axis tables built in LDS once per block.
@@ -354,6 +356,7 @@ constexpr int GS_TX = 64, GS_TY = 16, GS_TK = 16;
 constexpr int GS_OX = GS_TX + 4, GS_OY = GS_TY + 4;   // xo ring: halo 2
 constexpr int GS_HX = GS_TX + 2, GS_HY = GS_TY + 2;   // red-pass ring: halo 1
 constexpr int GS_OZ = GS_TK + 4;                      // z table: k0-2 .. k0+GS_TK+1
+constexpr int GS_ORING = 4;                           // xo planes staged: m-1, m, m+1 and the next
 constexpr int GS_THREADS = 512;
 constexpr int GS_OLD_PER = (GS_OY * GS_OX + GS_THREADS - 1) / GS_THREADS;
 constexpr int GS_RED = GS_HY * (GS_HX / 2);           // red points of one halo plane
@@ -384,7 +387,7 @@ static GsTiles make_gs_tiles(const Dims &d) {
 __global__ void __launch_bounds__(GS_THREADS) k_helm_gs_pair(HelmOp H, const double *__restrict__ xo,
                                                              double *__restrict__ xn, const double *__restrict__ b,
                                                              GsTiles T) {
-    __shared__ double so[3][GS_OY * GS_OX];
+    __shared__ double so[GS_ORING][GS_OY * GS_OX];
     __shared__ double sr[3][GS_HY * GS_HX];
     __shared__ AxisGS cx[GS_OX], cy[GS_OY], cz[GS_OZ];
     const Dims &d = H.d;
@@ -400,7 +403,7 @@ __global__ void __launch_bounds__(GS_THREADS) k_helm_gs_pair(HelmOp H, const dou
     const int t = threadIdx.x;
     const uint8_t *__restrict__ st = H.st;
 
-    for (int e = t; e < 3 * GS_OY * GS_OX; e += GS_THREADS) (&so[0][0])[e] = 0.0;
+    for (int e = t; e < GS_ORING * GS_OY * GS_OX; e += GS_THREADS) (&so[0][0])[e] = 0.0;
     for (int e = t; e < 3 * GS_HY * GS_HX; e += GS_THREADS) (&sr[0][0])[e] = 0.0;
     for (int e = t; e < GS_OX + GS_OY + GS_OZ; e += GS_THREADS) {
         if (e < GS_OX) {
@@ -432,7 +435,7 @@ __global__ void __launch_bounds__(GS_THREADS) k_helm_gs_pair(HelmOp H, const dou
         for (int u = 0; u < GS_OLD_PER; u++) ro[u] = ooff[u] >= 0 ? xo[m * plane + ooff[u]] : 0.0;
     };
     auto put_old = [&](int m) {
-        double *S = so[m % 3];
+        double *S = so[m % GS_ORING];
 #pragma unroll
         for (int u = 0; u < GS_OLD_PER; u++) {
             const int e = t + u * GS_THREADS;
@@ -482,6 +485,17 @@ __global__ void __launch_bounds__(GS_THREADS) k_helm_gs_pair(HelmOp H, const dou
         }
     };
 
+    // both colours of plane k, from its sr slot to xn (coalesced rows)
+    auto write_plane = [&](int k) {
+        if (k < k0 || k >= k1) return;
+        const double *R = sr[k % 3];
+        for (int e = t; e < GS_TY * GS_TX; e += GS_THREADS) {
+            const int r = e / GS_TX, cc = e - r * GS_TX;
+            const int i = i0 + cc, j = j0 + r;
+            if (i < NX && j < NY) st_stream(&xn[k * plane + j * px + i], R[(r + 1) * GS_HX + cc + 1]);
+        }
+    };
+
     if (k0 >= 2) { fetch_old(k0 - 2); put_old(k0 - 2); }
     if (k0 >= 1) { fetch_old(k0 - 1); put_old(k0 - 1); }
     fetch_old(k0);
@@ -498,9 +512,13 @@ __global__ void __launch_bounds__(GS_THREADS) k_helm_gs_pair(HelmOp H, const dou
         double bb_n = 0.0;
         if (m >= k0 && m < k1) fetch_black(m, bs_n, bb_n);
         __syncthreads();
+        // plane m - 2 is final (its black pass ran in the previous
+        // iteration) and its sr slot is next written by the red pass of m + 1
+        write_plane(m - 2);
         if (m >= 0 && m < NZ) {
             // red pass on plane m over the tile + halo 1
-            const double *S = so[m % 3], *Sb = so[(m + 2) % 3], *Sf = so[(m + 1) % 3];
+            const double *S = so[m % GS_ORING], *Sb = so[(m + GS_ORING - 1) % GS_ORING],
+                         *Sf = so[(m + 1) % GS_ORING];
             double *R = sr[m % 3];
             const AxisGS Z = cz[m - k0 + 2];
 #pragma unroll
@@ -522,23 +540,20 @@ __global__ void __launch_bounds__(GS_THREADS) k_helm_gs_pair(HelmOp H, const dou
             const double *Rb = sr[(k + 2) % 3], *Rf = sr[(k + 1) % 3];
             const int cc = 2 * bc2 + ((1 + bj + k) & 1);
             const int h = (br + 1) * GS_HX + cc + 1;
-            R[h] = gs_value(H, cx[cc + 2], cy[br + 2], cz[k - k0 + 2], bs, bbv, so[k % 3][(br + 2) * GS_OX + cc + 2],
-                            Rb[h], R[h - GS_HX], R[h - 1], R[h + 1], R[h + GS_HX], Rf[h]);
+            R[h] = gs_value(H, cx[cc + 2], cy[br + 2], cz[k - k0 + 2], bs, bbv,
+                            so[k % GS_ORING][(br + 2) * GS_OX + cc + 2], Rb[h], R[h - GS_HX], R[h - 1], R[h + 1],
+                            R[h + GS_HX], Rf[h]);
         }
-        __syncthreads();
-        if (k >= k0 && k < k1) {
-            const double *R = sr[k % 3];
-            for (int e = t; e < GS_TY * GS_TX; e += GS_THREADS) {
-                const int r = e / GS_TX, cc = e - r * GS_TX;
-                const int i = i0 + cc, j = j0 + r;
-                if (i < NX && j < NY) st_stream(&xn[k * plane + j * px + i], R[(r + 1) * GS_HX + cc + 1]);
-            }
-        }
+        // no barrier here: the next iteration's staging writes the xo slot of
+        // plane m + 2, which no pass of this iteration reads (GS_ORING = 4),
+        // and its first barrier precedes every read of this black pass
 #pragma unroll
         for (int u = 0; u < GS_RED_PER; u++) { rs[u] = rs_n[u]; rb[u] = rb_n[u]; }
         bs = bs_n;
         bbv = bb_n;
     }
+    __syncthreads();
+    write_plane(k1 - 1);
 }
 
 // IMMERSED rows, Jacobi over the IB table: new value of row a into tmp[a]
