@@ -123,7 +123,7 @@ def roofline(prof, stats, c):
         elif name == "RBGS0":
             total = 24 * ncell * n      # FP64 fused pair: x, b read, x written
         elif name == "RBGS0_F32":
-            total = 8 * ncell * n
+            total = 12 * ncell * n      # FP32 fused pair: x, b read, x written
         else:
             continue
         if best is None or ms > best[1]:

@@ -525,17 +525,19 @@ __global__ void __launch_bounds__(256) k_prolong(LV<T> F, LV<T> C, int cx, int c
 // (same expression, rb_value) while xo and b cross HBM once.  ZERO: xo is
 // the zero initial guess and is not read.  The block layout is the one of
 // k_helm_gs_pair: a 64 x 8 column tile walking PAIR_TK planes through LDS
-// rings of xo (halo 2) and first-colour values (halo 1), the next plane's
-// xo and b fetched into registers one plane ahead.
+// rings of three planes of xo (halo 2) and of first-colour values (halo 1),
+// the next plane's xo and b fetched into registers one plane ahead, two
+// barriers per plane (plane m-2 is stored after the first).
 constexpr int PAIR_TX = 64, PAIR_TY = 8, PAIR_TK = 16;
 constexpr int PAIR_OX = PAIR_TX + 4, PAIR_OY = PAIR_TY + 4;
 constexpr int PAIR_HX = PAIR_TX + 2, PAIR_HY = PAIR_TY + 2;
 constexpr int PAIR_THREADS = 256;
+constexpr int PAIR_ORING = 3;   // the second-colour pass reads no xo, so staging may overwrite plane m-1
 constexpr int PAIR_OLD_PER = (PAIR_OY * PAIR_OX + PAIR_THREADS - 1) / PAIR_THREADS;
 constexpr int PAIR_RED = PAIR_HY * (PAIR_HX / 2);
 constexpr int PAIR_RED_PER = (PAIR_RED + PAIR_THREADS - 1) / PAIR_THREADS;
 static_assert(PAIR_TX * PAIR_TY == 2 * PAIR_THREADS, "one second-colour cell per thread");
-// FP64 levels at least this large use the fused pair (smaller ones have too
+// Levels at least this large use the fused pair (smaller ones have too
 // few tiles to hide the plane-by-plane chain of a block)
 constexpr long PAIR_LEVEL_CELLS = 8L << 20;
 
@@ -557,7 +559,7 @@ static PairTiles make_pair_tiles(int nx, int ny, int nz) {
 template <typename T, int F, bool ZERO>
 __global__ void __launch_bounds__(PAIR_THREADS) k_rbgs_pair(LV<T> L, const T *__restrict__ xo, T *__restrict__ xn,
                                                             const T *__restrict__ b, PairTiles P) {
-    __shared__ T so[3][PAIR_OY * PAIR_OX];
+    __shared__ T so[PAIR_ORING][PAIR_OY * PAIR_OX];
     __shared__ T sr[3][PAIR_HY * PAIR_HX];
     const int nx = L.nx, ny = L.ny, nz = L.nz, px = L.px;
     const int plane = L.px * L.py;
@@ -570,7 +572,7 @@ __global__ void __launch_bounds__(PAIR_THREADS) k_rbgs_pair(LV<T> L, const T *__
     const int k1 = k0 + PAIR_TK < nz ? k0 + PAIR_TK : nz;
     const int t = threadIdx.x;
 
-    for (int e = t; e < 3 * PAIR_OY * PAIR_OX; e += PAIR_THREADS) (&so[0][0])[e] = (T)0;
+    for (int e = t; e < PAIR_ORING * PAIR_OY * PAIR_OX; e += PAIR_THREADS) (&so[0][0])[e] = (T)0;
     for (int e = t; e < 3 * PAIR_HY * PAIR_HX; e += PAIR_THREADS) (&sr[0][0])[e] = (T)0;
     __syncthreads();
 
@@ -588,7 +590,7 @@ __global__ void __launch_bounds__(PAIR_THREADS) k_rbgs_pair(LV<T> L, const T *__
         for (int u = 0; u < PAIR_OLD_PER; u++) ro[u] = ooff[u] >= 0 ? xo[m * plane + ooff[u]] : (T)0;
     };
     auto put_old = [&](int m) {
-        T *S = so[m % 3];
+        T *S = so[m % PAIR_ORING];
 #pragma unroll
         for (int u = 0; u < PAIR_OLD_PER; u++) {
             const int e = t + u * PAIR_THREADS;
@@ -624,6 +626,17 @@ __global__ void __launch_bounds__(PAIR_THREADS) k_rbgs_pair(LV<T> L, const T *__
         return (i < nx && bj < ny) ? b[k * plane + bj * px + i] : (T)0;
     };
 
+    // both colours of plane k, from its sr slot to xn (coalesced rows)
+    auto write_plane = [&](int k) {
+        if (k < k0 || k >= k1) return;
+        const T *R = sr[k % 3];
+        for (int e = t; e < PAIR_TY * PAIR_TX; e += PAIR_THREADS) {
+            const int r = e / PAIR_TX, c2 = e - r * PAIR_TX;
+            const int ii = i0 + c2, jj = j0 + r;
+            if (ii < nx && jj < ny) xn[k * plane + jj * px + ii] = R[(r + 1) * PAIR_HX + c2 + 1];
+        }
+    };
+
     if (!ZERO) {
         if (k0 >= 2) { fetch_old(k0 - 2); put_old(k0 - 2); }
         if (k0 >= 1) { fetch_old(k0 - 1); put_old(k0 - 1); }
@@ -642,8 +655,10 @@ __global__ void __launch_bounds__(PAIR_THREADS) k_rbgs_pair(LV<T> L, const T *__
         if (m + 1 < nz) fetch_red(m + 1, rb_n, rv_n);
         const T bb_n = (m >= k0 && m < k1) ? fetch_black(m) : (T)0;
         __syncthreads();
+        write_plane(m - 2);   // final; its sr slot is next written by the first-colour pass of m + 1
         if (m >= 0 && m < nz) {
-            const T *S = so[m % 3], *Sb = so[(m + 2) % 3], *Sf = so[(m + 1) % 3];
+            const T *S = so[m % PAIR_ORING], *Sb = so[(m + PAIR_ORING - 1) % PAIR_ORING],
+                    *Sf = so[(m + 1) % PAIR_ORING];
             T *R = sr[m % 3];
 #pragma unroll
             for (int u = 0; u < PAIR_RED_PER; u++) {
@@ -667,19 +682,14 @@ __global__ void __launch_bounds__(PAIR_THREADS) k_rbgs_pair(LV<T> L, const T *__
             const int h = (br + 1) * PAIR_HX + cc + 1;
             R[h] = rb_value(L, i, bj, k, bb, R[h - 1], R[h + 1], R[h - PAIR_HX], R[h + PAIR_HX], Rb[h], Rf[h]);
         }
-        __syncthreads();
-        if (k >= k0 && k < k1) {
-            const T *R = sr[k % 3];
-            for (int e = t; e < PAIR_TY * PAIR_TX; e += PAIR_THREADS) {
-                const int r = e / PAIR_TX, c2 = e - r * PAIR_TX;
-                const int ii = i0 + c2, jj = j0 + r;
-                if (ii < nx && jj < ny) xn[k * plane + jj * px + ii] = R[(r + 1) * PAIR_HX + c2 + 1];
-            }
-        }
+        // no barrier: the next staging writes the xo slot of plane m - 1,
+        // which the second-colour pass does not read
 #pragma unroll
         for (int u = 0; u < PAIR_RED_PER; u++) { rb[u] = rb_n[u]; rv[u] = rv_n[u]; }
         bb = bb_n;
     }
+    __syncthreads();
+    write_plane(k1 - 1);
 }
 
 template <typename T>
@@ -921,12 +931,15 @@ int tb_mg_build(tb_ctx *c) {
             TB_CHECK(c, hipMemsetAsync(L.r64, 0, sizeof(double) * L.ntot, c->s));
             c->bytes += 3 * sizeof(double) * L.ntot;
         }
-        // FP64 only: the FP32 arrays of a level stay resident in the 256 MB
-        // Infinity Cache, where two half sweeps cost no more than a pair
+        // second iterate buffers of the fused sweep pairs (both precisions;
+        // with two barriers per plane the FP32 pair beats two half sweeps
+        // even though those find their arrays in the 256 MB Infinity Cache)
         if ((long)L.nx * L.ny * L.nz >= PAIR_LEVEL_CELLS && L.ntot < (1L << 31)) {
             TB_CHECK(c, hipMalloc((void **)&L.a64, sizeof(double) * L.ntot));
             TB_CHECK(c, hipMemsetAsync(L.a64, 0, sizeof(double) * L.ntot, c->s));
-            c->bytes += sizeof(double) * L.ntot;
+            TB_CHECK(c, hipMalloc((void **)&L.a32, sizeof(float) * L.ntot));
+            TB_CHECK(c, hipMemsetAsync(L.a32, 0, sizeof(float) * L.ntot, c->s));
+            c->bytes += (sizeof(double) + sizeof(float)) * L.ntot;
         }
         TB_CHECK(c, hipMalloc((void **)&L.x32, sizeof(float) * L.ntot));
         TB_CHECK(c, hipMalloc((void **)&L.b32, sizeof(float) * L.ntot));
