@@ -369,11 +369,14 @@ struct GsTiles {
     int n, per_xcd;
 };
 
+// The tiles cover the box without its last plane along each axis (see
+// k_gs_last_planes), so 512 x 256 x 128 cells take exactly 8 x 16 x 8 tiles
+// instead of 9 x 17 x 9 with nearly empty ones walking the full plane chain.
 static GsTiles make_gs_tiles(const Dims &d) {
     GsTiles t;
-    t.tx = (d.NX + GS_TX - 1) / GS_TX;
-    t.ty = (d.NY + GS_TY - 1) / GS_TY;
-    t.kc = (d.NZ + GS_TK - 1) / GS_TK;
+    t.tx = (d.NX - 1 + GS_TX - 1) / GS_TX;
+    t.ty = (d.NY - 1 + GS_TY - 1) / GS_TY;
+    t.kc = (d.NZ - 1 + GS_TK - 1) / GS_TK;
     t.n = t.tx * t.ty * t.kc;
     t.per_xcd = (t.n + NXCD - 1) / NXCD;
     return t;
@@ -399,7 +402,7 @@ __global__ void __launch_bounds__(GS_THREADS) k_helm_gs_pair(HelmOp H, const dou
     const int tx = lt % T.tx, rest = lt / T.tx;
     const int i0 = tx * GS_TX, j0 = (rest % T.ty) * GS_TY;
     const int k0 = (rest / T.ty) * GS_TK;
-    const int k1 = k0 + GS_TK < NZ ? k0 + GS_TK : NZ;
+    const int k1 = k0 + GS_TK < NZ - 1 ? k0 + GS_TK : NZ - 1;   // the last plane: k_gs_last_planes
     const int t = threadIdx.x;
     const uint8_t *__restrict__ st = H.st;
 
@@ -556,6 +559,34 @@ __global__ void __launch_bounds__(GS_THREADS) k_helm_gs_pair(HelmOp H, const dou
     write_plane(k1 - 1);
 }
 
+// The last plane of each axis, outside the sweep tiles.  Its points are
+// BOUNDARY by classification: their rows read ap_solid q = b and no other
+// row couples to them (zero coefficients), so their Gauss-Seidel value
+// b / (ap_solid + 1/dt), the expression of gs_value, does not depend on x and
+// is written into both iterate buffers once per gs_pairs call.  Points where
+// two planes meet are written twice with the same value.
+__global__ void k_gs_last_planes(HelmOp H, const double *__restrict__ b, double *__restrict__ x1,
+                                 double *__restrict__ x2) {
+    const Dims &d = H.d;
+    const long n1 = (long)d.NY * d.NZ, n2 = (long)d.NX * d.NZ, n3 = (long)d.NX * d.NY;
+    long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    int i, j, k;
+    if (e < n1) {
+        i = d.NX - 1; j = (int)(e % d.NY); k = (int)(e / d.NY);
+    } else if ((e -= n1) < n2) {
+        j = d.NY - 1; i = (int)(e % d.NX); k = (int)(e / d.NX);
+    } else if ((e -= n2) < n3) {
+        k = d.NZ - 1; i = (int)(e % d.NX); j = (int)(e / d.NX);
+    } else {
+        return;
+    }
+    const long id = gidx(d, i, j, k);
+    if (H.st[id] == TB_IMMERSED) return;   // kept, as the sweep keeps IMMERSED rows
+    const double v = b[id] / (H.ap_solid + H.mod);
+    x1[id] = v;
+    x2[id] = v;
+}
+
 // IMMERSED rows, Jacobi over the IB table: new value of row a into tmp[a]
 // (x_im = (b_im - sum_{g != im} v_g x_g) / v_im), then scattered, so the
 // result does not depend on the order in which rows run
@@ -603,6 +634,12 @@ static int gs_pairs(tb_ctx *c, const HelmOp &H, double *&cur, double *&alt, cons
     if (c->d.ntot >= (1L << 31)) {
         c->err = "Helmholtz sweeps: field exceeds 2^31 elements";
         return -1;
+    }
+    if (n > 0) {
+        const Dims &d = c->d;
+        const long np = (long)d.NY * d.NZ + (long)d.NX * d.NZ + (long)d.NX * d.NY;
+        hipLaunchKernelGGL(k_gs_last_planes, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, c->s, H, b, cur, alt);
+        TB_LAUNCH_CHECK(c);
     }
     for (int sw = 0; sw < n; sw++) {
         TB_PROF(c, TB_K_HELM_GS,
