@@ -94,22 +94,34 @@ __device__ inline bool tile_point_kfast(const Tiles &t, int &i, int &j, int &k) 
 
 // Column tiles for k-marching stencils: a 64 x 4 block owns a tile of
 // columns and walks KCHUNK planes upwards, keeping the k-1, k, k+1 values of
-// its column in registers, so each plane is read from HBM once.
+// its column in registers, so each plane is read from HBM once.  Only full
+// tiles are used; the columns they leave (a point box of 513 x 257 leaves
+// column 512 and row 256) are listed 256 to a block, and the last k chunk
+// takes up to KCHUNK/2 extra planes (129 planes: 8 chunks, not 9).  Each
+// block is a chain of plane steps, so nearly empty tiles would cost as much
+// time as full ones.
 constexpr int KCHUNK = 16;
 
 struct KTiles {
     int nx, ny, nz;
-    int tx, ty, kc;     // tiles per axis, k chunks
-    long n, per_xcd;
+    int tx, ty, kc;     // full tiles per axis, k chunks
+    int fx, fy;         // extent of the full tiles
+    int ne, eb;         // columns outside the full tiles, blocks listing them (per chunk)
+    long n, per_xcd;    // blocks: (tx * ty + eb) * kc
 };
 
 inline KTiles make_ktiles(int nx, int ny, int nz) {
     KTiles t;
     t.nx = nx; t.ny = ny; t.nz = nz;
-    t.tx = (nx + TILE_X - 1) / TILE_X;
-    t.ty = (ny + TILE_Y - 1) / TILE_Y;
-    t.kc = (nz + KCHUNK - 1) / KCHUNK;
-    t.n = (long)t.tx * t.ty * t.kc;
+    t.tx = nx / TILE_X;
+    t.ty = ny / TILE_Y;
+    t.fx = t.tx * TILE_X;
+    t.fy = t.ty * TILE_Y;
+    t.ne = nx * ny - t.fx * t.fy;
+    t.eb = (t.ne + TILE_X * TILE_Y - 1) / (TILE_X * TILE_Y);
+    t.kc = (nz + KCHUNK / 2) / KCHUNK;
+    if (t.kc < 1) t.kc = 1;
+    t.n = (long)(t.tx * t.ty + t.eb) * t.kc;
     t.per_xcd = (t.n + NXCD - 1) / NXCD;
     return t;
 }
@@ -123,17 +135,37 @@ __device__ inline void ktile_column(const KTiles &t, int &i, int &j, int &k0, in
     int b = blockIdx.x;
     int per = (int)t.per_xcd;
     int lt = (b % NXCD) * per + b / NXCD;
-    if (lt >= t.n) { i = j = k0 = k1 = 0; active = false; return; }
-    int tx = lt % t.tx;
-    int r = lt / t.tx;
-    int ty = r % t.ty;
-    int kc = r / t.ty;
-    i = tx * TILE_X + threadIdx.x;
-    j = ty * TILE_Y + threadIdx.y;
+    i = j = k0 = k1 = 0;
+    active = false;
+    if (lt >= t.n) return;
+    const int nmain = t.tx * t.ty;
+    const int per_chunk = nmain + t.eb;
+    const int kc = lt / per_chunk;
+    // the listed (uncoalesced, slower) blocks of a chunk come first, so they
+    // do not form the tail of the launch
+    const int r0 = lt - kc * per_chunk;
+    const int r = r0 < t.eb ? r0 + nmain : r0 - t.eb;
     k0 = kc * KCHUNK;
-    k1 = k0 + KCHUNK < t.nz ? k0 + KCHUNK : t.nz;
-    active = i < t.nx && j < t.ny;
-    return;
+    k1 = (kc == t.kc - 1) ? t.nz : k0 + KCHUNK;
+    if (r < nmain) {
+        i = (r % t.tx) * TILE_X + threadIdx.x;
+        j = (r / t.tx) * TILE_Y + threadIdx.y;
+        active = true;
+        return;
+    }
+    // columns outside the full tiles: i >= fx (all j) first, then j >= fy (i < fx)
+    int e = (r - nmain) * (TILE_X * TILE_Y) + threadIdx.y * TILE_X + threadIdx.x;
+    if (e >= t.ne) return;
+    const int wx = t.nx - t.fx;
+    if (e < wx * t.ny) {
+        i = t.fx + e % wx;
+        j = e / wx;
+    } else {
+        e -= wx * t.ny;
+        i = e % t.fx;
+        j = t.fy + e / t.fx;
+    }
+    active = true;
 }
 
 // ---------------------------------------------------------------------------
