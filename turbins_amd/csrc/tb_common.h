@@ -266,6 +266,8 @@ struct tb_ctx {
     double *red = nullptr;       // partial sums [RED_SLOTS][RED_MAX_BLOCKS]
     double *scal = nullptr;      // device scalars
     double *hscal = nullptr;     // pinned host mirror of scal
+    unsigned long long *hseq = nullptr;   // pinned word the device writes after publishing scalars
+    unsigned long long pub_seq = 0;       // last sequence number asked for
 
     // multigrid
     int nlev = 0;
@@ -353,6 +355,8 @@ enum {
 // internal entry points shared between translation units
 int tb_geometry_run(tb_ctx *c, const double *h_xz);
 int tb_reduce_finish(tb_ctx *c, int nblocks, int nslots, int slot0);
+int tb_read_scalars(tb_ctx *c, int first, int count);
+int tb_sync(tb_ctx *c);
 int tb_dot(tb_ctx *c, const double *a, const double *b, int slot);
 int tb_mg_build(tb_ctx *c);
 void tb_mg_free(tb_ctx *c);

@@ -1,5 +1,6 @@
 // tb_ctx.hip -- context lifecycle, grid arrays, field transfers, reductions
 // and the vector sweeps of the RK step (store_old / rk_average).
+#include <chrono>
 #include <cmath>
 #include <cstring>
 
@@ -67,7 +68,11 @@ static int dev_alloc(tb_ctx *c, void **p, size_t bytes) {
 
 double tb_elapsed_ms(tb_ctx *c) {
     float ms = 0.f;
-    hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    if (hipEventElapsedTime(&ms, c->ev0, c->ev1) != hipSuccess) {
+        // the event may not yet be marked complete after a spin wait
+        hipEventSynchronize(c->ev1);
+        hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    }
     return (double)ms;
 }
 
@@ -117,6 +122,8 @@ extern "C" tb_ctx *tb_create(const tb_config *cfg) {
     ok = ok && dev_alloc(c, (void **)&c->red, sizeof(double) * RED_SLOTS * RED_MAX_BLOCKS) == 0;
     ok = ok && dev_alloc(c, (void **)&c->scal, sizeof(double) * S_COUNT) == 0;
     ok = ok && hipHostMalloc((void **)&c->hscal, sizeof(double) * S_COUNT, hipHostMallocCoherent) == hipSuccess;
+    ok = ok && hipHostMalloc((void **)&c->hseq, sizeof(unsigned long long), hipHostMallocCoherent) == hipSuccess;
+    if (ok) *c->hseq = 0;
     ok = ok && hipStreamSynchronize(c->s) == hipSuccess;
     if (!ok) {
         tb_destroy(c);
@@ -150,6 +157,7 @@ extern "C" void tb_destroy(tb_ctx *c) {
     hipFree(c->red);
     hipFree(c->scal);
     if (c->hscal) hipHostFree(c->hscal);
+    if (c->hseq) hipHostFree(c->hseq);
     for (int e = 0; e < 2; e++)
         if (c->net_ev[e]) hipEventDestroy(c->net_ev[e]);
     if (c->ev0) hipEventDestroy(c->ev0);
@@ -308,6 +316,46 @@ __global__ void k_reduce_finish(const double *__restrict__ part, double *__restr
     if (threadIdx.x == 0)
         for (int a = 0; a < nslots; a++) out[slot0 + a] = v[a];
 }
+
+// Scalar read-back without a blocking stream synchronisation: one thread
+// copies scal[first, first + count) into the pinned mirror, fences at
+// system scope and then writes the sequence number into a pinned word, on
+// which the host spins (a blocking wait wakes the host tens of microseconds
+// after the GPU is done, and the GPU idles that long at every convergence
+// test).  Every kernel queued before it has finished once the word changes.
+// A queue that stops (a failed kernel) never writes it: after one second of
+// spinning the host falls back to hipStreamSynchronize, which reports the
+// error.
+__global__ void k_publish(const double *__restrict__ scal, int first, int count, double *hscal,
+                          unsigned long long *hseq, unsigned long long seq) {
+    if (threadIdx.x != 0) return;
+    for (int a = 0; a < count; a++) hscal[first + a] = scal[first + a];
+    __threadfence_system();
+    *(volatile unsigned long long *)hseq = seq;
+}
+
+int tb_read_scalars(tb_ctx *c, int first, int count) {
+    const unsigned long long want = ++c->pub_seq;
+    hipLaunchKernelGGL(k_publish, dim3(1), dim3(64), 0, c->s, (const double *)c->scal, first, count, c->hscal, c->hseq,
+                       want);
+    TB_LAUNCH_CHECK(c);
+    const auto t0 = std::chrono::steady_clock::now();
+    for (unsigned n = 1; __atomic_load_n(c->hseq, __ATOMIC_ACQUIRE) != want; n++) {
+        if ((n & 4095) == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::seconds(1)) {
+            TB_CHECK(c, hipStreamSynchronize(c->s));
+            if (__atomic_load_n(c->hseq, __ATOMIC_ACQUIRE) != want) {
+                c->err = "scalar read-back: the device did not publish";
+                return -1;
+            }
+            break;
+        }
+        __builtin_ia32_pause();
+    }
+    return 0;
+}
+
+// wait until the context's stream has drained (spin, as tb_read_scalars)
+int tb_sync(tb_ctx *c) { return tb_read_scalars(c, 0, 0); }
 
 int tb_reduce_finish(tb_ctx *c, int nblocks, int nslots, int slot0) {
     hipLaunchKernelGGL(k_reduce_finish, dim3(1), dim3(1024), 0, c->s, c->red, c->scal, nblocks, nslots, slot0);

@@ -617,8 +617,7 @@ static int helm_residual(tb_ctx *c, const HelmOp &H, const double *x, const doub
                          double &bb) {
     int nb = helm_apply(c, H, x, ibuf, false, 3, b);
     if (nb < 0 || tb_reduce_finish(c, nb, 2, S_TMP1)) return -1;
-    TB_CHECK(c, hipMemcpyAsync(c->hscal + S_TMP1, c->scal + S_TMP1, sizeof(double) * 2, hipMemcpyDeviceToHost, c->s));
-    TB_CHECK(c, hipStreamSynchronize(c->s));
+    if (tb_read_scalars(c, S_TMP1, 2)) return -1;
     rr = c->hscal[S_TMP1];
     bb = c->hscal[S_TMP2];
     return 0;
@@ -853,11 +852,7 @@ __global__ void k_copy_slot(double *s, int dst, int src) {
     if (threadIdx.x == 0) s[dst] = s[src];
 }
 
-static int read_scalars(tb_ctx *c, int first, int count) {
-    TB_CHECK(c, hipMemcpyAsync(c->hscal + first, c->scal + first, sizeof(double) * count, hipMemcpyDeviceToHost, c->s));
-    TB_CHECK(c, hipStreamSynchronize(c->s));
-    return 0;
-}
+static int read_scalars(tb_ctx *c, int first, int count) { return tb_read_scalars(c, first, count); }
 
 // BiCGStab (van der Vorst 1992), the method of KSPIBCGS (Solver.c:39-86,
 // 269-322), with the IB equations rescaled (see k_ib_apply); nonzero initial
@@ -880,7 +875,7 @@ extern "C" int tb_helmholtz_solve(tb_ctx *c, int q, double dt, tb_solve_stats *s
         if (helm_gs_solve(c, H, x, b, rtol, maxit, gits, g0, gres, gbn, ok)) return -1;
         if (ok) {
             TB_CHECK(c, hipEventRecord(c->ev1, c->s));
-            TB_CHECK(c, hipStreamSynchronize(c->s));
+            if (tb_sync(c)) return -1;
             if (st) {
                 st->iterations = gits;
                 st->converged = 1;

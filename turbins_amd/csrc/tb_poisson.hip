@@ -1309,11 +1309,7 @@ __global__ void __launch_bounds__(256) k_add32(Dims d, double *__restrict__ x, c
     x[id] += (double)e[id];
 }
 
-static int read_s(tb_ctx *c, int first, int count) {
-    TB_CHECK(c, hipMemcpyAsync(c->hscal + first, c->scal + first, sizeof(double) * count, hipMemcpyDeviceToHost, c->s));
-    TB_CHECK(c, hipStreamSynchronize(c->s));
-    return 0;
-}
+static int read_s(tb_ctx *c, int first, int count) { return tb_read_scalars(c, first, count); }
 
 struct PoisCtx {
     Tiles all, cells;
@@ -1895,7 +1891,7 @@ extern "C" int tb_poisson_solve(tb_ctx *c, tb_solve_stats *st) {
         loc.converged = res <= rtol;
         if (c->net_used) {
             TB_CHECK(c, hipEventRecord(c->ev1, c->s));
-            TB_CHECK(c, hipStreamSynchronize(c->s));
+            if (tb_sync(c)) return -1;
             if (judge_network(c, x, b, loc.res0, res, its, tb_elapsed_ms(c))) return -1;
         }
     }
@@ -1924,7 +1920,7 @@ extern "C" int tb_poisson_solve(tb_ctx *c, tb_solve_stats *st) {
     }
     if (c->net_record && c->n_hist >= 3 && tb_net_record_one(c)) return -1;
     TB_CHECK(c, hipEventRecord(c->ev1, c->s));
-    TB_CHECK(c, hipStreamSynchronize(c->s));
+    if (tb_sync(c)) return -1;
     loc.iterations = its;
     loc.vcycles = vcy;
     loc.res = (bn == 0.0) ? 0.0 : res;
@@ -2050,21 +2046,20 @@ extern "C" int tb_div_max(tb_ctx *c, double *div_max, int32_t *ijk) {
     if (!c) return -1;
     Tiles t = make_tiles(c->d.NX, c->d.NY, c->d.NZ);
     long nb = t.per_xcd * NXCD;
-    double *pm = nullptr;
-    long *pi = nullptr;
-    TB_CHECK(c, hipMalloc((void **)&pm, sizeof(double) * (nb + 1)));
-    TB_CHECK(c, hipMalloc((void **)&pi, sizeof(long) * (nb + 1)));
+    if (nb > RED_MAX_BLOCKS) { c->err = "grid too large for the reduction buffer"; return -1; }
+    // per-block maxima and their indices in the first two partial-sum slots;
+    // the result in S_TMP0 and, as the bits of a long, S_TMP1
+    double *pm = c->red;
+    long *pi = reinterpret_cast<long *>(c->red + RED_MAX_BLOCKS);
     hipLaunchKernelGGL(k_div, tiles_grid(t), tiles_block(), 0, c->s, c->d, c->g, c->f[TB_F_U], c->f[TB_F_V],
                        c->f[TB_F_W], c->st[3], c->f[TB_F_DIV], pm, pi, 1.0 / c->dEta, 1.0 / c->dXi, 1.0 / c->dPhi, t);
-    hipLaunchKernelGGL(k_div_final, dim3(1), dim3(1024), 0, c->s, pm, pi, nb, pm + nb, pi + nb);
+    hipLaunchKernelGGL(k_div_final, dim3(1), dim3(1024), 0, c->s, pm, pi, nb, c->scal + S_TMP0,
+                       reinterpret_cast<long *>(c->scal + S_TMP1));
     TB_LAUNCH_CHECK(c);
-    double h;
+    if (tb_read_scalars(c, S_TMP0, 2)) return -1;
+    const double h = c->hscal[S_TMP0];
     long hi;
-    TB_CHECK(c, hipMemcpyAsync(&h, pm + nb, sizeof(double), hipMemcpyDeviceToHost, c->s));
-    TB_CHECK(c, hipMemcpyAsync(&hi, pi + nb, sizeof(long), hipMemcpyDeviceToHost, c->s));
-    TB_CHECK(c, hipStreamSynchronize(c->s));
-    hipFree(pm);
-    hipFree(pi);
+    memcpy(&hi, &c->hscal[S_TMP1], sizeof(long));
     if (div_max) *div_max = h;
     if (ijk) {
         ijk[0] = (int32_t)(hi % c->d.NX);
