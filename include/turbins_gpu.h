@@ -200,6 +200,12 @@ int tb_eno_conv(tb_ctx *ctx, int q);
 /* Velocity_u/v/w_set_RHS (Velocity.c:1607-2043, Velocity.h:15-17),
  * Conc_set_RHS (Conc.c:864-992, Conc.h:30) */
 int tb_set_rhs(tb_ctx *ctx, int q, double dt);
+/* tb_eno_conv(ctx, -1) followed by tb_set_rhs(ctx, q, dt) for q = 0..3, in
+ * one pass of the ENO kernels (same values, bit for bit; the convective terms
+ * are also stored): ENO_compute_convective_terms (ENO.c:103-245) and
+ * Velocity_u/v/w_set_RHS (Velocity.c:1607-2043) / Conc_set_RHS
+ * (Conc.c:864-992) of one RK stage; tb_step uses it */
+int tb_eno_conv_rhs(tb_ctx *ctx, double dt);
 /* y = A_q x with the rows of Velocity_*_form_LHS_matrix
  * (Velocity.c:449-1471) / Conc_form_LHS_matrix (Conc.c:390-730) after
  * Velocity_modify_diagonal / Conc_modify_diagonal for dt

@@ -131,6 +131,36 @@ def test_eno_conv(order, name, walls, box):
     o.close()
 
 
+@pytest.mark.parametrize("name,order", [("turbidity_small", 2), ("bump_small", 3), ("lock64", 1)])
+def test_fused_eno_rhs_bit_identical(name, order):
+    """tb_eno_conv_rhs (the pass tb_step uses) against tb_eno_conv followed by
+    tb_set_rhs for the four quantities: convective terms and right-hand
+    sides identical bit for bit (buoyancy on, particles in turbidity)."""
+    from turbins_amd import Solver
+    c = W.case(name)
+    cfg = W.config_for(c, eno_order=order)
+    st = _random_state(c, seed=9)
+    out = []
+    for fused in (False, True):
+        g = Solver(cfg)
+        g.setup_geometry(W.topography(c))
+        for k, v in st.items():
+            g.upload(k, v)
+        dt = 1e-3
+        if fused:
+            g.eno_conv_rhs(dt)
+        else:
+            g.eno_conv(-1)
+            for q in range(4):
+                g.set_rhs(q, dt)
+        out.append({n: g.download(n) for n in ["CONV_U", "CONV_V", "CONV_W", "CONV_C", "RHS_U", "RHS_V", "RHS_W",
+                                                  "RHS_C"]})
+        g.close()
+    for n in out[0]:
+        assert np.array_equal(out[0][n], out[1][n]), n
+        assert np.any(out[0][n] != 0.0), n
+
+
 def test_rhs_and_operators():
     c, cfg, g, o = _pair("turbidity_small")
     st = _random_state(c, seed=5)

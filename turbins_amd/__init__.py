@@ -50,6 +50,7 @@ def lib():
     L.tb_download_field.argtypes = [P, C.c_int, P]
     L.tb_eno_conv.argtypes = [P, C.c_int]
     L.tb_set_rhs.argtypes = [P, C.c_int, C.c_double]
+    L.tb_eno_conv_rhs.argtypes = [P, C.c_double]
     L.tb_helmholtz_apply.argtypes = [P, C.c_int, C.c_double, C.c_int, C.c_int]
     L.tb_helmholtz_gs_sweeps.argtypes = [P, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int]
     L.tb_helmholtz_solve.argtypes = [P, C.c_int, C.c_double, C.POINTER(abi.SolveStats)]
@@ -183,6 +184,10 @@ class Solver:
 
     def set_rhs(self, q, dt):
         return self._ok(self.L.tb_set_rhs(self.h, q, dt), "tb_set_rhs")
+
+    def eno_conv_rhs(self, dt):
+        """eno_conv(-1) and set_rhs(q, dt) for all four quantities in one pass."""
+        return self._ok(self.L.tb_eno_conv_rhs(self.h, dt), "tb_eno_conv_rhs")
 
     def helmholtz_apply(self, q, dt, x):
         self.upload("SCRATCH0", x)

@@ -221,9 +221,15 @@ struct MomArgs {
     const uint8_t *su, *sv, *sw;
     double *cu, *cv, *cw;
     QLines L[3];
+    // right-hand sides in the same pass (RHS = true): the k_rhs_vel formulas
+    const double *dpx, *dpy, *dpz, *cfld;
+    double *ru, *rv, *rw;
+    double a_dt, buoy;
 };
 
-template <int R>
+__device__ inline bool vel_row(int s) { return s == TB_FLUID || s == TB_BUFFER_SOLID; }
+
+template <int R, bool RHS>
 __global__ void __launch_bounds__(256) k_eno_momentum(MomArgs a, Tiles t) {
     int i, j, k;
     if (!tile_point_kfast(t, i, j, k)) return;
@@ -236,7 +242,8 @@ __global__ void __launch_bounds__(256) k_eno_momentum(MomArgs a, Tiles t) {
     // u node
     {
         double r = 0.0;
-        if (a.su[id] != TB_BOUNDARY) {
+        const int s = a.su[id];
+        if (s != TB_BOUNDARY) {
             double vu = interp4w(v[id - 1], v[id], v[id + d.px - 1], v[id + d.px], W[W_VU_X * wm + i],
                                  W[W_VU_Y * wm + j]);
             double wu = interp4w(w[id - 1], w[id], w[id + d.plane - 1], w[id + d.plane], W[W_WU_X * wm + i],
@@ -246,11 +253,13 @@ __global__ void __launch_bounds__(256) k_eno_momentum(MomArgs a, Tiles t) {
             r += eno_line_point<R>(a.L[0].ax[2], u, id, k, wu);
         }
         st_stream(&a.cu[id], r);
+        if constexpr (RHS) st_stream(&a.ru[id], vel_row(s) ? u[id] * a.a_dt - r - a.dpx[id] : 0.0);
     }
     // v node
     {
         double r = 0.0;
-        if (a.sv[id] != TB_BOUNDARY) {
+        const int s = a.sv[id];
+        if (s != TB_BOUNDARY) {
             double uv = interp4w(u[id - d.px], u[id - d.px + 1], u[id], u[id + 1], W[W_UV_X * wm + i],
                                  W[W_UV_Y * wm + j]);
             double wv = interp4w(w[id - d.px], w[id + d.plane - d.px], w[id], w[id + d.plane], W[W_WV_Z * wm + k],
@@ -260,11 +269,20 @@ __global__ void __launch_bounds__(256) k_eno_momentum(MomArgs a, Tiles t) {
             r += eno_line_point<R>(a.L[1].ax[2], v, id, k, wv);
         }
         st_stream(&a.cv[id], r);
+        if constexpr (RHS) {
+            double rh = 0.0;
+            if (vel_row(s)) {
+                const double c_term = interp2d(a.cfld[id - d.px], g.yc[j - 1], a.cfld[id], g.yc[j], g.yv[j]);
+                rh = v[id] * a.a_dt - r - a.dpy[id] - a.buoy * c_term;
+            }
+            st_stream(&a.rv[id], rh);
+        }
     }
     // w node
     {
         double r = 0.0;
-        if (a.sw[id] != TB_BOUNDARY) {
+        const int s = a.sw[id];
+        if (s != TB_BOUNDARY) {
             double uw = interp4w(u[id - d.plane], u[id - d.plane + 1], u[id], u[id + 1], W[W_UW_X * wm + i],
                                  W[W_UW_Z * wm + k]);
             double vw = interp4w(v[id - d.plane], v[id], v[id - d.plane + d.px], v[id + d.px], W[W_VW_Z * wm + k],
@@ -274,6 +292,7 @@ __global__ void __launch_bounds__(256) k_eno_momentum(MomArgs a, Tiles t) {
             r += eno_line_point<R>(a.L[2].ax[2], w, id, k, w[id]);
         }
         st_stream(&a.cw[id], r);
+        if constexpr (RHS) st_stream(&a.rw[id], vel_row(s) ? w[id] * a.a_dt - r - a.dpz[id] : 0.0);
     }
 }
 
@@ -286,9 +305,11 @@ struct ConcArgs {
     double v_settle;
     int particle;
     QLines L;
+    double *rc;     // right-hand side in the same pass (RHS = true): the k_rhs_conc formula
+    double a_dt;
 };
 
-template <int R>
+template <int R, bool RHS>
 __global__ void __launch_bounds__(256) k_eno_conc(ConcArgs a, Tiles t) {
     int i, j, k;
     if (!tile_point_kfast(t, i, j, k)) return;
@@ -296,7 +317,8 @@ __global__ void __launch_bounds__(256) k_eno_conc(ConcArgs a, Tiles t) {
     const GridDev &g = a.g;
     long id = gidx(d, i, j, k);
     double r = 0.0;
-    if (a.sc[id] != TB_BOUNDARY) {
+    const int s = a.sc[id];
+    if (s != TB_BOUNDARY) {
         // Velocity_cell_center (Velocity.c:1476-1602) at this cell
         double ubc = interp2d(a.u[id], g.xu[i], a.u[id + 1], g.xu[i + 1], g.xc[i]);
         double vbc = interp2d(a.v[id], g.yv[j], a.v[id + d.px], g.yv[j + 1], g.yc[j]);
@@ -307,6 +329,7 @@ __global__ void __launch_bounds__(256) k_eno_conc(ConcArgs a, Tiles t) {
         r += eno_line_point<R>(a.L.ax[2], a.c, id, k, wbc);
     }
     st_stream(&a.cc[id], r);
+    if constexpr (RHS) st_stream(&a.rc[id], s == TB_FLUID ? a.c[id] * a.a_dt - r : 0.0);
 }
 
 // True when every first gap fl(p[m+1] - p[m]) of [lo, hi] equals one power
@@ -410,7 +433,8 @@ static int eno_weights(tb_ctx *c) {
     return 0;
 }
 
-int tb_eno_momentum(tb_ctx *c) {
+// rhs_dt > 0: the four right-hand sides of step dt in the same pass
+static int eno_momentum(tb_ctx *c, double rhs_dt) {
     if (eno_weights(c)) return -1;
     MomArgs a;
     a.d = c->d; a.g = c->g;
@@ -420,19 +444,36 @@ int tb_eno_momentum(tb_ctx *c) {
     a.su = c->st[0]; a.sv = c->st[1]; a.sw = c->st[2];
     a.cu = c->f[TB_F_CONV_U]; a.cv = c->f[TB_F_CONV_V]; a.cw = c->f[TB_F_CONV_W];
     for (int q = 0; q < 3; q++) a.L[q] = make_lines(c, q);
+    a.dpx = c->f[TB_F_DPDX]; a.dpy = c->f[TB_F_DPDY]; a.dpz = c->f[TB_F_DPDZ]; a.cfld = c->f[TB_F_C];
+    a.ru = c->f[TB_F_RHS_U]; a.rv = c->f[TB_F_RHS_V]; a.rw = c->f[TB_F_RHS_W];
+    a.a_dt = rhs_dt > 0.0 ? 1.0 / rhs_dt : 0.0;
+    a.buoy = c->cfg.buoyancy;
     Tiles t = make_tiles(c->d.NX, c->d.NY, c->d.NZ);
+    const dim3 grid = tiles_grid(t), blk = tiles_block();
+    const bool rhs = rhs_dt > 0.0;
     if (c->prof_on) tb_prof_begin(c, TB_K_ENO_MOM);
     switch (c->cfg.eno_order) {
-    case 1: hipLaunchKernelGGL(k_eno_momentum<1>, tiles_grid(t), tiles_block(), 0, c->s, a, t); break;
-    case 2: hipLaunchKernelGGL(k_eno_momentum<2>, tiles_grid(t), tiles_block(), 0, c->s, a, t); break;
-    default: hipLaunchKernelGGL(k_eno_momentum<3>, tiles_grid(t), tiles_block(), 0, c->s, a, t); break;
+    case 1:
+        if (rhs) hipLaunchKernelGGL((k_eno_momentum<1, true>), grid, blk, 0, c->s, a, t);
+        else hipLaunchKernelGGL((k_eno_momentum<1, false>), grid, blk, 0, c->s, a, t);
+        break;
+    case 2:
+        if (rhs) hipLaunchKernelGGL((k_eno_momentum<2, true>), grid, blk, 0, c->s, a, t);
+        else hipLaunchKernelGGL((k_eno_momentum<2, false>), grid, blk, 0, c->s, a, t);
+        break;
+    default:
+        if (rhs) hipLaunchKernelGGL((k_eno_momentum<3, true>), grid, blk, 0, c->s, a, t);
+        else hipLaunchKernelGGL((k_eno_momentum<3, false>), grid, blk, 0, c->s, a, t);
+        break;
     }
     if (c->prof_on) tb_prof_end(c, TB_K_ENO_MOM);
     TB_LAUNCH_CHECK(c);
     return 0;
 }
 
-int tb_eno_conc(tb_ctx *c) {
+int tb_eno_momentum(tb_ctx *c) { return eno_momentum(c, 0.0); }
+
+static int eno_conc(tb_ctx *c, double rhs_dt) {
     ConcArgs a;
     a.d = c->d; a.g = c->g;
     a.u = c->f[TB_F_U]; a.v = c->f[TB_F_V]; a.w = c->f[TB_F_W]; a.c = c->f[TB_F_C];
@@ -441,15 +482,48 @@ int tb_eno_conc(tb_ctx *c) {
     a.v_settle = c->cfg.v_settle;
     a.particle = (c->cfg.conc_type == TB_PARTICLE);
     a.L = make_lines(c, TB_Q_C);
+    a.rc = c->f[TB_F_RHS_C];
+    a.a_dt = rhs_dt > 0.0 ? 1.0 / rhs_dt : 0.0;
     Tiles t = make_tiles(c->d.NX, c->d.NY, c->d.NZ);
+    const dim3 grid = tiles_grid(t), blk = tiles_block();
+    const bool rhs = rhs_dt > 0.0;
     if (c->prof_on) tb_prof_begin(c, TB_K_ENO_CONC);
     switch (c->cfg.eno_order) {
-    case 1: hipLaunchKernelGGL(k_eno_conc<1>, tiles_grid(t), tiles_block(), 0, c->s, a, t); break;
-    case 2: hipLaunchKernelGGL(k_eno_conc<2>, tiles_grid(t), tiles_block(), 0, c->s, a, t); break;
-    default: hipLaunchKernelGGL(k_eno_conc<3>, tiles_grid(t), tiles_block(), 0, c->s, a, t); break;
+    case 1:
+        if (rhs) hipLaunchKernelGGL((k_eno_conc<1, true>), grid, blk, 0, c->s, a, t);
+        else hipLaunchKernelGGL((k_eno_conc<1, false>), grid, blk, 0, c->s, a, t);
+        break;
+    case 2:
+        if (rhs) hipLaunchKernelGGL((k_eno_conc<2, true>), grid, blk, 0, c->s, a, t);
+        else hipLaunchKernelGGL((k_eno_conc<2, false>), grid, blk, 0, c->s, a, t);
+        break;
+    default:
+        if (rhs) hipLaunchKernelGGL((k_eno_conc<3, true>), grid, blk, 0, c->s, a, t);
+        else hipLaunchKernelGGL((k_eno_conc<3, false>), grid, blk, 0, c->s, a, t);
+        break;
     }
     if (c->prof_on) tb_prof_end(c, TB_K_ENO_CONC);
     TB_LAUNCH_CHECK(c);
+    return 0;
+}
+
+int tb_eno_conc(tb_ctx *c) { return eno_conc(c, 0.0); }
+
+// ENO of all four quantities and their right-hand sides in one pass per
+// kernel: the same values as tb_eno_conv(c, -1) followed by tb_set_rhs for
+// q = 0..3 (the right-hand-side formulas of k_rhs_vel / k_rhs_conc applied to
+// the convective term in registers), without re-reading the convective terms
+// and fields; asynchronous (internal to tb_step) unless called through the
+// C entry point below.
+int tb_eno_rhs_async(tb_ctx *c, double dt) {
+    if (eno_momentum(c, dt)) return -1;
+    return eno_conc(c, dt);
+}
+
+extern "C" int tb_eno_conv_rhs(tb_ctx *c, double dt) {
+    if (!c || !(dt > 0.0)) { if (c) c->err = "tb_eno_conv_rhs: bad argument"; return -1; }
+    if (tb_eno_rhs_async(c, dt)) return -1;
+    TB_CHECK(c, hipStreamSynchronize(c->s));
     return 0;
 }
 

@@ -7,8 +7,7 @@
 
 #include "tb_common.h"
 
-int tb_eno_momentum(tb_ctx *c);
-int tb_eno_conc(tb_ctx *c);
+int tb_eno_rhs_async(tb_ctx *c, double dt);
 
 extern "C" int tb_step(tb_ctx *c, double dt, tb_step_stats *st) {
     if (!c || !(dt > 0.0)) { if (c) c->err = "tb_step: bad argument"; return -1; }
@@ -21,10 +20,8 @@ extern "C" int tb_step(tb_ctx *c, double dt, tb_step_stats *st) {
     int worst = 0;
     if (tb_store_old(c)) return -1;
     for (int stage = 0; stage < 2; stage++) {
-        if (tb_eno_momentum(c)) return -1;
-        if (tb_eno_conc(c)) return -1;
-        for (int q = 0; q < 4; q++)
-            if (tb_set_rhs(c, q, dt)) return -1;
+        // ENO and the four right-hand sides (tb_eno_conv + tb_set_rhs) in one pass
+        if (tb_eno_rhs_async(c, dt)) return -1;
         for (int q = 0; q < 4; q++) {
             int rc = tb_helmholtz_solve(c, q, dt, &loc.helm[stage][q]);
             if (rc < 0) return -1;
