@@ -40,7 +40,10 @@ HBM_PEAK_GBS = 8000.0
 # algorithmic bytes per point (each array read or written once per point at
 # its storage width); see DESIGN.md for the table
 BYTES_PER_POINT = {
-    "ENO_MOM": 51, "ENO_CONC": 41, "PROJECT": 115, "BICG_UPD": 56, "PCG_UPD": 48, "POIS_OP": 16,
+    # ENO as tb_step runs it, with the right-hand sides in the same pass:
+    # momentum u, v, w, three flags, three pressure gradients and c in, three
+    # right-hand sides out; concentration c, u, v, w and a flag in, one out
+    "ENO_MOM": 83, "ENO_CONC": 41, "PROJECT": 115, "BICG_UPD": 56, "PCG_UPD": 48, "POIS_OP": 16,
     "CG32_OP": 8, "CG32_UPD": 24,
 }
 WEIGHTS = os.path.join(ROOT, "turbins_amd", "weights", "poisson_guess.f32")

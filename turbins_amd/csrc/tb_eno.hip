@@ -225,6 +225,7 @@ struct MomArgs {
     const double *dpx, *dpy, *dpz, *cfld;
     double *ru, *rv, *rw;
     double a_dt, buoy;
+    int store_conv;   // 0 with RHS: the convective terms are consumed in registers only
 };
 
 __device__ inline bool vel_row(int s) { return s == TB_FLUID || s == TB_BUFFER_SOLID; }
@@ -252,7 +253,7 @@ __global__ void __launch_bounds__(256) k_eno_momentum(MomArgs a, Tiles t) {
             r += eno_line_point<R>(a.L[0].ax[1], u, id, j, vu);
             r += eno_line_point<R>(a.L[0].ax[2], u, id, k, wu);
         }
-        st_stream(&a.cu[id], r);
+        if (!RHS || a.store_conv) st_stream(&a.cu[id], r);
         if constexpr (RHS) st_stream(&a.ru[id], vel_row(s) ? u[id] * a.a_dt - r - a.dpx[id] : 0.0);
     }
     // v node
@@ -268,7 +269,7 @@ __global__ void __launch_bounds__(256) k_eno_momentum(MomArgs a, Tiles t) {
             r += eno_line_point<R>(a.L[1].ax[1], v, id, j, v[id]);
             r += eno_line_point<R>(a.L[1].ax[2], v, id, k, wv);
         }
-        st_stream(&a.cv[id], r);
+        if (!RHS || a.store_conv) st_stream(&a.cv[id], r);
         if constexpr (RHS) {
             double rh = 0.0;
             if (vel_row(s)) {
@@ -291,7 +292,7 @@ __global__ void __launch_bounds__(256) k_eno_momentum(MomArgs a, Tiles t) {
             r += eno_line_point<R>(a.L[2].ax[1], w, id, j, vw);
             r += eno_line_point<R>(a.L[2].ax[2], w, id, k, w[id]);
         }
-        st_stream(&a.cw[id], r);
+        if (!RHS || a.store_conv) st_stream(&a.cw[id], r);
         if constexpr (RHS) st_stream(&a.rw[id], vel_row(s) ? w[id] * a.a_dt - r - a.dpz[id] : 0.0);
     }
 }
@@ -307,6 +308,7 @@ struct ConcArgs {
     QLines L;
     double *rc;     // right-hand side in the same pass (RHS = true): the k_rhs_conc formula
     double a_dt;
+    int store_conv;
 };
 
 template <int R, bool RHS>
@@ -328,7 +330,7 @@ __global__ void __launch_bounds__(256) k_eno_conc(ConcArgs a, Tiles t) {
         r += eno_line_point<R>(a.L.ax[1], a.c, id, j, vbc);
         r += eno_line_point<R>(a.L.ax[2], a.c, id, k, wbc);
     }
-    st_stream(&a.cc[id], r);
+    if (!RHS || a.store_conv) st_stream(&a.cc[id], r);
     if constexpr (RHS) st_stream(&a.rc[id], s == TB_FLUID ? a.c[id] * a.a_dt - r : 0.0);
 }
 
@@ -433,8 +435,9 @@ static int eno_weights(tb_ctx *c) {
     return 0;
 }
 
-// rhs_dt > 0: the four right-hand sides of step dt in the same pass
-static int eno_momentum(tb_ctx *c, double rhs_dt) {
+// rhs_dt > 0: the four right-hand sides of step dt in the same pass, the
+// convective terms stored only if store_conv
+static int eno_momentum(tb_ctx *c, double rhs_dt, bool store_conv) {
     if (eno_weights(c)) return -1;
     MomArgs a;
     a.d = c->d; a.g = c->g;
@@ -448,6 +451,7 @@ static int eno_momentum(tb_ctx *c, double rhs_dt) {
     a.ru = c->f[TB_F_RHS_U]; a.rv = c->f[TB_F_RHS_V]; a.rw = c->f[TB_F_RHS_W];
     a.a_dt = rhs_dt > 0.0 ? 1.0 / rhs_dt : 0.0;
     a.buoy = c->cfg.buoyancy;
+    a.store_conv = store_conv ? 1 : 0;
     Tiles t = make_tiles(c->d.NX, c->d.NY, c->d.NZ);
     const dim3 grid = tiles_grid(t), blk = tiles_block();
     const bool rhs = rhs_dt > 0.0;
@@ -471,9 +475,9 @@ static int eno_momentum(tb_ctx *c, double rhs_dt) {
     return 0;
 }
 
-int tb_eno_momentum(tb_ctx *c) { return eno_momentum(c, 0.0); }
+int tb_eno_momentum(tb_ctx *c) { return eno_momentum(c, 0.0, true); }
 
-static int eno_conc(tb_ctx *c, double rhs_dt) {
+static int eno_conc(tb_ctx *c, double rhs_dt, bool store_conv) {
     ConcArgs a;
     a.d = c->d; a.g = c->g;
     a.u = c->f[TB_F_U]; a.v = c->f[TB_F_V]; a.w = c->f[TB_F_W]; a.c = c->f[TB_F_C];
@@ -484,6 +488,7 @@ static int eno_conc(tb_ctx *c, double rhs_dt) {
     a.L = make_lines(c, TB_Q_C);
     a.rc = c->f[TB_F_RHS_C];
     a.a_dt = rhs_dt > 0.0 ? 1.0 / rhs_dt : 0.0;
+    a.store_conv = store_conv ? 1 : 0;
     Tiles t = make_tiles(c->d.NX, c->d.NY, c->d.NZ);
     const dim3 grid = tiles_grid(t), blk = tiles_block();
     const bool rhs = rhs_dt > 0.0;
@@ -507,22 +512,22 @@ static int eno_conc(tb_ctx *c, double rhs_dt) {
     return 0;
 }
 
-int tb_eno_conc(tb_ctx *c) { return eno_conc(c, 0.0); }
+int tb_eno_conc(tb_ctx *c) { return eno_conc(c, 0.0, true); }
 
 // ENO of all four quantities and their right-hand sides in one pass per
 // kernel: the same values as tb_eno_conv(c, -1) followed by tb_set_rhs for
 // q = 0..3 (the right-hand-side formulas of k_rhs_vel / k_rhs_conc applied to
 // the convective term in registers), without re-reading the convective terms
-// and fields; asynchronous (internal to tb_step) unless called through the
-// C entry point below.
-int tb_eno_rhs_async(tb_ctx *c, double dt) {
-    if (eno_momentum(c, dt)) return -1;
-    return eno_conc(c, dt);
+// and fields.  tb_step passes store_conv = false (nothing downstream reads
+// the convective terms: the TB_F_CONV_* fields keep their last values).
+int tb_eno_rhs_async(tb_ctx *c, double dt, bool store_conv) {
+    if (eno_momentum(c, dt, store_conv)) return -1;
+    return eno_conc(c, dt, store_conv);
 }
 
 extern "C" int tb_eno_conv_rhs(tb_ctx *c, double dt) {
     if (!c || !(dt > 0.0)) { if (c) c->err = "tb_eno_conv_rhs: bad argument"; return -1; }
-    if (tb_eno_rhs_async(c, dt)) return -1;
+    if (tb_eno_rhs_async(c, dt, true)) return -1;
     TB_CHECK(c, hipStreamSynchronize(c->s));
     return 0;
 }

@@ -7,7 +7,7 @@
 
 #include "tb_common.h"
 
-int tb_eno_rhs_async(tb_ctx *c, double dt);
+int tb_eno_rhs_async(tb_ctx *c, double dt, bool store_conv);
 
 extern "C" int tb_step(tb_ctx *c, double dt, tb_step_stats *st) {
     if (!c || !(dt > 0.0)) { if (c) c->err = "tb_step: bad argument"; return -1; }
@@ -20,8 +20,9 @@ extern "C" int tb_step(tb_ctx *c, double dt, tb_step_stats *st) {
     int worst = 0;
     if (tb_store_old(c)) return -1;
     for (int stage = 0; stage < 2; stage++) {
-        // ENO and the four right-hand sides (tb_eno_conv + tb_set_rhs) in one pass
-        if (tb_eno_rhs_async(c, dt)) return -1;
+        // ENO and the four right-hand sides (tb_eno_conv + tb_set_rhs) in one
+        // pass; the convective terms stay in registers
+        if (tb_eno_rhs_async(c, dt, false)) return -1;
         for (int q = 0; q < 4; q++) {
             int rc = tb_helmholtz_solve(c, q, dt, &loc.helm[stage][q]);
             if (rc < 0) return -1;
