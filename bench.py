@@ -7,7 +7,7 @@ ENO, dt = 1e-3), every Poisson solve to a 1e-8 relative residual, every
 Helmholtz solve to 1e-12.  Two device paths are timed on the same state:
   plain    -- FP64 GMG-preconditioned CG from the previous solution (the
               reference's initial guess)
-  headline -- learned mode: Galerkin guess on the last six solutions and
+  headline -- learned mode: Galerkin guess on the last eight solutions and
               the trained network's direction (the network is dropped at
               run time if its saved iterations cost more than its
               inference), then FP32 CG with FP32 V-cycles inside FP64
@@ -266,7 +266,7 @@ def main():
             "cell_updates_per_s": round(head_sps * c["nx"] * c["ny"] * c["nz"], 1),
             "poisson_rtol": 1e-8,
             "helmholtz_rtol": 1e-12,
-            "headline_path": "Galerkin guess from the last six solutions"
+            "headline_path": "Galerkin guess from the last eight solutions"
                              + (" + network direction while its gate is open" if learned else "")
                              + ", FP32 CG + FP32 V-cycles inside FP64 iterative refinement",
             "learned_guess_loaded": learned,

@@ -266,6 +266,11 @@ int tb_set_mixed_inner(tb_ctx *ctx, double reduction_floor, int maxit);
  * attached and its gate open, is the initial guess (the reference starts
  * from the previous pressure alone, Solver.c:256) */
 int tb_set_guess_history(tb_ctx *ctx, int h);
+/* sweeps of the finest multigrid level per V-cycle side, FP32 and FP64
+ * cycles apart (0: mg_pre / mg_post of the config, which the coarser levels
+ * keep; 1..8 otherwise).  No reference counterpart: hypre's BoomerAMG
+ * settings (Solver.c:90-265) are fixed there. */
+int tb_set_mg_finest_sweeps(tb_ctx *ctx, int pre32, int post32, int pre64, int post64);
 /* multigrid V-cycles replayed from HIP graphs (default 1) or launched
  * kernel by kernel (0); results are identical */
 int tb_set_graphs(tb_ctx *ctx, int on);

@@ -68,6 +68,7 @@ def lib():
     L.tb_set_guess_history.argtypes = [P, C.c_int]
     L.tb_set_helmholtz_method.argtypes = [P, C.c_int]
     L.tb_set_graphs.argtypes = [P, C.c_int]
+    L.tb_set_mg_finest_sweeps.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_int]
     L.tb_profile_enable.argtypes = [P, C.c_int]
     L.tb_profile_read.argtypes = [P, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
     L.tb_net_create.restype = P
@@ -258,6 +259,12 @@ class Solver:
 
     def set_guess_history(self, h):
         return self._ok(self.L.tb_set_guess_history(self.h, h), "tb_set_guess_history")
+
+    def set_mg_finest_sweeps(self, pre32=0, post32=0, pre64=0, post64=0):
+        """Sweeps of the finest multigrid level per V-cycle side (0: the
+        config's mg_pre / mg_post, which the coarser levels keep)."""
+        return self._ok(self.L.tb_set_mg_finest_sweeps(self.h, pre32, post32, pre64, post64),
+                        "tb_set_mg_finest_sweeps")
 
     def profile(self, on=True):
         return self._ok(self.L.tb_profile_enable(self.h, 1 if on else 0), "tb_profile_enable")

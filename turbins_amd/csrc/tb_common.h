@@ -213,7 +213,8 @@ __device__ inline void block_sum(double (&v)[NV], double *sh /* [NV*16] */) {
 // GMG level description (cell-centred arrays)
 // history depth of the Galerkin initial guess, and its directions with the
 // network's
-constexpr int GUESS_HIST = 6;
+constexpr int GUESS_HIST = 10;     // stored solutions (upper bound of tb_set_guess_history)
+constexpr int GUESS_DEFAULT = 8;
 constexpr int GUESS_DIRS = GUESS_HIST + 1;
 
 struct Level {
@@ -284,6 +285,8 @@ struct tb_ctx {
     double helm_rate[4] = {0.0, 0.0, 0.0, 0.0};   // and its residual contraction per pair
     double ir_floor = 1e-5;      // smallest residual reduction asked of one inner FP32 solve
     int ir_maxit = 40;           // inner iterations per refinement pass
+    int mg_pre0[2] = {0, 0};     // sweeps on the finest level per V-cycle side (FP32, FP64);
+    int mg_post0[2] = {0, 0};    // 0: cfg.mg_pre / cfg.mg_post as on the coarser levels
     int coarse_n = 0;
     int small_l0[2] = {0, 0};   // first level of the one-workgroup cycle: FP32, FP64
     bool mg_built = false;
@@ -291,9 +294,17 @@ struct tb_ctx {
     // Poisson history for the learned guess
     double *phi_hist[GUESS_HIST] = {};   // last Poisson solutions, newest first
     int n_hist = 0;
+    // Galerkin matrix of the stored solutions, gram[i][j] = phi_i . A phi_j,
+    // valid for the leading gram_n entries.  A phi_j = b_j holds to the solve
+    // tolerance, so a new row is phi_k . b of the solve that made phi_0 (the
+    // products the guess of that solve formed, guess_r[0..guess_nr)) and the
+    // diagonal phi_0 . b (the final residual pass); no right-hand side is kept.
+    double gram[GUESS_HIST][GUESS_HIST] = {};
+    int gram_n = 0;
+    double guess_r[GUESS_HIST] = {};
+    int guess_nr = 0;
     bool phi_pinned = false;   // the phi field is the output of the last Poisson solve (corner exactly 0)
-    double *b_hist[GUESS_HIST] = {};     // their right-hand sides (learned mode only)
-    int guess_hist = GUESS_HIST; // solutions in the Galerkin initial guess (tb_set_guess_history)
+    int guess_hist = GUESS_DEFAULT; // solutions in the Galerkin initial guess (tb_set_guess_history)
     int nb_hist = 0;
     tb_net *net = nullptr;       // attached learned guess (not owned)
     tb_net *net_record = nullptr;    // snapshot sink after every solve (not owned)
@@ -330,7 +341,9 @@ int tb_prof_flush(tb_ctx *c);
 // device scalar slots (ctx->scal)
 enum {
     S_RHO = 0, S_ALPHA, S_OMEGA, S_BETA, S_RZ, S_PQ, S_RR, S_TS, S_TT, S_RHV,
-    S_BB, S_MEAN, S_CORNER, S_TMP0, S_TMP1, S_TMP2, S_SUB = 16, S_COUNT = 16 + GUESS_DIRS * GUESS_DIRS + GUESS_DIRS
+    S_BB, S_MEAN, S_CORNER, S_TMP0, S_TMP1, S_TMP2, S_SUB = 16,
+    S_RES = 16 + GUESS_DIRS * GUESS_DIRS + GUESS_DIRS,   // Poisson residual: r.r, b.b, x.b
+    S_COUNT = S_RES + 3
 };
 
 // error helpers -------------------------------------------------------------

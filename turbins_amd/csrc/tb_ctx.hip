@@ -118,7 +118,6 @@ extern "C" tb_ctx *tb_create(const tb_config *cfg) {
     for (int q = 0; ok && q < 4; q++) ok = dev_alloc(c, (void **)&c->st[q], (size_t)c->d.ntot) == 0;
     for (int a = 0; ok && a < 8; a++) ok = dev_alloc(c, (void **)&c->kv[a], fb) == 0;
     for (int a = 0; ok && a < GUESS_HIST; a++) ok = dev_alloc(c, (void **)&c->phi_hist[a], fb) == 0;
-    for (int a = 0; ok && a < GUESS_HIST; a++) ok = dev_alloc(c, (void **)&c->b_hist[a], fb) == 0;
     ok = ok && dev_alloc(c, (void **)&c->red, sizeof(double) * RED_SLOTS * RED_MAX_BLOCKS) == 0;
     ok = ok && dev_alloc(c, (void **)&c->scal, sizeof(double) * S_COUNT) == 0;
     ok = ok && hipHostMalloc((void **)&c->hscal, sizeof(double) * S_COUNT, hipHostMallocCoherent) == hipSuccess;
@@ -152,7 +151,6 @@ extern "C" void tb_destroy(tb_ctx *c) {
     for (int q = 0; q < 4; q++) { hipFree(c->old[q]); hipFree(c->st[q]); free_ib(c, q); }
     for (int a = 0; a < 8; a++) hipFree(c->kv[a]);
     for (int a = 0; a < GUESS_HIST; a++) hipFree(c->phi_hist[a]);
-    for (int a = 0; a < GUESS_HIST; a++) hipFree(c->b_hist[a]);
     for (auto &e : c->prof_ev) hipEventDestroy(e);
     hipFree(c->red);
     hipFree(c->scal);
@@ -183,7 +181,7 @@ extern "C" int tb_upload_field(tb_ctx *c, int f, const double *host) {
     TB_CHECK(c, hipMemcpy2DAsync(c->f[f], sizeof(double) * d.px, host, sizeof(double) * d.NX,
                                  sizeof(double) * d.NX, (size_t)d.NY * d.NZ, hipMemcpyHostToDevice, c->s));
     TB_CHECK(c, hipStreamSynchronize(c->s));
-    if (f == TB_F_PHI) c->n_hist = c->nb_hist = 0;
+    if (f == TB_F_PHI) c->n_hist = c->gram_n = c->guess_nr = 0;
     tb_field_written(c, f);
     return 0;
 }

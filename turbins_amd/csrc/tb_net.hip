@@ -1180,7 +1180,10 @@ extern "C" int tb_net_attach(tb_ctx *c, tb_net *net) {
 // context's stream; cells only (the caller owns the other entries)
 int tb_net_predict(tb_ctx *c, double *guess, int correction_only) {
     tb_net *net = c->net;
-    if (!net || c->n_hist < 2) return -1;
+    if (!net || c->n_hist < 2) {
+        c->err = "network guess: needs an attached network and two previous solutions";
+        return -1;
+    }
     if (acts_ensure(net, net->inf, c->ncx, c->ncy, c->ncz, false)) { c->err = net->err; return -1; }
     Acts &A = net->inf;
     if (tb_dot(c, c->f[TB_F_PRHS], c->f[TB_F_PRHS], S_TMP1)) return -1;

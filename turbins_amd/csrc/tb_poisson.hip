@@ -64,8 +64,8 @@ template <int RESID>
 __global__ void __launch_bounds__(256) k_pois_km(PoisOp P, const double *__restrict__ x, const double *__restrict__ b,
                                                  double *__restrict__ y, float *__restrict__ y32, double *part,
                                                  KTiles t) {
-    __shared__ double sh[32];
-    double acc[2] = {0.0, 0.0};
+    __shared__ double sh[48];
+    double acc[3] = {0.0, 0.0, 0.0};
     int i, j, k0, k1;
     bool act;
     ktile_column(t, i, j, k0, k1, act);
@@ -123,6 +123,7 @@ __global__ void __launch_bounds__(256) k_pois_km(PoisOp P, const double *__restr
                 if (y32) st_stream(&y32[id], (float)ri);
                 acc[0] += ri * ri;
                 acc[1] += bi * bi;
+                acc[2] += xc * bi;
             } else {
                 st_stream(&y[id], v);
                 acc[0] += xc * v;
@@ -132,10 +133,11 @@ __global__ void __launch_bounds__(256) k_pois_km(PoisOp P, const double *__restr
             id += d.plane;
         }
     }
-    block_sum<2>(acc, sh);
+    block_sum<3>(acc, sh);
     if (threadIdx.x == 0 && threadIdx.y == 0) {
         part[blockIdx.x] = acc[0];
         part[RED_MAX_BLOCKS + blockIdx.x] = acc[1];
+        part[2 * RED_MAX_BLOCKS + blockIdx.x] = acc[2];
     }
 }
 
@@ -1135,7 +1137,9 @@ static int vcycle_level(tb_ctx *c, int l) {
     Tiles all = make_tiles(L.nx, L.ny, L.nz);
     Tiles half = make_tiles((L.nx + 1) / 2, L.ny, L.nz);
     const int kid = (l == 0) ? (sizeof(T) == 8 ? TB_K_RBGS0 : TB_K_RBGS0_F32) : -1;
-    const int pre = c->cfg.mg_pre, post = c->cfg.mg_post;
+    const int prec = sizeof(T) == 8 ? 1 : 0;
+    const int pre = (l == 0 && c->mg_pre0[prec] > 0) ? c->mg_pre0[prec] : c->cfg.mg_pre;
+    const int post = (l == 0 && c->mg_post0[prec] > 0) ? c->mg_post0[prec] : c->cfg.mg_post;
     // large levels: fused sweep pairs alternating between x and the level's
     // second buffer; cur is the buffer holding the current iterate
     const bool pairs = B.a != nullptr;
@@ -1253,6 +1257,25 @@ int tb_vcycle32(tb_ctx *c) {
     return vcycle_run<float>(c, 1);
 }
 
+// sweeps (or sweep pairs) on the finest level of the FP32 / FP64 V-cycles;
+// 0 keeps cfg.mg_pre / cfg.mg_post there.  Captured V-cycles are dropped.
+extern "C" int tb_set_mg_finest_sweeps(tb_ctx *c, int pre32, int post32, int pre64, int post64) {
+    if (!c) return -1;
+    if (pre32 < 0 || post32 < 0 || pre64 < 0 || post64 < 0 || pre32 > 8 || post32 > 8 || pre64 > 8 || post64 > 8) {
+        c->err = "tb_set_mg_finest_sweeps: sweeps must be 0..8";
+        return -1;
+    }
+    c->mg_pre0[0] = pre32;
+    c->mg_post0[0] = post32;
+    c->mg_pre0[1] = pre64;
+    c->mg_post0[1] = post64;
+    for (int a = 0; a < 2; a++) {
+        if (c->vc_exec[a]) hipGraphExecDestroy(c->vc_exec[a]);
+        c->vc_exec[a] = nullptr;
+    }
+    return 0;
+}
+
 extern "C" int tb_set_graphs(tb_ctx *c, int on) {
     if (!c) return -1;
     c->use_graphs = on != 0;
@@ -1345,10 +1368,10 @@ static int residual(tb_ctx *c, const PoisCtx &P, bool neumann, const double *x, 
     hipLaunchKernelGGL(k_pois_km<1>, ktiles_grid(P.kt), tiles_block(), 0, c->s, neumann ? P.Pn : P.Pp, x, b, r, r32,
                        c->red, P.kt);
     TB_LAUNCH_CHECK(c);
-    if (tb_reduce_finish(c, (int)P.nblk, 2, S_TMP1)) return -1;
-    if (read_s(c, S_TMP1, 2)) return -1;
-    rr = c->hscal[S_TMP1];
-    bb = c->hscal[S_TMP2];
+    if (tb_reduce_finish(c, (int)P.nblk, 3, S_RES)) return -1;
+    if (read_s(c, S_RES, 3)) return -1;
+    rr = c->hscal[S_RES];
+    bb = c->hscal[S_RES + 1];   // x.b in S_RES + 2
     return 0;
 }
 
@@ -1595,34 +1618,44 @@ static int mixed_ir(tb_ctx *c, const PoisCtx &P, double *x, const double *b, dou
 //   x0 = sum_k a_k phi_{n-k} (k = 1..H) + a_d d
 // of the last H solutions and the network's correction d (when a network is
 // attached and its gate is open), i.e. the Galerkin solution of A x = b on
-// that subspace.  A phi_{n-k} = b_{n-k} holds to the solve tolerance, so only
-// d needs an operator application.  Returns the true relative residual of
-// x0 (pinned operator) in resg and the number of directions used in ndir
-// (0: none usable; the caller then restores phi_{n-1}).
+// that subspace.  The Galerkin matrix of the stored solutions is kept from
+// solve to solve (ctx->gram: A phi_{n-k} = b_{n-k} holds to the solve
+// tolerance, so phi_i . A phi_j is phi_i . b_j, formed when the later of the
+// two was solved), so one pass forms the products phi_k . b with the new
+// right-hand side and, with the network, the products with d and A d.
+// Returns the true relative residual of x0 (pinned operator) in resg and the
+// number of directions used in ndir (0: none usable; the caller then restores
+// phi_{n-1}).
 
 struct Dirs {
-    const double *g[GUESS_DIRS];    // directions
-    const double *ag[GUESS_DIRS];   // their images A g
+    const double *g[GUESS_HIST];    // stored solutions, newest first
 };
 
-// the M*M sums g_i . A g_j and the M sums g_i . b, per block
-template <int M>
-__global__ void __launch_bounds__(PT) k_subspace_dots(long n, Dirs D, const double *__restrict__ b, double *part) {
-    constexpr int NS = M * M + M;
+// per block: the H sums phi_k . b, then with the network's direction d and
+// its image q = A d the H sums phi_k . q, d . b and d . q
+template <int H, bool NET>
+__global__ void __launch_bounds__(PT) k_guess_dots(long n, Dirs D, const double *__restrict__ b,
+                                                   const double *__restrict__ d, const double *__restrict__ q,
+                                                   double *part) {
+    constexpr int NS = NET ? 2 * H + 2 : H;
     __shared__ double sh[NS * 16];
     double a[NS];
 #pragma unroll
     for (int k = 0; k < NS; k++) a[k] = 0.0;
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-        double g[M], A[M];
-#pragma unroll
-        for (int u = 0; u < M; u++) { g[u] = D.g[u][i]; A[u] = D.ag[u][i]; }
         const double bi = b[i];
+        double qi = 0.0;
+        if (NET) {
+            const double di = d[i];
+            qi = q[i];
+            a[2 * H] += di * bi;
+            a[2 * H + 1] += di * qi;
+        }
 #pragma unroll
-        for (int u = 0; u < M; u++) {
-#pragma unroll
-            for (int v = 0; v < M; v++) a[u * M + v] += g[u] * A[v];
-            a[M * M + u] += g[u] * bi;
+        for (int u = 0; u < H; u++) {
+            const double g = D.g[u][i];
+            a[u] += g * bi;
+            if (NET) a[H + u] += g * qi;
         }
     }
     block_sum<NS>(a, sh);
@@ -1655,10 +1688,12 @@ struct Coefs {
     double a[GUESS_DIRS];
 };
 
-__global__ void k_combine(long n, double *__restrict__ x, Dirs D, int m, Coefs C) {
+// x = sum_{u < m} a_u g_u (+ a_m d when d is given)
+__global__ void k_combine(long n, double *__restrict__ x, Dirs D, int m, const double *__restrict__ d, Coefs C) {
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
         double s = 0.0;
         for (int u = 0; u < m; u++) s += C.a[u] * D.g[u][i];
+        if (d) s += C.a[m] * d[i];
         x[i] = s;
     }
 }
@@ -1689,13 +1724,15 @@ static bool small_solve(int m, const double M[GUESS_DIRS][GUESS_DIRS], const dou
     return true;
 }
 
-template <int M>
-static void launch_dots(tb_ctx *c, long n, const Dirs &D, const double *b) {
-    hipLaunchKernelGGL(k_subspace_dots<M>, dim3(PB), dim3(PT), 0, c->s, n, D, b, c->red);
+template <int H>
+static void launch_dots(tb_ctx *c, long n, const Dirs &D, const double *b, const double *d, const double *q) {
+    if (d) hipLaunchKernelGGL((k_guess_dots<H, true>), dim3(PB), dim3(PT), 0, c->s, n, D, b, d, q, c->red);
+    else hipLaunchKernelGGL((k_guess_dots<H, false>), dim3(PB), dim3(PT), 0, c->s, n, D, b, d, q, c->red);
 }
 
 // On return with ndir > 0, x is pinned and lev[0].b32 holds the FP32 copy of
-// its residual, whose squared norm is rrg.
+// its residual, whose squared norm is rrg.  c->guess_r / guess_nr receive the
+// products phi_k . b for the Galerkin matrix of the next solve.
 static int learned_guess(tb_ctx *c, const PoisCtx &P, double *x, const double *b, double bn, double &resg, double &rrg,
                          int &ndir) {
     const long n = c->d.ntot;
@@ -1704,12 +1741,13 @@ static int learned_guess(tb_ctx *c, const PoisCtx &P, double *x, const double *b
     c->net_used = false;
     int H = c->guess_hist;
     if (H > c->n_hist) H = c->n_hist;
-    if (H > c->nb_hist) H = c->nb_hist;
+    if (H > c->gram_n) H = c->gram_n;
+    if (H < 1) return 0;
     Dirs D;
-    for (int k = 0; k < H; k++) { D.g[k] = c->phi_hist[k]; D.ag[k] = c->b_hist[k]; }
-    int m = H;
-    bool use_d = false;
-    if (c->net && c->net_open) {
+    for (int k = 0; k < H; k++) D.g[k] = c->phi_hist[k];
+    // the network reads the previous two solutions
+    const bool use_d = c->net && c->net_open && c->n_hist >= 2;
+    if (use_d) {
         TB_CHECK(c, hipMemsetAsync(dd, 0, sizeof(double) * n, c->s));
         TB_CHECK(c, hipEventRecord(c->net_ev[0], c->s));
         if (tb_net_predict(c, dd, 1)) return -1;
@@ -1717,31 +1755,40 @@ static int learned_guess(tb_ctx *c, const PoisCtx &P, double *x, const double *b
         hipLaunchKernelGGL(k_pois_km<0>, ktiles_grid(P.kt), tiles_block(), 0, c->s, P.Pn, (const double *)dd,
                            (const double *)nullptr, q, (float *)nullptr, c->red, P.kt);
         TB_LAUNCH_CHECK(c);
-        D.g[m] = dd;
-        D.ag[m] = q;
-        m++;
-        use_d = true;
     }
-    if (m < 1) return 0;
-    switch (m) {
-    case 1: launch_dots<1>(c, n, D, b); break;
-    case 2: launch_dots<2>(c, n, D, b); break;
-    case 3: launch_dots<3>(c, n, D, b); break;
-    case 4: launch_dots<4>(c, n, D, b); break;
-    case 5: launch_dots<5>(c, n, D, b); break;
-    case 6: launch_dots<6>(c, n, D, b); break;
-    default: launch_dots<7>(c, n, D, b); break;
+    const double *dp = use_d ? dd : nullptr, *qp = use_d ? q : nullptr;
+    switch (H) {
+    case 1: launch_dots<1>(c, n, D, b, dp, qp); break;
+    case 2: launch_dots<2>(c, n, D, b, dp, qp); break;
+    case 3: launch_dots<3>(c, n, D, b, dp, qp); break;
+    case 4: launch_dots<4>(c, n, D, b, dp, qp); break;
+    case 5: launch_dots<5>(c, n, D, b, dp, qp); break;
+    case 6: launch_dots<6>(c, n, D, b, dp, qp); break;
+    case 7: launch_dots<7>(c, n, D, b, dp, qp); break;
+    case 8: launch_dots<8>(c, n, D, b, dp, qp); break;
+    case 9: launch_dots<9>(c, n, D, b, dp, qp); break;
+    default: launch_dots<10>(c, n, D, b, dp, qp); break;
     }
-    const int ns = m * m + m;
+    static_assert(GUESS_HIST == 10, "one dot-product instantiation per history length");
+    TB_LAUNCH_CHECK(c);
+    const int ns = use_d ? 2 * H + 2 : H;
     hipLaunchKernelGGL(k_sum_cols, dim3(ns), dim3(SUM_COLS_THREADS), 0, c->s, (const double *)c->red, PB, ns,
                        c->scal + S_SUB);
     TB_LAUNCH_CHECK(c);
     if (read_s(c, S_SUB, ns)) return -1;
     const double *h = c->hscal + S_SUB;
+    for (int k = 0; k < H; k++) c->guess_r[k] = h[k];
+    c->guess_nr = H;
+    const int m = use_d ? H + 1 : H;
     double M[GUESS_DIRS][GUESS_DIRS], r[GUESS_DIRS], a[GUESS_DIRS];
-    for (int i = 0; i < m; i++) {
-        r[i] = h[m * m + i];
-        for (int j = 0; j < m; j++) M[i][j] = 0.5 * (h[i * m + j] + h[j * m + i]);
+    for (int i = 0; i < H; i++) {
+        r[i] = h[i];
+        for (int j = 0; j < H; j++) M[i][j] = c->gram[i][j];
+    }
+    if (use_d) {
+        for (int k = 0; k < H; k++) M[k][H] = M[H][k] = h[H + k];
+        r[H] = h[2 * H];
+        M[H][H] = h[2 * H + 1];
     }
     // the largest leading block that is well conditioned (the network's
     // direction comes last and is the first to go)
@@ -1749,7 +1796,8 @@ static int learned_guess(tb_ctx *c, const PoisCtx &P, double *x, const double *b
         if (small_solve(k, M, r, a)) { ndir = k; break; }
     }
     if (ndir == 0) return 0;
-    if (use_d && ndir == m && m > 1) {
+    const bool with_d = use_d && ndir == m;
+    if (with_d && m > 1) {
         // energies J = -a.r/2 of the Galerkin guesses with and without d
         double a2[GUESS_DIRS];
         if (small_solve(m - 1, M, r, a2)) {
@@ -1763,14 +1811,15 @@ static int learned_guess(tb_ctx *c, const PoisCtx &P, double *x, const double *b
     }
     Coefs C;
     for (int k = 0; k < GUESS_DIRS; k++) C.a[k] = k < ndir ? a[k] : 0.0;
-    hipLaunchKernelGGL(k_combine, dim3(PB), dim3(PT), 0, c->s, n, x, D, ndir, C);
+    hipLaunchKernelGGL(k_combine, dim3(PB), dim3(PT), 0, c->s, n, x, D, with_d ? H : ndir,
+                       with_d ? (const double *)dd : nullptr, C);
     TB_LAUNCH_CHECK(c);
     double bb;
     // Every stored solution is pinned (each solve ends with shift_corner:
     // corner exactly 0, half-cell planes 0), so a combination of them is
     // too and the shift would subtract exactly 0; only the network's
     // direction needs it.
-    if (use_d && ndir == m && shift_corner(c, P, x)) return -1;
+    if (with_d && shift_corner(c, P, x)) return -1;
     // With x[corner] = 0 the pinned and the Neumann residual are the same
     // vector (the rows differ only in the corner's own coefficient): one
     // pass gives the guess's residual norm and the FP32 residual the mixed
@@ -1780,9 +1829,28 @@ static int learned_guess(tb_ctx *c, const PoisCtx &P, double *x, const double *b
     return 0;
 }
 
+// The history rotates by one solution (newest first): the Galerkin matrix
+// gains the row of the new solution, phi_0 . A phi_0 = phi_0 . b (xb) and
+// phi_k . A phi_0 = phi_k . b (the guess's products, c->guess_r), and keeps
+// the block of the older ones.
+static void gram_push(tb_ctx *c, double xb) {
+    double G[GUESS_HIST][GUESS_HIST];
+    const int keep = std::min(c->gram_n, GUESS_HIST - 1);
+    const int nr = std::min(c->guess_nr, keep);
+    for (int i = 0; i < keep; i++)
+        for (int j = 0; j < keep; j++) G[i + 1][j + 1] = c->gram[i][j];
+    G[0][0] = xb;
+    for (int k = 0; k < nr; k++) G[0][k + 1] = G[k + 1][0] = c->guess_r[k];
+    const int n = 1 + nr;
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++) c->gram[i][j] = G[i][j];
+    c->gram_n = n;
+    c->guess_nr = 0;
+}
+
 extern "C" int tb_set_guess_history(tb_ctx *c, int h) {
     if (!c || h < 1 || h > GUESS_HIST) {
-        if (c) c->err = "tb_set_guess_history: h must be 1..6";
+        if (c) c->err = "tb_set_guess_history: h must be 1..10";
         return -1;
     }
     c->guess_hist = h;
@@ -1864,7 +1932,7 @@ extern "C" int tb_poisson_solve(tb_ctx *c, tb_solve_stats *st) {
         loc.res0 = sqrt(rr) / bn;
         bool guess_resid = false;   // lev[0].b32 holds the residual of the accepted guess
         double rr_guess = 0.0;
-        if (mode == TB_POIS_LEARNED_IR && c->n_hist >= 2 && c->nb_hist >= 2) {
+        if (mode == TB_POIS_LEARNED_IR && c->n_hist >= 1 && c->gram_n >= 1) {
             double resg = 0.0;
             int ndir = 0;
             if (learned_guess(c, P, x, b, bn, resg, rr_guess, ndir)) return -1;
@@ -1897,7 +1965,7 @@ extern "C" int tb_poisson_solve(tb_ctx *c, tb_solve_stats *st) {
     }
     // every path above leaves x pinned (shift_corner last, or all zero)
     c->phi_pinned = true;
-    // history of solutions (and, in learned mode, right-hand sides) for the
+    // history of solutions for the
     // initial guess and the network's inputs, newest first: the pointers
     // rotate, one copy per solve; kept only where something reads it
     if (c->cfg.pois_mode == TB_POIS_LEARNED_IR || c->net || c->net_record) {
@@ -1906,17 +1974,12 @@ extern "C" int tb_poisson_solve(tb_ctx *c, tb_solve_stats *st) {
         c->phi_hist[0] = oldest;
         TB_CHECK(c, hipMemcpyAsync(c->phi_hist[0], x, sizeof(double) * c->d.ntot, hipMemcpyDeviceToDevice, c->s));
         if (c->n_hist < GUESS_HIST) c->n_hist++;
+        // the last residual pass of every solve above ran on the final x
+        // (x = 0 when b = 0) and left x . b in S_RES + 2
+        if (c->cfg.pois_mode == TB_POIS_LEARNED_IR) gram_push(c, bn == 0.0 ? 0.0 : c->hscal[S_RES + 2]);
+        else c->gram_n = c->guess_nr = 0;
     } else {
-        c->n_hist = 0;
-    }
-    if (c->cfg.pois_mode == TB_POIS_LEARNED_IR) {
-        double *oldest = c->b_hist[GUESS_HIST - 1];
-        for (int k = GUESS_HIST - 1; k > 0; k--) c->b_hist[k] = c->b_hist[k - 1];
-        c->b_hist[0] = oldest;
-        TB_CHECK(c, hipMemcpyAsync(c->b_hist[0], b, sizeof(double) * c->d.ntot, hipMemcpyDeviceToDevice, c->s));
-        if (c->nb_hist < GUESS_HIST) c->nb_hist++;
-    } else {
-        c->nb_hist = 0;
+        c->n_hist = c->gram_n = c->guess_nr = 0;
     }
     if (c->net_record && c->n_hist >= 3 && tb_net_record_one(c)) return -1;
     TB_CHECK(c, hipEventRecord(c->ev1, c->s));

@@ -59,8 +59,8 @@ def generate(net, case, runs, steps, seed, log=print):
 def compare_iterations(net, case, steps, seed, runs=1, log=print):
     """Poisson iterations and initial residuals on held-out runs (same seeds
     for every variant): 'previous' starts from phi_{n-1} (the reference's
-    guess), 'subspace' from the Galerkin combination of phi_{n-1}, phi_{n-2}
-    (learned mode without a network), 'learned' adds the network's
+    guess), 'subspace' from the Galerkin combination of the last solutions
+    (learned mode without a network: up to eight), 'learned' adds the network's
     correction as a third direction."""
     res = {}
     variants = (("previous", abi.POIS_MIXED_IR, None), ("subspace", abi.POIS_LEARNED_IR, None),
