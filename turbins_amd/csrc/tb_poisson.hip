@@ -388,13 +388,24 @@ __global__ void __launch_bounds__(256) k_resid_restrict_011(LV<T> F, LV<T> C, co
 #pragma unroll
     for (int r6 = 0; r6 < 6; r6++) roff[r6] = clampi(2 * Jc - 2 + r6, 0, ny - 1) * F.px + i;
     // planes kc-1 .. kc+2 of x and kc, kc+1 of b in registers: the loads of
-    // a plane are issued two steps before its residual needs them
-    T xm[6], xc[6], xp[6], xq[6], bc0[4], bc1[4];
+    // a plane are issued two steps before its residual needs them.  The
+    // wave's edge lanes also prefetch the x neighbour across the wave edge
+    // (west for lane 0, east for lane 63) of rows 2J-1 .. 2J+2 of planes
+    // kc .. kc+2, so no load waits inside the residual.  FP32 only: the
+    // FP64 form runs faster with the registers left to occupancy and loads
+    // those values where it uses them.
+    constexpr bool PF = sizeof(T) == 4;
+    T xm[6], xc[6], xp[6], xq[6], bc0[4], bc1[4], ec[4], ep[4], eq[4];
+    const int eoff = !PF ? 0 : (lane == 0 && i > 0) ? -1 : ((lane == 63 && i < nx - 1) ? 1 : 0);
     int kc = 0;   // fine plane held in xc
-    auto load_plane = [&](int kf, T *dst) {
+    auto load_plane = [&](int kf, T *dst, T *edst) {
         if (kf < 0 || kf >= nz) return;
 #pragma unroll
         for (int r6 = 0; r6 < 6; r6++) dst[r6] = x[kf * pl + roff[r6]];
+        if (eoff != 0) {
+#pragma unroll
+            for (int tj = 0; tj < 4; tj++) edst[tj] = x[kf * pl + roff[1 + tj] + eoff];
+        }
     };
     auto load_b = [&](int kf, T *dst) {
         if (kf < 0 || kf >= nz) return;
@@ -405,9 +416,9 @@ __global__ void __launch_bounds__(256) k_resid_restrict_011(LV<T> F, LV<T> C, co
 #pragma unroll
         for (int r6 = 0; r6 < 6; r6++) { xm[r6] = xc[r6]; xc[r6] = xp[r6]; xp[r6] = xq[r6]; }
 #pragma unroll
-        for (int tj = 0; tj < 4; tj++) bc0[tj] = bc1[tj];
+        for (int tj = 0; tj < 4; tj++) { bc0[tj] = bc1[tj]; ec[tj] = ep[tj]; ep[tj] = eq[tj]; }
         kc++;
-        load_plane(kc + 2, xq);
+        load_plane(kc + 2, xq, eq);
         load_b(kc + 1, bc1);
     };
     // y-restricted residual of fine plane kc (absent planes are not read)
@@ -418,14 +429,15 @@ __global__ void __launch_bounds__(256) k_resid_restrict_011(LV<T> F, LV<T> C, co
         for (int tj = 0; tj < 4; tj++) {
             const int jf = clampi(2 * Jc - 1 + tj, 0, ny - 1);
             const int r6 = jf - (2 * Jc - 2);
-            const int id = kc * pl + jf * F.px + i;
             const bool hw = i > 0, he = i < nx - 1, hs = jf > 0, hn = jf < ny - 1;
             const T xcc = xc[r6];
             // x neighbours from the adjacent lanes (a wave is 64 consecutive
-            // i); the wave's edge lanes load theirs
+            // i); the wave's edge lanes take their prefetched values (ec[tj]
+            // is row roff[1 + tj] = jf)
             T xw = __shfl_up(xcc, 1, 64), xe = __shfl_down(xcc, 1, 64);
-            if (lane == 0) xw = hw ? x[id - 1] : xcc;
-            if (lane == 63) xe = he ? x[id + 1] : xcc;
+            const int id = kc * pl + jf * F.px + i;
+            if (lane == 0) xw = hw ? (PF ? ec[tj] : x[id - 1]) : xcc;
+            if (lane == 63) xe = he ? (PF ? ec[tj] : x[id + 1]) : xcc;
             xw = hw ? xw : xcc;
             xe = he ? xe : xcc;
             const T xs = hs ? xc[r6 - 1] : xcc, xn = hn ? xc[r6 + 1] : xcc;
@@ -437,10 +449,11 @@ __global__ void __launch_bounds__(256) k_resid_restrict_011(LV<T> F, LV<T> C, co
         return sum;
     };
     kc = 2 * K0 - 1 < 0 ? 0 : 2 * K0 - 1;
-    load_plane(kc - 1, xm);
-    load_plane(kc, xc);
-    load_plane(kc + 1, xp);
-    load_plane(kc + 2, xq);
+    T edummy[4];
+    load_plane(kc - 1, xm, edummy);
+    load_plane(kc, xc, ec);
+    load_plane(kc + 1, xp, ep);
+    load_plane(kc + 2, xq, eq);
     load_b(kc, bc0);
     load_b(kc + 1, bc1);
     T Rm1 = plane_sum();
@@ -1324,12 +1337,21 @@ __global__ void __launch_bounds__(256) k_shift(Dims d, double *__restrict__ x, c
     x[id] = bnd ? 0.0 : x[id] - s[S_CORNER];
 }
 
-// x += (double) e32
-__global__ void __launch_bounds__(256) k_add32(Dims d, double *__restrict__ x, const float *__restrict__ e, Tiles t) {
+// the corner value after x += e: s[S_CORNER] = x[corner] + (double) e[corner]
+__global__ void k_corner_after_add(const double *__restrict__ x, const float *__restrict__ e, long corner,
+                                   double *__restrict__ s) {
+    if (threadIdx.x == 0) s[S_CORNER] = x[corner] + (double)e[corner];
+}
+
+// x += (double) e32 followed by the pinned shift, in one pass over the cells:
+// x = (x + e) - s[S_CORNER] (the same values as adding e and then
+// k_shift; the boundary entries stay 0)
+__global__ void __launch_bounds__(256) k_add32_pin(Dims d, double *__restrict__ x, const float *__restrict__ e,
+                                                   const double *__restrict__ s, Tiles t) {
     int i, j, k;
     if (!tile_point(t, i, j, k)) return;
     long id = gidx(d, i, j, k);
-    x[id] += (double)e[id];
+    x[id] = (x[id] + (double)e[id]) - s[S_CORNER];
 }
 
 static int read_s(tb_ctx *c, int first, int count) { return tb_read_scalars(c, first, count); }
@@ -1514,7 +1536,11 @@ __global__ void __launch_bounds__(PT) k_cg32_p(long n4, const float4 *__restrict
 // Inner solve A e = r on level 0 by FP32 CG preconditioned with FP32
 // V-cycles, from e = 0, until ||r|| <= tol_rel * ||r0|| or maxit iterations.
 // r is lev[0].b32 (updated in place), z is lev[0].x32, e is cg32[2].
-static int cg32_inner(tb_ctx *c, const KTiles &kt, double r0n, double tol_rel, int maxit, int &its, int &vcy) {
+// The first `skip` iterations run without reading their residual back (each
+// read-back stalls the stream for a host round trip); the caller takes skip
+// from the iterations an earlier solve needed.
+static int cg32_inner(tb_ctx *c, const KTiles &kt, double r0n, double tol_rel, int maxit, int skip, int &its,
+                      int &vcy) {
     Level &L0 = c->lev[0];
     const long n4 = L0.ntot / 4;
     float *r = L0.b32, *z = L0.x32, *p = c->cg32[0], *q = c->cg32[1], *e = c->cg32[2];
@@ -1541,10 +1567,13 @@ static int cg32_inner(tb_ctx *c, const KTiles &kt, double r0n, double tol_rel, i
         if (tb_reduce_finish(c, PB, 1, S_RR)) return -1;
         inner++;
         its++;
-        if (read_s(c, S_RR, 1)) return -1;
-        const double rel = sqrt(c->hscal[S_RR]) / r0n;
-        if (!(rel == rel)) { c->err = "Poisson FP32 CG produced NaN"; return -1; }
-        if (rel <= tol_rel || inner >= maxit) break;
+        if (inner >= maxit) break;
+        if (inner > skip) {
+            if (read_s(c, S_RR, 1)) return -1;
+            const double rel = sqrt(c->hscal[S_RR]) / r0n;
+            if (!(rel == rel)) { c->err = "Poisson FP32 CG produced NaN"; return -1; }
+            if (rel <= tol_rel) break;
+        }
         if (tb_vcycle32(c)) return -1;
         vcy++;
         hipLaunchKernelGGL(k_dot32, dim3(PB), dim3(PT), 0, c->s, n4, (const float4 *)r, (const float4 *)z, c->red);
@@ -1576,7 +1605,11 @@ extern "C" int tb_set_mixed_inner(tb_ctx *c, double reduction_floor, int maxit) 
 
 // rr_known: x is already pinned (corner exactly 0) and lev[0].b32 holds its
 // FP32 residual, of squared norm *rr_known; the first shift and residual are
-// skipped.
+// skipped.  Every refinement pass adds the correction and pins x in one
+// kernel, so its residual (Neumann rows, with x[corner] = 0 the same vector
+// as the pinned one) is the true residual of the result: no further pass
+// follows the last one.  On return x is pinned, res is its pinned relative
+// residual and the last residual pass ran on x.
 static int mixed_ir(tb_ctx *c, const PoisCtx &P, double *x, const double *b, double bn, double rtol, int maxit,
                     int &its, int &vcy, double &res, bool &stalled, const double *rr_known) {
     double *r = nullptr;   // only the FP32 copy of the residual and its norms are needed
@@ -1598,18 +1631,22 @@ static int mixed_ir(tb_ctx *c, const PoisCtx &P, double *x, const double *b, dou
         double want = 0.5 * rtol / res;
         if (want < c->ir_floor) want = c->ir_floor;
         int room = maxit - its;
-        if (cg32_inner(c, kt0, sqrt(rr), want, room < c->ir_maxit ? room : c->ir_maxit, its, vcy)) return -1;
-        hipLaunchKernelGGL(k_add32, tiles_grid(P.cells), tiles_block(), 0, c->s, c->d, x, (const float *)c->cg32[2],
-                           P.cells);
+        // the first pass skips the tests of all but the last two iterations
+        // the previous solve's first pass needed
+        const int skip = (its == 0 && c->ir_first_its > 2) ? c->ir_first_its - 2 : 0;
+        const int its_before = its;
+        if (cg32_inner(c, kt0, sqrt(rr), want, room < c->ir_maxit ? room : c->ir_maxit, skip, its, vcy)) return -1;
+        if (its_before == 0) c->ir_first_its = its;
+        hipLaunchKernelGGL(k_corner_after_add, dim3(1), dim3(64), 0, c->s, (const double *)x,
+                           (const float *)c->cg32[2], P.corner, c->scal);
+        hipLaunchKernelGGL(k_add32_pin, tiles_grid(P.cells), tiles_block(), 0, c->s, c->d, x,
+                           (const float *)c->cg32[2], (const double *)c->scal, P.cells);
         TB_LAUNCH_CHECK(c);
         if (residual(c, P, true, x, b, r, L0.b32, rr, bb)) return -1;
         res = sqrt(rr) / bn;
         if (!(res == res)) { c->err = "Poisson IR produced NaN"; return -1; }
         if (res > 0.5 * prev) { stalled = true; break; }
     }
-    if (shift_corner(c, P, x)) return -1;
-    if (residual(c, P, false, x, b, r, nullptr, rr, bb)) return -1;
-    res = sqrt(rr) / bn;
     return 0;
 }
 
@@ -1688,13 +1725,36 @@ struct Coefs {
     double a[GUESS_DIRS];
 };
 
-// x = sum_{u < m} a_u g_u (+ a_m d when d is given)
-__global__ void k_combine(long n, double *__restrict__ x, Dirs D, int m, const double *__restrict__ d, Coefs C) {
+// x = sum_{u < M} a_u g_u (+ a_M d when d is given); M is a template
+// argument so that all loads of a point are in flight together
+template <int M>
+__global__ void __launch_bounds__(PT) k_combine(long n, double *__restrict__ x, Dirs D,
+                                                const double *__restrict__ d, Coefs C) {
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        double g[M];
+#pragma unroll
+        for (int u = 0; u < M; u++) g[u] = D.g[u][i];
+        const double dv = d ? d[i] : 0.0;
         double s = 0.0;
-        for (int u = 0; u < m; u++) s += C.a[u] * D.g[u][i];
-        if (d) s += C.a[m] * d[i];
+#pragma unroll
+        for (int u = 0; u < M; u++) s += C.a[u] * g[u];
+        if (d) s += C.a[M] * dv;
         x[i] = s;
+    }
+}
+
+static void launch_combine(tb_ctx *c, long n, double *x, const Dirs &D, int m, const double *d, const Coefs &C) {
+    switch (m) {
+    case 1: hipLaunchKernelGGL(k_combine<1>, dim3(PB), dim3(PT), 0, c->s, n, x, D, d, C); break;
+    case 2: hipLaunchKernelGGL(k_combine<2>, dim3(PB), dim3(PT), 0, c->s, n, x, D, d, C); break;
+    case 3: hipLaunchKernelGGL(k_combine<3>, dim3(PB), dim3(PT), 0, c->s, n, x, D, d, C); break;
+    case 4: hipLaunchKernelGGL(k_combine<4>, dim3(PB), dim3(PT), 0, c->s, n, x, D, d, C); break;
+    case 5: hipLaunchKernelGGL(k_combine<5>, dim3(PB), dim3(PT), 0, c->s, n, x, D, d, C); break;
+    case 6: hipLaunchKernelGGL(k_combine<6>, dim3(PB), dim3(PT), 0, c->s, n, x, D, d, C); break;
+    case 7: hipLaunchKernelGGL(k_combine<7>, dim3(PB), dim3(PT), 0, c->s, n, x, D, d, C); break;
+    case 8: hipLaunchKernelGGL(k_combine<8>, dim3(PB), dim3(PT), 0, c->s, n, x, D, d, C); break;
+    case 9: hipLaunchKernelGGL(k_combine<9>, dim3(PB), dim3(PT), 0, c->s, n, x, D, d, C); break;
+    default: hipLaunchKernelGGL(k_combine<10>, dim3(PB), dim3(PT), 0, c->s, n, x, D, d, C); break;
     }
 }
 
@@ -1731,9 +1791,10 @@ static void launch_dots(tb_ctx *c, long n, const Dirs &D, const double *b, const
 }
 
 // On return with ndir > 0, x is pinned and lev[0].b32 holds the FP32 copy of
-// its residual, whose squared norm is rrg.  c->guess_r / guess_nr receive the
-// products phi_k . b for the Galerkin matrix of the next solve.
-static int learned_guess(tb_ctx *c, const PoisCtx &P, double *x, const double *b, double bn, double &resg, double &rrg,
+// its residual, whose squared norm is rrg (and that of b bbg); with ndir = 0
+// x is untouched.  c->guess_r / guess_nr receive the products phi_k . b for
+// the Galerkin matrix of the next solve.
+static int learned_guess(tb_ctx *c, const PoisCtx &P, double *x, const double *b, double &rrg, double &bbg,
                          int &ndir) {
     const long n = c->d.ntot;
     double *dd = c->kv[5], *q = c->kv[6];
@@ -1811,21 +1872,18 @@ static int learned_guess(tb_ctx *c, const PoisCtx &P, double *x, const double *b
     }
     Coefs C;
     for (int k = 0; k < GUESS_DIRS; k++) C.a[k] = k < ndir ? a[k] : 0.0;
-    hipLaunchKernelGGL(k_combine, dim3(PB), dim3(PT), 0, c->s, n, x, D, with_d ? H : ndir,
-                       with_d ? (const double *)dd : nullptr, C);
+    launch_combine(c, n, x, D, with_d ? H : ndir, with_d ? (const double *)dd : nullptr, C);
     TB_LAUNCH_CHECK(c);
-    double bb;
-    // Every stored solution is pinned (each solve ends with shift_corner:
-    // corner exactly 0, half-cell planes 0), so a combination of them is
-    // too and the shift would subtract exactly 0; only the network's
-    // direction needs it.
+    // Every stored solution is pinned (each solve leaves its x with corner
+    // exactly 0 and half-cell planes 0), so a combination of them is too and
+    // the shift would subtract exactly 0; only the network's direction needs
+    // it.
     if (with_d && shift_corner(c, P, x)) return -1;
     // With x[corner] = 0 the pinned and the Neumann residual are the same
     // vector (the rows differ only in the corner's own coefficient): one
-    // pass gives the guess's residual norm and the FP32 residual the mixed
-    // solve starts from.
-    if (residual(c, P, true, x, b, nullptr, c->lev[0].b32, rrg, bb)) return -1;
-    resg = sqrt(rrg) / bn;
+    // pass gives the guess's residual norm, the norm of b and the FP32
+    // residual the mixed solve starts from.
+    if (residual(c, P, true, x, b, nullptr, c->lev[0].b32, rrg, bbg)) return -1;
     return 0;
 }
 
@@ -1906,50 +1964,61 @@ extern "C" int tb_poisson_solve(tb_ctx *c, tb_solve_stats *st) {
     double rtol = c->cfg.pois_rtol;
     int maxit = c->cfg.pois_maxit;
     TB_CHECK(c, hipEventRecord(c->ev0, c->s));
-    double rr, bb;
-    // the field left by the previous solve is pinned already (the shift
-    // would subtract exactly 0)
-    if (!c->phi_pinned && shift_corner(c, P, x)) return -1;
-    c->phi_pinned = false;
-    // this residual is also the first one of the method that follows: FP64
-    // PCG takes r (kv[0]), the mixed solve its FP32 copy (x is pinned, so the
-    // Neumann and the pinned residual are the same vector); the learned mode
-    // replaces x by its guess and needs the norms only
+    double rr = 0.0, bb = 0.0;
     const int mode = c->cfg.pois_mode;
-    double *r_top = (mode == TB_POIS_FP64_PCG) ? c->kv[0] : nullptr;
-    float *r32_top = (mode == TB_POIS_MIXED_IR) ? c->lev[0].b32 : nullptr;
-    if (residual(c, P, mode == TB_POIS_MIXED_IR, x, b, r_top, r32_top, rr, bb)) return -1;
-    const double rr_top = rr;
-    double bn = sqrt(bb);
     tb_solve_stats loc;
     memset(&loc, 0, sizeof(loc));
     int its = 0, vcy = 0;
     double res = 0.0;
+    // Learned mode with a stored Galerkin matrix: the guess replaces x first;
+    // its residual pass also gives ||b||, and the guess is kept when it is
+    // better than the zero vector (relative residual < 1; the previous
+    // solution, the only other candidate, is in its span and typically
+    // worse than zero, this phi being an increment).  Otherwise x returns
+    // to the previous solution, measured as in the other modes.
+    bool guess_resid = false;   // lev[0].b32 holds the residual of the accepted guess
+    double rr_guess = 0.0;
+    if (mode == TB_POIS_LEARNED_IR && c->n_hist >= 1 && c->gram_n >= 1) {
+        double bbg = 0.0;
+        int ndir = 0;
+        if (learned_guess(c, P, x, b, rr_guess, bbg, ndir)) return -1;
+        if (ndir > 0 && bbg > 0.0 && rr_guess < bbg) {
+            guess_resid = true;
+            bb = bbg;
+            rr = rr_guess;
+        } else {
+            loc.fallback = 2;
+            // keep the previous solution (still in phi_hist[0], pinned)
+            if (ndir > 0)
+                TB_CHECK(c, hipMemcpyAsync(x, c->phi_hist[0], sizeof(double) * c->d.ntot, hipMemcpyDeviceToDevice,
+                                           c->s));
+        }
+    }
+    double rr_top = 0.0;
+    if (!guess_resid) {
+        // the field left by the previous solve is pinned already (the shift
+        // would subtract exactly 0)
+        if (!c->phi_pinned && shift_corner(c, P, x)) return -1;
+        // this residual is also the first one of the method that follows:
+        // FP64 PCG takes r (kv[0]), the mixed solve its FP32 copy (x is
+        // pinned, so the Neumann and the pinned residual are the same vector)
+        double *r_top = (mode == TB_POIS_FP64_PCG) ? c->kv[0] : nullptr;
+        float *r32_top = (mode != TB_POIS_FP64_PCG) ? c->lev[0].b32 : nullptr;
+        if (residual(c, P, mode != TB_POIS_FP64_PCG, x, b, r_top, r32_top, rr, bb)) return -1;
+        rr_top = rr;
+    }
+    c->phi_pinned = false;
+    double bn = sqrt(bb);
     if (bn == 0.0) {
         TB_CHECK(c, hipMemsetAsync(x, 0, sizeof(double) * c->d.ntot, c->s));
         loc.converged = 1;
     } else {
         loc.res0 = sqrt(rr) / bn;
-        bool guess_resid = false;   // lev[0].b32 holds the residual of the accepted guess
-        double rr_guess = 0.0;
-        if (mode == TB_POIS_LEARNED_IR && c->n_hist >= 1 && c->gram_n >= 1) {
-            double resg = 0.0;
-            int ndir = 0;
-            if (learned_guess(c, P, x, b, bn, resg, rr_guess, ndir)) return -1;
-            if (ndir > 0 && resg < loc.res0) {
-                loc.res0 = resg;
-                guess_resid = true;
-            } else {
-                // keep the previous solution (still in phi_hist[0])
-                TB_CHECK(c, hipMemcpyAsync(x, c->phi_hist[0], sizeof(double) * c->d.ntot, hipMemcpyDeviceToDevice, c->s));
-                loc.fallback = 2;
-            }
-        }
         if (mode == TB_POIS_FP64_PCG) {
             if (pcg64(c, P, x, b, bn, rtol, maxit, its, vcy, res, &rr_top)) return -1;
         } else {
             bool stalled = false;
-            const double *rr_known = guess_resid ? &rr_guess : (mode == TB_POIS_MIXED_IR ? &rr_top : nullptr);
+            const double *rr_known = guess_resid ? &rr_guess : &rr_top;
             if (mixed_ir(c, P, x, b, bn, rtol, maxit, its, vcy, res, stalled, rr_known)) return -1;
             if (res > rtol) {
                 loc.fallback |= 1;   // plain FP64 GMG-PCG takes over
@@ -1963,10 +2032,11 @@ extern "C" int tb_poisson_solve(tb_ctx *c, tb_solve_stats *st) {
             if (judge_network(c, x, b, loc.res0, res, its, tb_elapsed_ms(c))) return -1;
         }
     }
-    // every path above leaves x pinned (shift_corner last, or all zero)
+    // every path above leaves x pinned (by its last shift or correction
+    // pass, or all zero)
     c->phi_pinned = true;
-    // history of solutions for the
-    // initial guess and the network's inputs, newest first: the pointers
+    // history of solutions for the initial guess and the network's inputs,
+    // newest first: the pointers
     // rotate, one copy per solve; kept only where something reads it
     if (c->cfg.pois_mode == TB_POIS_LEARNED_IR || c->net || c->net_record) {
         double *oldest = c->phi_hist[GUESS_HIST - 1];
