@@ -421,21 +421,26 @@ __global__ void __launch_bounds__(256) k_resid_restrict_011(LV<T> F, LV<T> C, co
         load_plane(kc + 2, xq, eq);
         load_b(kc + 1, bc1);
     };
-    // y-restricted residual of fine plane kc (absent planes are not read)
+    // y-restricted residual of fine plane kc (absent planes are not read).
+    // Register row r6 = 1 + tj holds fine row 2J-1+tj (clamped), so every
+    // register index is static; the rows 2J-1 and 2J+2 that fall outside
+    // the level reflect onto rows 2J and 2J+1, whose residuals they repeat.
+    const bool lo_wall = 2 * Jc - 1 < 0, hi_wall = 2 * Jc + 2 > ny - 1;
     auto plane_sum = [&]() -> T {
         const bool hb = kc > 0, hf = kc < nz - 1;
-        T sum = (T)0;
+        const bool hw = i > 0, he = i < nx - 1;
+        T rs[4];
 #pragma unroll
         for (int tj = 0; tj < 4; tj++) {
-            const int jf = clampi(2 * Jc - 1 + tj, 0, ny - 1);
-            const int r6 = jf - (2 * Jc - 2);
-            const bool hw = i > 0, he = i < nx - 1, hs = jf > 0, hn = jf < ny - 1;
+            const int r6 = 1 + tj;
+            const int jr = 2 * Jc - 1 + tj;   // unclamped row (the wall cases are replaced below)
+            const bool hs = jr > 0, hn = jr < ny - 1;
             const T xcc = xc[r6];
             // x neighbours from the adjacent lanes (a wave is 64 consecutive
             // i); the wave's edge lanes take their prefetched values (ec[tj]
-            // is row roff[1 + tj] = jf)
+            // is row roff[1 + tj])
             T xw = __shfl_up(xcc, 1, 64), xe = __shfl_down(xcc, 1, 64);
-            const int id = kc * pl + jf * F.px + i;
+            const int id = kc * pl + roff[r6];
             if (lane == 0) xw = hw ? (PF ? ec[tj] : x[id - 1]) : xcc;
             if (lane == 63) xe = he ? (PF ? ec[tj] : x[id + 1]) : xcc;
             xw = hw ? xw : xcc;
@@ -444,8 +449,13 @@ __global__ void __launch_bounds__(256) k_resid_restrict_011(LV<T> F, LV<T> C, co
             const T xb = hb ? xm[r6] : xcc, xf = hf ? xp[r6] : xcc;
             const T sr = F.cx * (xcc - xw) + F.cx * (xcc - xe) + F.cy * (xcc - xs) + F.cy * (xcc - xn) +
                          F.cz * (xcc - xb) + F.cz * (xcc - xf);
-            sum += ((tj == 0 || tj == 3) ? W0 : W1) * (bc0[tj] - sr);   // bc0[tj]: b of row jf
+            rs[tj] = bc0[tj] - sr;   // bc0[tj]: b of the same row
         }
+        if (lo_wall) rs[0] = rs[1];
+        if (hi_wall) rs[3] = rs[2];
+        T sum = (T)0;
+#pragma unroll
+        for (int tj = 0; tj < 4; tj++) sum += ((tj == 0 || tj == 3) ? W0 : W1) * rs[tj];
         return sum;
     };
     kc = 2 * K0 - 1 < 0 ? 0 : 2 * K0 - 1;
@@ -459,6 +469,7 @@ __global__ void __launch_bounds__(256) k_resid_restrict_011(LV<T> F, LV<T> C, co
     T Rm1 = plane_sum();
     if (kc < 2 * K0) step();
     T R0 = plane_sum();
+#pragma unroll 1
     for (int K = K0; K < K1; K++) {
         step();
         const T R1 = plane_sum();
