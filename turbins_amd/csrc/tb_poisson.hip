@@ -554,7 +554,7 @@ __global__ void __launch_bounds__(256) k_prolong(LV<T> F, LV<T> C, int cx, int c
 // rings of three planes of xo (halo 2) and of first-colour values (halo 1),
 // the next plane's xo and b fetched into registers one plane ahead, two
 // barriers per plane (plane m-2 is stored after the first).
-constexpr int PAIR_TX = 64, PAIR_TY = 8, PAIR_TK = 16;
+constexpr int PAIR_TX = 64, PAIR_TY = 8;
 constexpr int PAIR_OX = PAIR_TX + 4, PAIR_OY = PAIR_TY + 4;
 constexpr int PAIR_HX = PAIR_TX + 2, PAIR_HY = PAIR_TY + 2;
 constexpr int PAIR_THREADS = 256;
@@ -564,19 +564,23 @@ constexpr int PAIR_RED = PAIR_HY * (PAIR_HX / 2);
 constexpr int PAIR_RED_PER = (PAIR_RED + PAIR_THREADS - 1) / PAIR_THREADS;
 static_assert(PAIR_TX * PAIR_TY == 2 * PAIR_THREADS, "one second-colour cell per thread");
 // Levels at least this large use the fused pair (smaller ones have too
-// few tiles to hide the plane-by-plane chain of a block)
-constexpr long PAIR_LEVEL_CELLS = 8L << 20;
+// few tiles to hide the plane-by-plane chain of a block); a block walks 16
+// planes on levels of 8 M cells or more, 8 below (twice the blocks)
+constexpr long PAIR_LEVEL_CELLS = 4L << 20;
+
+static int pair_chunk(const Level &L) { return (long)L.nx * L.ny * L.nz >= (8L << 20) ? 16 : 8; }
 
 struct PairTiles {
-    int tx, ty, kc;
+    int tx, ty, kc, tk;   // tk: planes per block (the chain a block walks)
     int n, per_xcd;
 };
 
-static PairTiles make_pair_tiles(int nx, int ny, int nz) {
+static PairTiles make_pair_tiles(int nx, int ny, int nz, int tk) {
     PairTiles t;
     t.tx = (nx + PAIR_TX - 1) / PAIR_TX;
     t.ty = (ny + PAIR_TY - 1) / PAIR_TY;
-    t.kc = (nz + PAIR_TK - 1) / PAIR_TK;
+    t.tk = tk;
+    t.kc = (nz + tk - 1) / tk;
     t.n = t.tx * t.ty * t.kc;
     t.per_xcd = (t.n + NXCD - 1) / NXCD;
     return t;
@@ -594,8 +598,8 @@ __global__ void __launch_bounds__(PAIR_THREADS) k_rbgs_pair(LV<T> L, const T *__
     if (lt >= P.n) return;                                  // whole block
     const int tx = lt % P.tx, rest = lt / P.tx;
     const int i0 = tx * PAIR_TX, j0 = (rest % P.ty) * PAIR_TY;
-    const int k0 = (rest / P.ty) * PAIR_TK;
-    const int k1 = k0 + PAIR_TK < nz ? k0 + PAIR_TK : nz;
+    const int k0 = (rest / P.ty) * P.tk;
+    const int k1 = k0 + P.tk < nz ? k0 + P.tk : nz;
     const int t = threadIdx.x;
 
     for (int e = t; e < PAIR_ORING * PAIR_OY * PAIR_OX; e += PAIR_THREADS) (&so[0][0])[e] = (T)0;
@@ -1169,7 +1173,7 @@ static int vcycle_level(tb_ctx *c, int l) {
     const bool pairs = B.a != nullptr;
     T *cur = B.x;
     if (pairs) {
-        const PairTiles P = make_pair_tiles(L.nx, L.ny, L.nz);
+        const PairTiles P = make_pair_tiles(L.nx, L.ny, L.nz, pair_chunk(L));
         const dim3 grid(P.per_xcd * NXCD), blk(PAIR_THREADS);
         auto pair = [&](int first, bool zero) {
             T *dst = zero ? B.x : (cur == B.x ? B.a : B.x);
