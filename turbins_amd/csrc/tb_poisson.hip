@@ -587,7 +587,7 @@ static PairTiles make_pair_tiles(int nx, int ny, int nz, int tk) {
 }
 
 template <typename T, int F, bool ZERO>
-__global__ void __launch_bounds__(PAIR_THREADS) k_rbgs_pair(LV<T> L, const T *__restrict__ xo, T *__restrict__ xn,
+__global__ void __launch_bounds__(PAIR_THREADS, sizeof(T) == 4 ? 8 : 4) k_rbgs_pair(LV<T> L, const T *__restrict__ xo, T *__restrict__ xn,
                                                             const T *__restrict__ b, PairTiles P) {
     __shared__ T so[PAIR_ORING][PAIR_OY * PAIR_OX];
     __shared__ T sr[3][PAIR_HY * PAIR_HX];
