@@ -286,6 +286,9 @@ struct tb_ctx {
     double ir_floor = 1e-5;      // smallest residual reduction asked of one inner FP32 solve
     int ir_maxit = 40;           // inner iterations per refinement pass
     int ir_first_its = 0;        // iterations of the last mixed solve's first pass
+    // V-cycles (FP32, FP64) whose last finest-level pair leaves the partial
+    // sums of r . z in red[0 .. vc_dot_blocks); 0: that level has no pairs
+    int vc_dot_blocks[2] = {0, 0};
     int mg_pre0[2] = {0, 0};     // sweeps on the finest level per V-cycle side (FP32, FP64);
     int mg_post0[2] = {0, 0};    // 0: cfg.mg_pre / cfg.mg_post as on the coarser levels
     int coarse_n = 0;

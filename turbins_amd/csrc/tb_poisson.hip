@@ -586,16 +586,24 @@ static PairTiles make_pair_tiles(int nx, int ny, int nz, int tk) {
     return t;
 }
 
-template <typename T, int F, bool ZERO>
-__global__ void __launch_bounds__(PAIR_THREADS, sizeof(T) == 4 ? 8 : 4) k_rbgs_pair(LV<T> L, const T *__restrict__ xo, T *__restrict__ xn,
-                                                            const T *__restrict__ b, PairTiles P) {
+// DOT: the block's partial sum of b . xn over its own cells (the final
+// values of both colours) goes to part[blockIdx.x] -- r . z of a V-cycle
+// whose last sweep pair this is.
+template <typename T, int F, bool ZERO, bool DOT>
+__global__ void __launch_bounds__(PAIR_THREADS, sizeof(T) == 4 ? 8 : 4)
+    k_rbgs_pair(LV<T> L, const T *__restrict__ xo, T *__restrict__ xn, const T *__restrict__ b, PairTiles P,
+                double *__restrict__ part) {
     __shared__ T so[PAIR_ORING][PAIR_OY * PAIR_OX];
     __shared__ T sr[3][PAIR_HY * PAIR_HX];
     const int nx = L.nx, ny = L.ny, nz = L.nz, px = L.px;
     const int plane = L.px * L.py;
     const int bid = blockIdx.x;
     const int lt = (bid % NXCD) * P.per_xcd + bid / NXCD;
-    if (lt >= P.n) return;                                  // whole block
+    if (lt >= P.n) {                                        // whole block
+        if (DOT && threadIdx.x == 0) part[bid] = 0.0;
+        return;
+    }
+    double dacc[1] = {0.0};
     const int tx = lt % P.tx, rest = lt / P.tx;
     const int i0 = tx * PAIR_TX, j0 = (rest % P.ty) * PAIR_TY;
     const int k0 = (rest / P.ty) * P.tk;
@@ -698,8 +706,11 @@ __global__ void __launch_bounds__(PAIR_THREADS, sizeof(T) == 4 ? 8 : 4) k_rbgs_p
                 const int cc = 2 * rc2[u] + ((F + 1 + j + m) & 1);
                 const int i = i0 - 1 + cc;
                 const int o = (r + 1) * PAIR_OX + cc + 1;
-                R[r * PAIR_HX + cc] = rb_value(L, i, j, m, rb[u], S[o - 1], S[o + 1], S[o - PAIR_OX], S[o + PAIR_OX],
-                                               Sb[o], Sf[o]);
+                const T v = rb_value(L, i, j, m, rb[u], S[o - 1], S[o + 1], S[o - PAIR_OX], S[o + PAIR_OX], Sb[o],
+                                     Sf[o]);
+                R[r * PAIR_HX + cc] = v;
+                if (DOT && m >= k0 && m < k1 && i >= i0 && i < i0 + PAIR_TX && j >= j0 && j < j0 + PAIR_TY)
+                    dacc[0] += (double)v * (double)rb[u];
             }
         }
         __syncthreads();
@@ -710,7 +721,9 @@ __global__ void __launch_bounds__(PAIR_THREADS, sizeof(T) == 4 ? 8 : 4) k_rbgs_p
             T *R = sr[k % 3];
             const T *Rb = sr[(k + 2) % 3], *Rf = sr[(k + 1) % 3];
             const int h = (br + 1) * PAIR_HX + cc + 1;
-            R[h] = rb_value(L, i, bj, k, bb, R[h - 1], R[h + 1], R[h - PAIR_HX], R[h + PAIR_HX], Rb[h], Rf[h]);
+            const T v = rb_value(L, i, bj, k, bb, R[h - 1], R[h + 1], R[h - PAIR_HX], R[h + PAIR_HX], Rb[h], Rf[h]);
+            R[h] = v;
+            if (DOT) dacc[0] += (double)v * (double)bb;
         }
         // no barrier: the next staging writes the xo slot of plane m - 1,
         // which the second-colour pass does not read
@@ -720,6 +733,11 @@ __global__ void __launch_bounds__(PAIR_THREADS, sizeof(T) == 4 ? 8 : 4) k_rbgs_p
     }
     __syncthreads();
     write_plane(k1 - 1);
+    if (DOT) {
+        __shared__ double dsh[16];
+        block_sum<1>(dacc, dsh);
+        if (threadIdx.x == 0) part[bid] = dacc[0];
+    }
 }
 
 template <typename T>
@@ -1135,6 +1153,7 @@ static int vcycle_level(tb_ctx *c, int l) {
     const Level &L = c->lev[l];
     MGBuf<T> B = bufs<T>(c, l);
     LV<T> lv = lv_of<T>(L);
+    if (l == 0) c->vc_dot_blocks[sizeof(T) == 8 ? 1 : 0] = 0;
     if (l == c->small_l0[sizeof(T) == 8 ? 1 : 0]) {
         SmallCycle<T> A;
         A.l0 = l;
@@ -1175,18 +1194,20 @@ static int vcycle_level(tb_ctx *c, int l) {
     if (pairs) {
         const PairTiles P = make_pair_tiles(L.nx, L.ny, L.nz, pair_chunk(L));
         const dim3 grid(P.per_xcd * NXCD), blk(PAIR_THREADS);
-        auto pair = [&](int first, bool zero) {
+        auto pair = [&](int first, bool zero, bool dot) {
             T *dst = zero ? B.x : (cur == B.x ? B.a : B.x);
+            const T *src = cur;
             if (c->prof_on && kid >= 0) tb_prof_begin(c, kid);
-            if (zero) hipLaunchKernelGGL((k_rbgs_pair<T, 0, true>), grid, blk, 0, c->s, lv, (const T *)cur, dst, (const T *)B.b, P);
-            else if (first == 0) hipLaunchKernelGGL((k_rbgs_pair<T, 0, false>), grid, blk, 0, c->s, lv, (const T *)cur, dst, (const T *)B.b, P);
-            else hipLaunchKernelGGL((k_rbgs_pair<T, 1, false>), grid, blk, 0, c->s, lv, (const T *)cur, dst, (const T *)B.b, P);
+            if (zero) hipLaunchKernelGGL((k_rbgs_pair<T, 0, true, false>), grid, blk, 0, c->s, lv, src, dst, (const T *)B.b, P, nullptr);
+            else if (first == 0) hipLaunchKernelGGL((k_rbgs_pair<T, 0, false, false>), grid, blk, 0, c->s, lv, src, dst, (const T *)B.b, P, nullptr);
+            else if (!dot) hipLaunchKernelGGL((k_rbgs_pair<T, 1, false, false>), grid, blk, 0, c->s, lv, src, dst, (const T *)B.b, P, nullptr);
+            else hipLaunchKernelGGL((k_rbgs_pair<T, 1, false, true>), grid, blk, 0, c->s, lv, src, dst, (const T *)B.b, P, c->red);
             if (c->prof_on && kid >= 0) tb_prof_end(c, kid);
             cur = dst;
         };
         if (pre > 0) {
-            pair(0, true);
-            for (int s = 1; s < pre; s++) pair(0, false);
+            pair(0, true, false);
+            for (int s = 1; s < pre; s++) pair(0, false, false);
         } else {
             hipLaunchKernelGGL(k_zero<T>, tiles_grid(all), tiles_block(), 0, c->s, lv, B.x, all);
         }
@@ -1201,7 +1222,13 @@ static int vcycle_level(tb_ctx *c, int l) {
         T *dst = (post % 2 == 0) ? B.x : B.a;
         prolong<T>(c, l, (const T *)cur, dst);
         cur = dst;
-        for (int s = 0; s < post; s++) pair(1, false);
+        // on the finest level the last pair also forms the partial sums of
+        // r . z (b . x) for the Krylov method around the cycle
+        const bool dot = l == 0 && post > 0;
+        for (int s = 0; s < post; s++) pair(1, false, dot && s == post - 1);
+        if (l == 0) {
+            c->vc_dot_blocks[sizeof(T) == 8 ? 1 : 0] = dot ? (int)(P.per_xcd * NXCD) : 0;
+        }
         TB_LAUNCH_CHECK(c);
         return 0;
     }
@@ -1429,11 +1456,17 @@ static int pcg64(tb_ctx *c, const PoisCtx &P, double *x, const double *b, double
         if (residual(c, P, false, x, b, r, nullptr, rr, bb)) return -1;
     }
     res = sqrt(rr) / bn;
+    // r . z of the V-cycle just run: from its last pair's partial sums, or a
+    // separate pass when the finest level has no pairs
+    auto rz = [&](int slot) -> int {
+        if (c->vc_dot_blocks[1] > 0) return tb_reduce_finish(c, c->vc_dot_blocks[1], 1, slot);
+        return tb_dot(c, r, z, slot);
+    };
     while (res > rtol && its < maxit) {
         if (tb_vcycle64(c, r, z)) return -1;
         vcy++;
         TB_CHECK(c, hipMemcpyAsync(p, z, sizeof(double) * n, hipMemcpyDeviceToDevice, c->s));
-        if (tb_dot(c, r, z, S_RZ)) return -1;
+        if (rz(S_RZ)) return -1;
         int inner = 0;
         while (its < maxit) {
             TB_PROF(c, TB_K_POIS_OP,
@@ -1454,7 +1487,7 @@ static int pcg64(tb_ctx *c, const PoisCtx &P, double *x, const double *b, double
             if (rrel <= 0.5 * rtol || inner >= 200) break;
             if (tb_vcycle64(c, r, z)) return -1;
             vcy++;
-            if (tb_dot(c, r, z, S_TMP0)) return -1;
+            if (rz(S_TMP0)) return -1;
             hipLaunchKernelGGL(k_pcg_p, dim3(PB), dim3(PT), 0, c->s, n, (const double *)z, p, (const double *)c->scal);
             hipLaunchKernelGGL(k_copy_scalar, dim3(1), dim3(64), 0, c->s, c->scal, S_RZ, S_TMP0);
             TB_LAUNCH_CHECK(c);
@@ -1562,12 +1595,18 @@ static int cg32_inner(tb_ctx *c, const KTiles &kt, double r0n, double tol_rel, i
     const LV<float> lv = lv_of<float>(L0);
     const int nblk = (int)(kt.per_xcd * NXCD);
     TB_CHECK(c, hipMemsetAsync(e, 0, sizeof(float) * L0.ntot, c->s));
+    // r . z of the V-cycle just run: from its last pair's partial sums, or
+    // a separate pass when the finest level has no pairs
+    auto rz = [&](int slot) -> int {
+        if (c->vc_dot_blocks[0] > 0) return tb_reduce_finish(c, c->vc_dot_blocks[0], 1, slot);
+        hipLaunchKernelGGL(k_dot32, dim3(PB), dim3(PT), 0, c->s, n4, (const float4 *)r, (const float4 *)z, c->red);
+        TB_LAUNCH_CHECK(c);
+        return tb_reduce_finish(c, PB, 1, slot);
+    };
     if (tb_vcycle32(c)) return -1;
     vcy++;
     TB_CHECK(c, hipMemcpyAsync(p, z, sizeof(float) * L0.ntot, hipMemcpyDeviceToDevice, c->s));
-    hipLaunchKernelGGL(k_dot32, dim3(PB), dim3(PT), 0, c->s, n4, (const float4 *)r, (const float4 *)z, c->red);
-    TB_LAUNCH_CHECK(c);
-    if (tb_reduce_finish(c, PB, 1, S_RZ)) return -1;
+    if (rz(S_RZ)) return -1;
     int inner = 0;
     while (inner < maxit) {
         TB_PROF(c, TB_K_CG32_OP,
@@ -1591,9 +1630,7 @@ static int cg32_inner(tb_ctx *c, const KTiles &kt, double r0n, double tol_rel, i
         }
         if (tb_vcycle32(c)) return -1;
         vcy++;
-        hipLaunchKernelGGL(k_dot32, dim3(PB), dim3(PT), 0, c->s, n4, (const float4 *)r, (const float4 *)z, c->red);
-        TB_LAUNCH_CHECK(c);
-        if (tb_reduce_finish(c, PB, 1, S_TMP0)) return -1;
+        if (rz(S_TMP0)) return -1;
         hipLaunchKernelGGL(k_cg32_p, dim3(PB), dim3(PT), 0, c->s, n4, (const float4 *)z, (float4 *)p,
                            (const double *)c->scal);
         hipLaunchKernelGGL(k_copy_scalar, dim3(1), dim3(64), 0, c->s, c->scal, S_RZ, S_TMP0);
