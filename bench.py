@@ -50,7 +50,7 @@ WEIGHTS = os.path.join(ROOT, "turbins_amd", "weights", "poisson_guess.f32")
 # HBM bytes per instrumented launch measured with rocprofv3 FETCH_SIZE /
 # WRITE_SIZE on this workload, scaled by a plain copy of known size
 # (profiles/hbm_traffic_bump512x256.md); only valid for the kernels as built
-MEASURED_TRAFFIC = {"HELM_GS": 501.6e6}
+MEASURED_TRAFFIC = {"HELM_GS": 501.1e6}
 
 
 def log(*a):
