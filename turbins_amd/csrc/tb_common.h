@@ -309,7 +309,6 @@ struct tb_ctx {
     int guess_nr = 0;
     bool phi_pinned = false;   // the phi field is the output of the last Poisson solve (corner exactly 0)
     int guess_hist = GUESS_DEFAULT; // solutions in the Galerkin initial guess (tb_set_guess_history)
-    int nb_hist = 0;
     tb_net *net = nullptr;       // attached learned guess (not owned)
     tb_net *net_record = nullptr;    // snapshot sink after every solve (not owned)
     // run-time judgement of the network: iterations its direction saves
