@@ -190,6 +190,39 @@ int tb_reinit_iterations(tb_ctx *ctx);
 int tb_upload_field(tb_ctx *ctx, int f, const double *host);
 int tb_download_field(tb_ctx *ctx, int f, double *host);
 
+/* restart and field files ------------------------------------------------- */
+/* PETSc binary vectors as the reference's PetscViewerBinary + VecView /
+ * VecLoadIntoVector write and read them (Resume.c:96-139, Output.c:1610):
+ * int32 class id 1211214, int32 N, N doubles, all big-endian, DA natural
+ * order.  The array functions need no context (and no GPU). */
+#define TB_IO_BAD_ARGUMENT (-1)
+#define TB_IO_OPEN_FAILED (-2)
+#define TB_IO_NOT_A_VEC (-3)   /* wrong class id, negative length or bad record */
+#define TB_IO_SIZE (-4)        /* length larger than the buffer / the field */
+#define TB_IO_SHORT (-5)       /* short read or write */
+int tb_petsc_write_array(const char *path, const double *a, int64_t n);
+/* returns N (<= capacity) or a TB_IO_ code */
+int64_t tb_petsc_read_array(const char *path, double *a, int64_t capacity);
+/* one field <-> one PETSc binary file (VecView / VecLoadIntoVector of a
+ * G_data Vec); the file's length must equal the field's */
+int tb_save_field(tb_ctx *ctx, int f, const char *path);
+int tb_load_field(tb_ctx *ctx, int f, const char *path);
+/* Resume_primary.dat (Resume.c:247-319): the ASCII record
+ * "time=\n%lf\ndt=\n%lf\ndt_old=\n%lf\niteration=\n%d\noutput_iter=\n%d\n" */
+typedef struct {
+    double time, dt, dt_old;
+    int32_t iter, output_iter;
+} tb_resume_info;
+int tb_write_resume_primary(const char *dir, const tb_resume_info *info);
+int tb_read_resume_primary(const char *dir, tb_resume_info *info);
+/* the restart set of Resume_write_data / Resume_read_data with the files of
+ * Resume_set_data (Resume.c:172-244): Resume_u, _v, _w, _dpdx, _dpdy, _dpdz,
+ * _conc0 (.bin) and Resume_primary.dat in directory dir (Resume_gridc.bin,
+ * written by the reference only on request, is not: the geometry is set up
+ * again from the topography) */
+int tb_write_resume(tb_ctx *ctx, const char *dir, const tb_resume_info *info);
+int tb_read_resume(tb_ctx *ctx, const char *dir, tb_resume_info *info);
+
 /* per-operation mirrors of the reference entry points ---------------------- */
 /* advective term q -> TB_F_CONV_q for the whole field: replaces the
  * per-line ENO_compute_convective_terms (ENO.c:103-245, ENO.h:6) together

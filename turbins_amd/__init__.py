@@ -69,6 +69,15 @@ def lib():
     L.tb_set_helmholtz_method.argtypes = [P, C.c_int]
     L.tb_set_graphs.argtypes = [P, C.c_int]
     L.tb_set_mg_finest_sweeps.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_int]
+    L.tb_petsc_write_array.argtypes = [C.c_char_p, P, C.c_int64]
+    L.tb_petsc_read_array.restype = C.c_int64
+    L.tb_petsc_read_array.argtypes = [C.c_char_p, P, C.c_int64]
+    L.tb_save_field.argtypes = [P, C.c_int, C.c_char_p]
+    L.tb_load_field.argtypes = [P, C.c_int, C.c_char_p]
+    L.tb_write_resume_primary.argtypes = [C.c_char_p, C.POINTER(abi.ResumeInfo)]
+    L.tb_read_resume_primary.argtypes = [C.c_char_p, C.POINTER(abi.ResumeInfo)]
+    L.tb_write_resume.argtypes = [P, C.c_char_p, C.POINTER(abi.ResumeInfo)]
+    L.tb_read_resume.argtypes = [P, C.c_char_p, C.POINTER(abi.ResumeInfo)]
     L.tb_profile_enable.argtypes = [P, C.c_int]
     L.tb_profile_read.argtypes = [P, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
     L.tb_net_create.restype = P
@@ -110,6 +119,42 @@ def exported_symbols():
 
 def _ptr(a):
     return a.ctypes.data_as(C.c_void_p)
+
+
+def _io_check(rc, what, path):
+    if rc < 0:
+        raise OSError(f"{what} {path}: {abi.IO_ERRORS.get(int(rc), rc)}")
+    return rc
+
+
+def petsc_write(path, a):
+    """Write a 1-D float64 array as a PETSc binary Vec (VecView on a binary
+    viewer, Resume.c:102-103): big-endian class id 1211214, length, values."""
+    a = np.ascontiguousarray(a, dtype=np.float64).ravel()
+    _io_check(lib().tb_petsc_write_array(os.fsencode(path), _ptr(a), a.size), "petsc_write", path)
+
+
+def petsc_read(path, capacity=None):
+    """Read a PETSc binary Vec (VecLoadIntoVector, Resume.c:134) into a
+    float64 array; capacity defaults to what the file size allows."""
+    if capacity is None:
+        capacity = max(0, (os.path.getsize(path) - 8) // 8)
+    out = np.empty(capacity, dtype=np.float64)
+    n = _io_check(lib().tb_petsc_read_array(os.fsencode(path), _ptr(out), capacity), "petsc_read", path)
+    return out[:n]
+
+
+def write_resume_primary(directory, time, dt, dt_old, iteration, output_iter):
+    info = abi.ResumeInfo(time, dt, dt_old, iteration, output_iter)
+    _io_check(lib().tb_write_resume_primary(os.fsencode(directory), C.byref(info)), "write_resume_primary",
+              directory)
+
+
+def read_resume_primary(directory):
+    info = abi.ResumeInfo()
+    _io_check(lib().tb_read_resume_primary(os.fsencode(directory), C.byref(info)), "read_resume_primary", directory)
+    return {"time": info.time, "dt": info.dt, "dt_old": info.dt_old, "iteration": info.iter,
+            "output_iter": info.output_iter}
 
 
 class TBError(RuntimeError):
@@ -259,6 +304,26 @@ class Solver:
 
     def set_guess_history(self, h):
         return self._ok(self.L.tb_set_guess_history(self.h, h), "tb_set_guess_history")
+
+    def save_field(self, name, path):
+        """Field -> PETSc binary Vec file (VecView of its G_data Vec)."""
+        return self._ok(self.L.tb_save_field(self.h, abi.F[name], os.fsencode(path)), "tb_save_field")
+
+    def load_field(self, name, path):
+        """PETSc binary Vec file -> field (VecLoadIntoVector)."""
+        return self._ok(self.L.tb_load_field(self.h, abi.F[name], os.fsencode(path)), "tb_load_field")
+
+    def write_resume(self, directory, time, dt, dt_old, iteration, output_iter):
+        """The reference's restart set (Resume_write_data + Resume_write_primary_data)."""
+        info = abi.ResumeInfo(time, dt, dt_old, iteration, output_iter)
+        return self._ok(self.L.tb_write_resume(self.h, os.fsencode(directory), C.byref(info)), "tb_write_resume")
+
+    def read_resume(self, directory):
+        """Load the restart set; returns the Resume_primary.dat record."""
+        info = abi.ResumeInfo()
+        self._ok(self.L.tb_read_resume(self.h, os.fsencode(directory), C.byref(info)), "tb_read_resume")
+        return {"time": info.time, "dt": info.dt, "dt_old": info.dt_old, "iteration": info.iter,
+                "output_iter": info.output_iter}
 
     def set_mg_finest_sweeps(self, pre32=0, post32=0, pre64=0, post64=0):
         """Sweeps of the finest multigrid level per V-cycle side (0: the

@@ -64,6 +64,19 @@ class SolveStats(C.Structure):
     ]
 
 
+class ResumeInfo(C.Structure):
+    """tb_resume_info: the fields of Resume_primary.dat (Resume.c:247-319)."""
+    _fields_ = [
+        ("time", C.c_double), ("dt", C.c_double), ("dt_old", C.c_double),
+        ("iter", C.c_int32), ("output_iter", C.c_int32),
+    ]
+
+
+# tb_petsc_* / tb_*_resume_primary status codes
+IO_ERRORS = {-1: "bad argument", -2: "cannot open the file", -3: "not a PETSc binary vector",
+             -4: "vector length exceeds the buffer", -5: "short read or write"}
+
+
 class StepStats(C.Structure):
     _fields_ = [
         ("helm", (SolveStats * 4) * 2),
