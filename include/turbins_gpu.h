@@ -265,6 +265,12 @@ int tb_poisson_apply(tb_ctx *ctx, int f_in, int f_out);
  * iteration count; here st->iterations.  Krylov/multigrid method chosen by
  * tb_set_poisson_mode (the reference uses GMRES + BoomerAMG, Solver.c:90-265) */
 int tb_poisson_solve(tb_ctx *ctx, tb_solve_stats *st);
+/* Pressure_compute_real_pressure (Pressure.c:590-685, called before pressure
+ * output, Output.c:2198-2204): TB_F_PRHS = -div(grad p) from TB_F_DPDX/Y/Z
+ * (0 on BOUNDARY cells), then the Poisson solve into TB_F_PHI, which becomes
+ * the next solve's initial guess as in the reference */
+int tb_real_pressure_rhs(tb_ctx *ctx);
+int tb_real_pressure(tb_ctx *ctx, tb_solve_stats *st);
 /* Pressure_project_velocity (Pressure.c:280-439, Pressure.h:10) */
 int tb_project(tb_ctx *ctx, double dt);
 /* Pressure_compute_velocity_divergence (Pressure.c:462-586, Pressure.h:12) */

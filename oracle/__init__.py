@@ -49,6 +49,7 @@ def lib():
         L.ora_helmholtz_solve.argtypes = [P, C.c_int, C.c_double, C.c_double, C.c_int,
                                           C.POINTER(abi.SolveStats)]
         L.ora_poisson_rhs.argtypes = [P, C.c_double]
+        L.ora_real_pressure_rhs.argtypes = [P]
         L.ora_poisson_apply.argtypes = [P, P, P]
         L.ora_poisson_solve.argtypes = [P, C.c_double, C.c_int, C.POINTER(abi.SolveStats)]
         L.ora_project.argtypes = [P, C.c_double]
@@ -168,6 +169,9 @@ class Oracle:
 
     def poisson_rhs(self, dt):
         return self.L.ora_poisson_rhs(self.h, dt)
+
+    def real_pressure_rhs(self):
+        return self.L.ora_real_pressure_rhs(self.h)
 
     def poisson_apply(self, x):
         x = np.ascontiguousarray(x, dtype=np.float64)

@@ -608,6 +608,38 @@ int ora_helmholtz_solve(ora_ctx *o, int q, double dt, double rtol, int maxit, tb
 }
 
 /* ---------------------------------------------------------------------- */
+/* Pressure_compute_real_pressure (Pressure.c:590-685): the right-hand side  */
+/* -div(grad p) from the stored pressure gradients, 0 on BOUNDARY cells;    */
+/* the caller then solves the Poisson system into phi (Pressure_solve).     */
+
+int ora_real_pressure_rhs(ora_ctx *o) {
+    const double *dpx = o->f[TB_F_DPDX], *dpy = o->f[TB_F_DPDY], *dpz = o->f[TB_F_DPDZ];
+    double *rhs = o->f[TB_F_PRHS];
+    double a_dEta = 1.0 / o->dEta, a_dXi = 1.0 / o->dXi, a_dPhi = 1.0 / o->dPhi;
+    size_t pl = (size_t)o->NX * o->NY;
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < o->NZ; k++) {
+        double C_z = o->mzc[k] * a_dPhi;
+        for (int j = 0; j < o->NY; j++) {
+            double C_y = o->myc[j] * a_dXi;
+            for (int i = 0; i < o->NX; i++) {
+                size_t id = ora_idx(o, i, j, k);
+                double r = 0.0;
+                if (o->st[TB_Q_C][id] != TB_BOUNDARY) {
+                    double C_x = o->mxc[i] * a_dEta;
+                    double ddx = (dpx[id + 1] - dpx[id]) * C_x;
+                    double ddy = (dpy[id + o->NX] - dpy[id]) * C_y;
+                    double ddz = (dpz[id + pl] - dpz[id]) * C_z;
+                    r = -(ddx + ddy + ddz);
+                }
+                rhs[id] = r;
+            }
+        }
+    }
+    return 0;
+}
+
+/* ---------------------------------------------------------------------- */
 /* Poisson: Pressure_set_RHS (Pressure.c:171-276)                           */
 
 int ora_poisson_rhs(ora_ctx *o, double dt) {

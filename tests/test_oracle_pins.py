@@ -285,6 +285,38 @@ def test_projection_divergence_identity():
     np.testing.assert_allclose(div[m], np.abs(dt * r[m]), rtol=1e-7, atol=1e-9 * np.abs(dt * r).max())
 
 
+def test_real_pressure_rhs_closed_form_and_poisson_rhs():
+    """Pressure_compute_real_pressure's right-hand side: for the gradient of
+    p = x^2 + 2 y^2 - z^2 sampled on the faces, -div grad p = -(2 + 4 - 2)
+    on every non-BOUNDARY cell (central differences of linear functions are
+    exact), 0 on BOUNDARY cells; and it is Pressure_set_RHS's expression on
+    the same arrays with 1/dt = 1."""
+    c = W.case("bump_small", nx=32, ny=16, nz=8)
+    o = oracle.Oracle(W.config_for(c))
+    o.setup_geometry(W.topography(c))
+    xu, yv, zw = o.grid("xu"), o.grid("yv"), o.grid("zw")
+    Z, Y, X = np.meshgrid(zw, yv, xu, indexing="ij")
+    o.upload("DPDX", (2.0 * X).ravel())
+    o.upload("DPDY", (4.0 * Y).ravel())
+    o.upload("DPDZ", (-2.0 * Z).ravel())
+    o.real_pressure_rhs()
+    b = o.download("PRHS")
+    fl = o.flags(abi.Q_C)
+    m = fl != abi.BOUNDARY
+    np.testing.assert_allclose(b[m], -4.0, rtol=1e-9)
+    assert np.all(b[~m] == 0.0)
+    rng = np.random.default_rng(11)
+    f = [rng.standard_normal(o.NT) for _ in range(3)]
+    for name, a in zip(["DPDX", "DPDY", "DPDZ"], f):
+        o.upload(name, a)
+    o.real_pressure_rhs()
+    b1 = o.download("PRHS")
+    for name, a in zip(["U", "V", "W"], f):
+        o.upload(name, a)
+    o.poisson_rhs(1.0)
+    assert np.array_equal(b1, o.download("PRHS"))
+
+
 def test_vcycle_symmetric_and_contracting():
     c = W.case("bump_small", nx=32, ny=16, nz=16)
     o = oracle.Oracle(W.config_for(c))

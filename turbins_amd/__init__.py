@@ -72,6 +72,8 @@ def lib():
     L.tb_petsc_write_array.argtypes = [C.c_char_p, P, C.c_int64]
     L.tb_petsc_read_array.restype = C.c_int64
     L.tb_petsc_read_array.argtypes = [C.c_char_p, P, C.c_int64]
+    L.tb_real_pressure_rhs.argtypes = [P]
+    L.tb_real_pressure.argtypes = [P, C.POINTER(abi.SolveStats)]
     L.tb_save_field.argtypes = [P, C.c_int, C.c_char_p]
     L.tb_load_field.argtypes = [P, C.c_int, C.c_char_p]
     L.tb_write_resume_primary.argtypes = [C.c_char_p, C.POINTER(abi.ResumeInfo)]
@@ -265,6 +267,16 @@ class Solver:
     def poisson_solve(self):
         st = abi.SolveStats()
         self._ok(self.L.tb_poisson_solve(self.h, C.byref(st)), "tb_poisson_solve")
+        return st
+
+    def real_pressure_rhs(self):
+        return self._ok(self.L.tb_real_pressure_rhs(self.h), "tb_real_pressure_rhs")
+
+    def real_pressure(self):
+        """Pressure_compute_real_pressure: PHI <- the pressure whose gradient
+        is closest to DPDX/Y/Z (a Poisson solve); it seeds the next solve."""
+        st = abi.SolveStats()
+        self._ok(self.L.tb_real_pressure(self.h, C.byref(st)), "tb_real_pressure")
         return st
 
     def project(self, dt):

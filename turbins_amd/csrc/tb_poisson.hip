@@ -47,6 +47,26 @@ extern "C" int tb_poisson_rhs(tb_ctx *c, double dt) {
     return 0;
 }
 
+// Pressure_compute_real_pressure (Pressure.c:590-685): b = -div(grad p) from
+// the stored pressure gradients (0 on BOUNDARY cells), the expression of
+// k_prhs with the gradients for the velocities and a factor 1.0 (exact) for
+// 1/dt; tb_real_pressure then solves into phi, which (as in the reference)
+// is the next solve's initial guess.
+extern "C" int tb_real_pressure_rhs(tb_ctx *c) {
+    if (!c) return -1;
+    Tiles t = make_tiles(c->d.NX, c->d.NY, c->d.NZ);
+    hipLaunchKernelGGL(k_prhs, tiles_grid(t), tiles_block(), 0, c->s, c->d, c->g, c->f[TB_F_DPDX], c->f[TB_F_DPDY],
+                       c->f[TB_F_DPDZ], c->st[3], c->f[TB_F_PRHS], 1.0 / c->dEta, 1.0 / c->dXi, 1.0 / c->dPhi, 1.0,
+                       t);
+    TB_LAUNCH_CHECK(c);
+    return 0;
+}
+
+extern "C" int tb_real_pressure(tb_ctx *c, tb_solve_stats *st) {
+    if (tb_real_pressure_rhs(c)) return -1;
+    return tb_poisson_solve(c, st);
+}
+
 // ---------------------------------------------------------------------------
 // Operator rows (Pressure.c:740-924); neumann = 1 drops the corner pin.
 struct PoisOp {
