@@ -270,6 +270,12 @@ int tb_poisson_solve(tb_ctx *ctx, tb_solve_stats *st);
  * (0 on BOUNDARY cells), then the Poisson solve into TB_F_PHI, which becomes
  * the next solve's initial guess as in the reference */
 int tb_real_pressure_rhs(tb_ctx *ctx);
+/* diagnostics: Extract_total_kinetic_energy (Extract.c:803-884) and
+ * Conc_compute_total_suspended_mass (Conc.c:1471-1512), both over FLUID
+ * cells of the concentration grid, in one pass; either pointer may be NULL.
+ * Summation order differs from the reference's serial loop (FP64, relative
+ * differences of order 1e-15 per 1e6 cells). */
+int tb_diagnostics(tb_ctx *ctx, double *kinetic_energy, double *suspended_mass);
 int tb_real_pressure(tb_ctx *ctx, tb_solve_stats *st);
 /* Pressure_project_velocity (Pressure.c:280-439, Pressure.h:10) */
 int tb_project(tb_ctx *ctx, double dt);

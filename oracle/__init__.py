@@ -50,6 +50,7 @@ def lib():
                                           C.POINTER(abi.SolveStats)]
         L.ora_poisson_rhs.argtypes = [P, C.c_double]
         L.ora_real_pressure_rhs.argtypes = [P]
+        L.ora_diagnostics.argtypes = [P, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.ora_poisson_apply.argtypes = [P, P, P]
         L.ora_poisson_solve.argtypes = [P, C.c_double, C.c_int, C.POINTER(abi.SolveStats)]
         L.ora_project.argtypes = [P, C.c_double]
@@ -172,6 +173,11 @@ class Oracle:
 
     def real_pressure_rhs(self):
         return self.L.ora_real_pressure_rhs(self.h)
+
+    def diagnostics(self):
+        ke, m = C.c_double(), C.c_double()
+        self.L.ora_diagnostics(self.h, C.byref(ke), C.byref(m))
+        return {"kinetic_energy": ke.value, "suspended_mass": m.value}
 
     def poisson_apply(self, x):
         x = np.ascontiguousarray(x, dtype=np.float64)

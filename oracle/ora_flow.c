@@ -608,6 +608,36 @@ int ora_helmholtz_solve(ora_ctx *o, int q, double dt, double rtol, int maxit, tb
 }
 
 /* ---------------------------------------------------------------------- */
+/* Extract_total_kinetic_energy (Extract.c:803-884) and                     */
+/* Conc_compute_total_suspended_mass (Conc.c:1471-1512): serial k, j, i sums */
+/* over FLUID cells of the concentration grid, as the reference loops them. */
+
+int ora_diagnostics(ora_ctx *o, double *ke, double *mass) {
+    cell_center(o);
+    const double *c = o->f[TB_F_C];
+    double e = 0.0, m = 0.0;
+    for (int k = 0; k < o->NZ; k++) {
+        double dz = o->dz_c[k];
+        for (int j = 0; j < o->NY; j++) {
+            double dy = o->dy_c[j];
+            for (int i = 0; i < o->NX; i++) {
+                if (ora_status(o, TB_Q_C, i, j, k) != TB_FLUID) continue;
+                size_t id = ora_idx(o, i, j, k);
+                double dx = o->dx_c[i];
+                double dV = dx * dy * dz;
+                double u_ = o->ubc[id], v_ = o->vbc[id], w_ = o->wbc[id];
+                e += (u_ * u_ + v_ * v_ + w_ * w_) * dV;
+                double dVc = dz * dy * o->dx_c[i];
+                m += c[id] * dVc;
+            }
+        }
+    }
+    if (ke) *ke = 0.5 * e;
+    if (mass) *mass = m;
+    return 0;
+}
+
+/* ---------------------------------------------------------------------- */
 /* Pressure_compute_real_pressure (Pressure.c:590-685): the right-hand side  */
 /* -div(grad p) from the stored pressure gradients, 0 on BOUNDARY cells;    */
 /* the caller then solves the Poisson system into phi (Pressure_solve).     */

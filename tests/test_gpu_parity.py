@@ -271,6 +271,26 @@ def test_poisson_solve_matches_oracle(mode):
     o.close()
 
 
+def test_diagnostics_match_oracle():
+    """Kinetic energy and suspended mass over FLUID cells after a step of
+    the bump case (IB cells excluded): FP64 sums in another order, 1e-12."""
+    c, cfg, g, o = _pair("turbidity_small", helm_rtol=1e-13, pois_rtol=1e-11)
+    init = W.initial_fields(c, [o.flags(q) for q in range(4)])
+    for k, v in init.items():
+        g.upload(k, v)
+        o.upload(k, v)
+    g.step(c["dt"])
+    for n in ("U", "V", "W", "C"):
+        o.upload(n, g.download(n))
+    dg, do = g.diagnostics(), o.diagnostics()
+    print("diagnostics", dg, do)
+    for key in dg:
+        assert do[key] > 0.0
+        assert abs(dg[key] - do[key]) <= 1e-12 * abs(do[key]), key
+    g.close()
+    o.close()
+
+
 def test_real_pressure_matches_oracle():
     """Pressure_compute_real_pressure (Pressure.c:590-685): the right-hand
     side from the pressure gradients bit for bit, then the Poisson solve into

@@ -317,6 +317,32 @@ def test_real_pressure_rhs_closed_form_and_poisson_rhs():
     assert np.array_equal(b1, o.download("PRHS"))
 
 
+def test_diagnostics_closed_form():
+    """Total kinetic energy and suspended mass over FLUID cells: without
+    topography every interior cell is FLUID, so a uniform velocity (2, -1, 0.5)
+    and c = 1 give 0.5 |U|^2 Lx Ly Lz and Lx Ly Lz; a velocity linear in x
+    has exact midpoint cell-centre values, so the energy is the closed-form
+    sum of 0.5 x_c^2 over the cells."""
+    c = W.case("lock64", nx=16, ny=8, nz=8, lx=2.0)
+    o = oracle.Oracle(W.config_for(c))
+    o.setup_geometry(None)
+    vol = c["lx"] * c["ly"] * c["lz"]
+    for name, val in (("U", 2.0), ("V", -1.0), ("W", 0.5), ("C", 1.0)):
+        o.upload(name, np.full(o.NT, val))
+    d = o.diagnostics()
+    np.testing.assert_allclose(d["suspended_mass"], vol, rtol=1e-12)
+    np.testing.assert_allclose(d["kinetic_energy"], 0.5 * (4.0 + 1.0 + 0.25) * vol, rtol=1e-12)
+    xu = o.grid("xu")
+    NX, NY, NZ = c["nx"] + 1, c["ny"] + 1, c["nz"] + 1
+    o.upload("U", np.tile(xu, NY * NZ))
+    o.upload("V", np.zeros(o.NT))
+    o.upload("W", np.zeros(o.NT))
+    xc = 0.5 * (xu[:-1] + xu[1:])
+    dv = (c["lx"] / c["nx"]) * (c["ly"] / c["ny"]) * (c["lz"] / c["nz"])
+    want = 0.5 * np.sum(xc[: c["nx"]] ** 2) * dv * c["ny"] * c["nz"]
+    np.testing.assert_allclose(o.diagnostics()["kinetic_energy"], want, rtol=1e-12)
+
+
 def test_vcycle_symmetric_and_contracting():
     c = W.case("bump_small", nx=32, ny=16, nz=16)
     o = oracle.Oracle(W.config_for(c))

@@ -73,6 +73,7 @@ def lib():
     L.tb_petsc_read_array.restype = C.c_int64
     L.tb_petsc_read_array.argtypes = [C.c_char_p, P, C.c_int64]
     L.tb_real_pressure_rhs.argtypes = [P]
+    L.tb_diagnostics.argtypes = [P, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.tb_real_pressure.argtypes = [P, C.POINTER(abi.SolveStats)]
     L.tb_save_field.argtypes = [P, C.c_int, C.c_char_p]
     L.tb_load_field.argtypes = [P, C.c_int, C.c_char_p]
@@ -268,6 +269,12 @@ class Solver:
         st = abi.SolveStats()
         self._ok(self.L.tb_poisson_solve(self.h, C.byref(st)), "tb_poisson_solve")
         return st
+
+    def diagnostics(self):
+        """Total kinetic energy and suspended mass over the FLUID cells."""
+        ke, m = C.c_double(), C.c_double()
+        self._ok(self.L.tb_diagnostics(self.h, C.byref(ke), C.byref(m)), "tb_diagnostics")
+        return {"kinetic_energy": ke.value, "suspended_mass": m.value}
 
     def real_pressure_rhs(self):
         return self._ok(self.L.tb_real_pressure_rhs(self.h), "tb_real_pressure_rhs")

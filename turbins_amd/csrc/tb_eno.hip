@@ -587,3 +587,54 @@ extern "C" int tb_set_rhs(tb_ctx *c, int q, double dt) {
     TB_LAUNCH_CHECK(c);
     return 0;
 }
+
+// ---------------------------------------------------------------------------
+// Extract_total_kinetic_energy (Extract.c:803-884) and
+// Conc_compute_total_suspended_mass (Conc.c:1471-1512) in one pass over the
+// cells: over FLUID cells (concentration status), the sums of
+// (ubc^2 + vbc^2 + wbc^2) dV with the cell-centre velocities of
+// Velocity_cell_center (Velocity.c:1476-1602) and dV = (dx dy) dz, and of
+// c dV with dV = (dz dy) dx, each as the reference associates it.
+__global__ void __launch_bounds__(256) k_diag(Dims d, GridDev g, const double *__restrict__ u,
+                                              const double *__restrict__ v, const double *__restrict__ w,
+                                              const double *__restrict__ c, const uint8_t *__restrict__ sc,
+                                              double *part, Tiles t) {
+    __shared__ double sh[32];
+    double acc[2] = {0.0, 0.0};
+    int i, j, k;
+    if (tile_point(t, i, j, k)) {
+        const long id = gidx(d, i, j, k);
+        if (sc[id] == TB_FLUID) {
+            const double ubc = interp2d(u[id], g.xu[i], u[id + 1], g.xu[i + 1], g.xc[i]);
+            const double vbc = interp2d(v[id], g.yv[j], v[id + d.px], g.yv[j + 1], g.yc[j]);
+            const double wbc = interp2d(w[id], g.zw[k], w[id + d.plane], g.zw[k + 1], g.zc[k]);
+            const double dx = g.dxc[i], dy = g.dyc[j], dz = g.dzc[k];
+            acc[0] = (ubc * ubc + vbc * vbc + wbc * wbc) * (dx * dy * dz);
+            acc[1] = c[id] * (dz * dy * dx);
+        }
+    }
+    block_sum<2>(acc, sh);
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        part[blockIdx.x] = acc[0];
+        part[RED_MAX_BLOCKS + blockIdx.x] = acc[1];
+    }
+}
+
+extern "C" int tb_diagnostics(tb_ctx *c, double *kinetic_energy, double *suspended_mass) {
+    if (!c) return -1;
+    const Tiles t = make_tiles(c->ncx, c->ncy, c->ncz);
+    const dim3 grid = tiles_grid(t);
+    if ((long)grid.x * grid.y * grid.z > RED_MAX_BLOCKS || grid.y != 1 || grid.z != 1) {
+        c->err = "tb_diagnostics: grid too large for the reduction buffer";
+        return -1;
+    }
+    hipLaunchKernelGGL(k_diag, grid, tiles_block(), 0, c->s, c->d, c->g, (const double *)c->f[TB_F_U],
+                       (const double *)c->f[TB_F_V], (const double *)c->f[TB_F_W], (const double *)c->f[TB_F_C],
+                       (const uint8_t *)c->st[3], c->red, t);
+    TB_LAUNCH_CHECK(c);
+    if (tb_reduce_finish(c, (int)grid.x, 2, S_TMP1)) return -1;
+    if (tb_read_scalars(c, S_TMP1, 2)) return -1;
+    if (kinetic_energy) *kinetic_energy = 0.5 * c->hscal[S_TMP1];
+    if (suspended_mass) *suspended_mass = c->hscal[S_TMP2];
+    return 0;
+}
