@@ -276,6 +276,10 @@ int tb_real_pressure_rhs(tb_ctx *ctx);
  * Summation order differs from the reference's serial loop (FP64, relative
  * differences of order 1e-15 per 1e6 cells). */
 int tb_diagnostics(tb_ctx *ctx, double *kinetic_energy, double *suspended_mass);
+/* Extract_Stokes_dissipation_rate (Extract.c:886-950): sum of c dV over FLUID
+ * cells times |settling speed| times conc_alpha (params->Conc_alpha[i]) for
+ * a particle concentration, 0 for salinity */
+int tb_stokes_dissipation(tb_ctx *ctx, double conc_alpha, double *rate);
 int tb_real_pressure(tb_ctx *ctx, tb_solve_stats *st);
 /* Pressure_project_velocity (Pressure.c:280-439, Pressure.h:10) */
 int tb_project(tb_ctx *ctx, double dt);

@@ -51,6 +51,8 @@ def lib():
         L.ora_poisson_rhs.argtypes = [P, C.c_double]
         L.ora_real_pressure_rhs.argtypes = [P]
         L.ora_diagnostics.argtypes = [P, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.ora_stokes_dissipation.restype = C.c_double
+        L.ora_stokes_dissipation.argtypes = [P, C.c_int, C.c_double, C.c_double]
         L.ora_poisson_apply.argtypes = [P, P, P]
         L.ora_poisson_solve.argtypes = [P, C.c_double, C.c_int, C.POINTER(abi.SolveStats)]
         L.ora_project.argtypes = [P, C.c_double]
@@ -173,6 +175,9 @@ class Oracle:
 
     def real_pressure_rhs(self):
         return self.L.ora_real_pressure_rhs(self.h)
+
+    def stokes_dissipation(self, particle, v_settl, conc_alpha=1.0):
+        return self.L.ora_stokes_dissipation(self.h, 1 if particle else 0, v_settl, conc_alpha)
 
     def diagnostics(self):
         ke, m = C.c_double(), C.c_double()

@@ -637,6 +637,27 @@ int ora_diagnostics(ora_ctx *o, double *ke, double *mass) {
     return 0;
 }
 
+/* Extract_Stokes_dissipation_rate (Extract.c:886-950) */
+double ora_stokes_dissipation(ora_ctx *o, int particle, double v_settl, double conc_alpha) {
+    if (!particle) return 0.0;
+    const double *c = o->f[TB_F_C];
+    double r = 0.0;
+    for (int k = 0; k < o->NZ; k++) {
+        double dz = o->dz_c[k];
+        for (int j = 0; j < o->NY; j++) {
+            double dy = o->dy_c[j];
+            for (int i = 0; i < o->NX; i++) {
+                if (ora_status(o, TB_Q_C, i, j, k) != TB_FLUID) continue;
+                double dx = o->dx_c[i];
+                double dV = dx * dy * dz;
+                r += c[ora_idx(o, i, j, k)] * dV;
+            }
+        }
+    }
+    r *= fabs(v_settl) * conc_alpha;
+    return r;
+}
+
 /* ---------------------------------------------------------------------- */
 /* Pressure_compute_real_pressure (Pressure.c:590-685): the right-hand side  */
 /* -div(grad p) from the stored pressure gradients, 0 on BOUNDARY cells;    */

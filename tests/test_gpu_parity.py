@@ -287,6 +287,11 @@ def test_diagnostics_match_oracle():
     for key in dg:
         assert do[key] > 0.0
         assert abs(dg[key] - do[key]) <= 1e-12 * abs(do[key]), key
+    particle = cfg.conc_type == abi.PARTICLE
+    sg, so = g.stokes_dissipation(0.7), o.stokes_dissipation(particle, cfg.v_settle, 0.7)
+    print("stokes dissipation", particle, sg, so)
+    assert particle and so > 0.0
+    assert abs(sg - so) <= 1e-12 * so
     g.close()
     o.close()
 

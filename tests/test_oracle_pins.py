@@ -341,6 +341,10 @@ def test_diagnostics_closed_form():
     dv = (c["lx"] / c["nx"]) * (c["ly"] / c["ny"]) * (c["lz"] / c["nz"])
     want = 0.5 * np.sum(xc[: c["nx"]] ** 2) * dv * c["ny"] * c["nz"]
     np.testing.assert_allclose(o.diagnostics()["kinetic_energy"], want, rtol=1e-12)
+    # Extract_Stokes_dissipation_rate: |v_s| alpha times the particle volume
+    o.upload("C", np.full(o.NT, 0.25))
+    np.testing.assert_allclose(o.stokes_dissipation(True, -0.02, 0.5), 0.02 * 0.5 * 0.25 * vol, rtol=1e-12)
+    assert o.stokes_dissipation(False, -0.02, 0.5) == 0.0
 
 
 def test_vcycle_symmetric_and_contracting():

@@ -74,6 +74,7 @@ def lib():
     L.tb_petsc_read_array.argtypes = [C.c_char_p, P, C.c_int64]
     L.tb_real_pressure_rhs.argtypes = [P]
     L.tb_diagnostics.argtypes = [P, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.tb_stokes_dissipation.argtypes = [P, C.c_double, C.POINTER(C.c_double)]
     L.tb_real_pressure.argtypes = [P, C.POINTER(abi.SolveStats)]
     L.tb_save_field.argtypes = [P, C.c_int, C.c_char_p]
     L.tb_load_field.argtypes = [P, C.c_int, C.c_char_p]
@@ -275,6 +276,11 @@ class Solver:
         ke, m = C.c_double(), C.c_double()
         self._ok(self.L.tb_diagnostics(self.h, C.byref(ke), C.byref(m)), "tb_diagnostics")
         return {"kinetic_energy": ke.value, "suspended_mass": m.value}
+
+    def stokes_dissipation(self, conc_alpha=1.0):
+        r = C.c_double()
+        self._ok(self.L.tb_stokes_dissipation(self.h, conc_alpha, C.byref(r)), "tb_stokes_dissipation")
+        return r.value
 
     def real_pressure_rhs(self):
         return self._ok(self.L.tb_real_pressure_rhs(self.h), "tb_real_pressure_rhs")

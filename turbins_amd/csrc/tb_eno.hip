@@ -599,8 +599,8 @@ __global__ void __launch_bounds__(256) k_diag(Dims d, GridDev g, const double *_
                                               const double *__restrict__ v, const double *__restrict__ w,
                                               const double *__restrict__ c, const uint8_t *__restrict__ sc,
                                               double *part, Tiles t) {
-    __shared__ double sh[32];
-    double acc[2] = {0.0, 0.0};
+    __shared__ double sh[48];
+    double acc[3] = {0.0, 0.0, 0.0};
     int i, j, k;
     if (tile_point(t, i, j, k)) {
         const long id = gidx(d, i, j, k);
@@ -611,17 +611,20 @@ __global__ void __launch_bounds__(256) k_diag(Dims d, GridDev g, const double *_
             const double dx = g.dxc[i], dy = g.dyc[j], dz = g.dzc[k];
             acc[0] = (ubc * ubc + vbc * vbc + wbc * wbc) * (dx * dy * dz);
             acc[1] = c[id] * (dz * dy * dx);
+            acc[2] = c[id] * (dx * dy * dz);   // Extract_Stokes_dissipation_rate's dV
         }
     }
-    block_sum<2>(acc, sh);
+    block_sum<3>(acc, sh);
     if (threadIdx.x == 0 && threadIdx.y == 0) {
         part[blockIdx.x] = acc[0];
         part[RED_MAX_BLOCKS + blockIdx.x] = acc[1];
+        part[2 * RED_MAX_BLOCKS + blockIdx.x] = acc[2];
     }
 }
 
-extern "C" int tb_diagnostics(tb_ctx *c, double *kinetic_energy, double *suspended_mass) {
-    if (!c) return -1;
+static int diag_sums(tb_ctx *c, double out[3]);
+
+static int diag_sums(tb_ctx *c, double out[3]) {
     const Tiles t = make_tiles(c->ncx, c->ncy, c->ncz);
     const dim3 grid = tiles_grid(t);
     if ((long)grid.x * grid.y * grid.z > RED_MAX_BLOCKS || grid.y != 1 || grid.z != 1) {
@@ -632,9 +635,35 @@ extern "C" int tb_diagnostics(tb_ctx *c, double *kinetic_energy, double *suspend
                        (const double *)c->f[TB_F_V], (const double *)c->f[TB_F_W], (const double *)c->f[TB_F_C],
                        (const uint8_t *)c->st[3], c->red, t);
     TB_LAUNCH_CHECK(c);
-    if (tb_reduce_finish(c, (int)grid.x, 2, S_TMP1)) return -1;
-    if (tb_read_scalars(c, S_TMP1, 2)) return -1;
-    if (kinetic_energy) *kinetic_energy = 0.5 * c->hscal[S_TMP1];
-    if (suspended_mass) *suspended_mass = c->hscal[S_TMP2];
+    if (tb_reduce_finish(c, (int)grid.x, 3, S_TMP0)) return -1;
+    if (tb_read_scalars(c, S_TMP0, 3)) return -1;
+    for (int a = 0; a < 3; a++) out[a] = c->hscal[S_TMP0 + a];
+    return 0;
+}
+
+extern "C" int tb_diagnostics(tb_ctx *c, double *kinetic_energy, double *suspended_mass) {
+    if (!c) return -1;
+    double s[3];
+    if (diag_sums(c, s)) return -1;
+    if (kinetic_energy) *kinetic_energy = 0.5 * s[0];
+    if (suspended_mass) *suspended_mass = s[1];
+    return 0;
+}
+
+// Extract_Stokes_dissipation_rate (Extract.c:886-950): |v_settl0| alpha
+// times the sum of c dV over FLUID cells for a particle concentration, 0
+// for salinity
+extern "C" int tb_stokes_dissipation(tb_ctx *c, double conc_alpha, double *rate) {
+    if (!c || !rate) {
+        if (c) c->err = "tb_stokes_dissipation: bad argument";
+        return -1;
+    }
+    if (c->cfg.conc_type != TB_PARTICLE) {
+        *rate = 0.0;
+        return 0;
+    }
+    double s[3];
+    if (diag_sums(c, s)) return -1;
+    *rate = s[2] * (fabs(c->cfg.v_settle) * conc_alpha);
     return 0;
 }
