@@ -13,6 +13,7 @@
 // The set of restart files and their names follow Resume_set_data
 // (Resume.c:172-244); Resume_primary.dat is the ASCII record of
 // Resume_write_primary_data / Resume_read_primary_data (Resume.c:247-319).
+#include <algorithm>
 #include <cerrno>
 #include <cstdio>
 #include <cstring>
@@ -196,5 +197,8 @@ extern "C" int tb_read_resume(tb_ctx *c, const char *dir, tb_resume_info *info) 
     }
     for (const ResumeVec &v : RESUME_VECS)
         if (tb_load_field(c, v.field, join(dir, v.name).c_str())) return -1;
+    // a restart begins a new trajectory: the learned guess forgets the
+    // solutions of the old one
+    c->n_hist = c->gram_n = c->guess_nr = 0;
     return 0;
 }
