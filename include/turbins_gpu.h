@@ -280,6 +280,13 @@ int tb_diagnostics(tb_ctx *ctx, double *kinetic_energy, double *suspended_mass);
  * cells times |settling speed| times conc_alpha (params->Conc_alpha[i]) for
  * a particle concentration, 0 for salinity */
 int tb_stokes_dissipation(tb_ctx *ctx, double conc_alpha, double *rate);
+/* Conc_compute_sedimentation_rate (Conc.c:2098-2157) and Conc_bottom_sed_rate
+ * (Conc.c:2158-2229): per (i, k) column |v_s| c dx dz at the column's first
+ * IMMERSED cell, or at row 0 when it is FLUID (Grid_set_interface_y_index,
+ * Grid.c:1972-2009), 0 elsewhere; bottom receives the (NZ+1) x (NX+1) column
+ * values (x fastest, may be NULL), total their sum in the reference's order;
+ * 0 for salinity */
+int tb_sedimentation_rate(tb_ctx *ctx, double *total, double *bottom);
 int tb_real_pressure(tb_ctx *ctx, tb_solve_stats *st);
 /* Pressure_project_velocity (Pressure.c:280-439, Pressure.h:10) */
 int tb_project(tb_ctx *ctx, double dt);

@@ -53,6 +53,8 @@ def lib():
         L.ora_diagnostics.argtypes = [P, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.ora_stokes_dissipation.restype = C.c_double
         L.ora_stokes_dissipation.argtypes = [P, C.c_int, C.c_double, C.c_double]
+        L.ora_sedimentation_rate.restype = C.c_double
+        L.ora_sedimentation_rate.argtypes = [P, C.c_int, C.c_double, P]
         L.ora_poisson_apply.argtypes = [P, P, P]
         L.ora_poisson_solve.argtypes = [P, C.c_double, C.c_int, C.POINTER(abi.SolveStats)]
         L.ora_project.argtypes = [P, C.c_double]
@@ -175,6 +177,12 @@ class Oracle:
 
     def real_pressure_rhs(self):
         return self.L.ora_real_pressure_rhs(self.h)
+
+    def sedimentation_rate(self, particle, v_settl):
+        nxp, nzp = self.cfg.nx + 1, self.cfg.nz + 1
+        b = np.zeros((nzp, nxp))
+        tot = self.L.ora_sedimentation_rate(self.h, 1 if particle else 0, v_settl, _ptr(b))
+        return tot, b
 
     def stokes_dissipation(self, particle, v_settl, conc_alpha=1.0):
         return self.L.ora_stokes_dissipation(self.h, 1 if particle else 0, v_settl, conc_alpha)

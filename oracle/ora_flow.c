@@ -637,6 +637,30 @@ int ora_diagnostics(ora_ctx *o, double *ke, double *mass) {
     return 0;
 }
 
+/* Grid_set_interface_y_index (Grid.c:1972-2009) + Conc_bottom_sed_rate      */
+/* (Conc.c:2158-2229) + Conc_compute_sedimentation_rate (Conc.c:2098-2157)  */
+double ora_sedimentation_rate(ora_ctx *o, int particle, double v_settl, double *bottom) {
+    double tot = 0.0;
+    double us = fabs(v_settl);
+    for (int k = 0; k < o->NZ; k++) {
+        double dz = o->dz_c[k];
+        for (int i = 0; i < o->NX; i++) {
+            int jf = -1;
+            for (int j = 0; j < o->NY; j++) {
+                int s = ora_status(o, TB_Q_C, i, j, k);
+                if (s == TB_IMMERSED) { jf = j; break; }
+                if (j == 0 && s == TB_FLUID) { jf = j; break; }
+            }
+            double dx = o->dx_c[i];
+            double r = 0.0;
+            if (particle && jf >= 0) r = us * o->f[TB_F_C][ora_idx(o, i, jf, k)] * dx * dz;
+            if (bottom) bottom[(size_t)k * o->NX + i] = r;
+            if (particle && jf >= 0) tot += r;
+        }
+    }
+    return tot;
+}
+
 /* Extract_Stokes_dissipation_rate (Extract.c:886-950) */
 double ora_stokes_dissipation(ora_ctx *o, int particle, double v_settl, double conc_alpha) {
     if (!particle) return 0.0;

@@ -58,6 +58,7 @@ int ora_poisson_rhs(ora_ctx *o, double dt);
 int ora_real_pressure_rhs(ora_ctx *o);
 int ora_diagnostics(ora_ctx *o, double *ke, double *mass);
 double ora_stokes_dissipation(ora_ctx *o, int particle, double v_settl, double conc_alpha);
+double ora_sedimentation_rate(ora_ctx *o, int particle, double v_settl, double *bottom);
 int ora_poisson_apply(ora_ctx *o, const double *x, double *y);
 int ora_poisson_solve(ora_ctx *o, double rtol, int maxit, tb_solve_stats *st);
 int ora_project(ora_ctx *o, double dt);

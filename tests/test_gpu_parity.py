@@ -292,6 +292,10 @@ def test_diagnostics_match_oracle():
     print("stokes dissipation", particle, sg, so)
     assert particle and so > 0.0
     assert abs(sg - so) <= 1e-12 * so
+    # sedimentation rate: per-column values and their ordered sum, bit for bit
+    (tg, bg), (to, bo) = g.sedimentation_rate(), o.sedimentation_rate(particle, cfg.v_settle)
+    print("sedimentation rate", tg, to)
+    assert np.array_equal(bg, bo) and tg == to and to > 0.0
     g.close()
     o.close()
 

@@ -347,6 +347,41 @@ def test_diagnostics_closed_form():
     assert o.stokes_dissipation(False, -0.02, 0.5) == 0.0
 
 
+def test_sedimentation_rate_brute_force():
+    """Grid_set_interface_y_index + Conc_compute_sedimentation_rate: without
+    topography every column's interface is row 0 (FLUID), so the rate is
+    |v_s| times the integral of c over the bottom row; over a bump it is
+    checked against a brute-force scan of the flags."""
+    c = W.case("lock64", nx=16, ny=8, nz=8, lx=2.0)
+    o = oracle.Oracle(W.config_for(c))
+    o.setup_geometry(None)
+    o.upload("C", np.full(o.NT, 0.5))
+    tot, b = o.sedimentation_rate(True, -0.02)
+    np.testing.assert_allclose(tot, 0.02 * 0.5 * c["lx"] * c["lz"], rtol=1e-12)
+    assert o.sedimentation_rate(False, -0.02)[0] == 0.0
+    c = W.case("bump_small", nx=32, ny=16, nz=8)
+    o = oracle.Oracle(W.config_for(c))
+    o.setup_geometry(W.topography(c))
+    rng = np.random.default_rng(5)
+    cf = rng.random(o.NT)
+    o.upload("C", cf)
+    tot, b = o.sedimentation_rate(True, 0.03)
+    NX, NY, NZ = c["nx"] + 1, c["ny"] + 1, c["nz"] + 1
+    fl = o.flags(abi.Q_C).reshape(NZ, NY, NX)
+    cc = cf.reshape(NZ, NY, NX)
+    dx, dz = c["lx"] / c["nx"], c["lz"] / c["nz"]
+    want = np.zeros((NZ, NX))
+    for k in range(NZ):
+        for i in range(NX):
+            for j in range(NY):
+                if fl[k, j, i] == abi.IMMERSED or (j == 0 and fl[k, j, i] == abi.FLUID):
+                    want[k, i] = 0.03 * cc[k, j, i] * o.grid("dx_c")[i] * o.grid("dz_c")[k]
+                    break
+    assert np.count_nonzero(want) > 0
+    np.testing.assert_allclose(b, want, rtol=1e-14, atol=0)
+    np.testing.assert_allclose(tot, want.sum(), rtol=1e-13)
+
+
 def test_vcycle_symmetric_and_contracting():
     c = W.case("bump_small", nx=32, ny=16, nz=16)
     o = oracle.Oracle(W.config_for(c))

@@ -75,6 +75,7 @@ def lib():
     L.tb_real_pressure_rhs.argtypes = [P]
     L.tb_diagnostics.argtypes = [P, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.tb_stokes_dissipation.argtypes = [P, C.c_double, C.POINTER(C.c_double)]
+    L.tb_sedimentation_rate.argtypes = [P, C.POINTER(C.c_double), P]
     L.tb_real_pressure.argtypes = [P, C.POINTER(abi.SolveStats)]
     L.tb_save_field.argtypes = [P, C.c_int, C.c_char_p]
     L.tb_load_field.argtypes = [P, C.c_int, C.c_char_p]
@@ -276,6 +277,13 @@ class Solver:
         ke, m = C.c_double(), C.c_double()
         self._ok(self.L.tb_diagnostics(self.h, C.byref(ke), C.byref(m)), "tb_diagnostics")
         return {"kinetic_energy": ke.value, "suspended_mass": m.value}
+
+    def sedimentation_rate(self):
+        """Total sedimentation rate and the per-column bottom rates (NZ+1, NX+1)."""
+        tot = C.c_double()
+        b = np.zeros((self.cfg.nz + 1, self.cfg.nx + 1))
+        self._ok(self.L.tb_sedimentation_rate(self.h, C.byref(tot), _ptr(b)), "tb_sedimentation_rate")
+        return tot.value, b
 
     def stokes_dissipation(self, conc_alpha=1.0):
         r = C.c_double()

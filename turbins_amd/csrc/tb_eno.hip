@@ -15,6 +15,8 @@
 #include <algorithm>
 #include <cmath>
 
+#include <cstring>
+
 #include "tb_common.h"
 
 __device__ inline double interp2d(double q0, double x0, double q1, double x1, double xi) {
@@ -665,5 +667,50 @@ extern "C" int tb_stokes_dissipation(tb_ctx *c, double conc_alpha, double *rate)
     double s[3];
     if (diag_sums(c, s)) return -1;
     *rate = s[2] * (fabs(c->cfg.v_settle) * conc_alpha);
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Conc_bottom_sed_rate (Conc.c:2158-2229) with the interface index of
+// Grid_set_interface_y_index (Grid.c:1972-2009): per (i, k) column the first
+// IMMERSED cell from the bottom, or row 0 when it is FLUID; the column's
+// sedimentation rate u_s c dx dz there, 0 where no such cell exists.
+__global__ void k_sed_bottom(Dims d, GridDev g, const double *__restrict__ c, const uint8_t *__restrict__ sc,
+                             double us, double *__restrict__ out) {
+    const long n = (long)d.NX * d.NZ;
+    const long a = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (a >= n) return;
+    const int i = (int)(a % d.NX), k = (int)(a / d.NX);
+    int jf = -1;
+    for (int j = 0; j < d.NY; j++) {
+        const int s = sc[gidx(d, i, j, k)];
+        if (s == TB_IMMERSED || (j == 0 && s == TB_FLUID)) {
+            jf = j;
+            break;
+        }
+    }
+    out[a] = jf >= 0 ? us * c[gidx(d, i, jf, k)] * g.dxc[i] * g.dzc[k] : 0.0;
+}
+
+// Conc_compute_sedimentation_rate (Conc.c:2098-2157): the column values
+// summed in the reference's order (k outer, i inner) on the host, so the
+// total is the reference's bits; bottom (NZ x NX, x fastest) may be NULL.
+extern "C" int tb_sedimentation_rate(tb_ctx *c, double *total, double *bottom) {
+    if (!c) return -1;
+    const Dims &d = c->d;
+    const long n = (long)d.NX * d.NZ;
+    std::vector<double> h((size_t)n, 0.0);
+    if (c->cfg.conc_type == TB_PARTICLE) {
+        double *dev = c->kv[7];   // scratch vector (any field-sized buffer of the context)
+        hipLaunchKernelGGL(k_sed_bottom, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->s, d, c->g,
+                           (const double *)c->f[TB_F_C], (const uint8_t *)c->st[3], fabs(c->cfg.v_settle), dev);
+        TB_LAUNCH_CHECK(c);
+        TB_CHECK(c, hipMemcpyAsync(h.data(), dev, sizeof(double) * n, hipMemcpyDeviceToHost, c->s));
+        TB_CHECK(c, hipStreamSynchronize(c->s));
+    }
+    double s = 0.0;
+    for (long a = 0; a < n; a++) s += h[a];
+    if (total) *total = s;
+    if (bottom) memcpy(bottom, h.data(), sizeof(double) * n);
     return 0;
 }
