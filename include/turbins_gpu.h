@@ -136,7 +136,12 @@ typedef struct tb_ib_node {
 typedef struct tb_solve_stats {
     int32_t iterations;         /* Krylov / refinement iterations */
     int32_t converged;          /* 1 if the target was met */
-    int32_t fallback;           /* 1 if the plain FP64 path had to take over */
+    int32_t fallback;           /* Poisson: bit 1 the mixed solve stagnated and FP64
+                                   GMG-PCG finished it; bit 2 the initial guess had a
+                                   larger residual than the previous phi, which FP64
+                                   GMG-PCG then started from.  Helmholtz: 1 if the
+                                   Gauss-Seidel sweeps stopped contracting and BiCGStab
+                                   finished the solve. */
     int32_t vcycles;            /* V-cycles applied (Poisson) */
     double res0;                /* initial true relative residual (NaN: not evaluated,
                                    the Helmholtz Gauss-Seidel path) */
@@ -265,10 +270,20 @@ int tb_poisson_apply(tb_ctx *ctx, int f_in, int f_out);
  * iteration count; here st->iterations.  Krylov/multigrid method chosen by
  * tb_set_poisson_mode (the reference uses GMRES + BoomerAMG, Solver.c:90-265) */
 int tb_poisson_solve(tb_ctx *ctx, tb_solve_stats *st);
+/* Pressure_solve from a caller-supplied initial guess in field f_guess (no
+ * reference counterpart: the reference starts from the previous phi,
+ * Solver.c:256; this is the entry point for a guess predicted outside the
+ * library).  The guess is pinned and kept when its residual is not larger
+ * than that of the previous phi; otherwise the previous phi is restored and
+ * FP64 GMG-PCG solves from it (st->fallback bit 2).  Mode as set by
+ * tb_set_poisson_mode; f_guess is neither TB_F_PHI nor TB_F_PRHS. */
+int tb_poisson_solve_guess(tb_ctx *ctx, int f_guess, tb_solve_stats *st);
 /* Pressure_compute_real_pressure (Pressure.c:590-685, called before pressure
  * output, Output.c:2198-2204): TB_F_PRHS = -div(grad p) from TB_F_DPDX/Y/Z
  * (0 on BOUNDARY cells), then the Poisson solve into TB_F_PHI, which becomes
- * the next solve's initial guess as in the reference */
+ * the next solve's plain initial guess as in the reference.  This output
+ * solve leaves the learned mode's solution history, its Galerkin matrix, the
+ * snapshot store and the network's judgement as they were. */
 int tb_real_pressure_rhs(tb_ctx *ctx);
 /* diagnostics: Extract_total_kinetic_energy (Extract.c:803-884) and
  * Conc_compute_total_suspended_mass (Conc.c:1471-1512), both over FLUID
@@ -290,7 +305,10 @@ int tb_sedimentation_rate(tb_ctx *ctx, double *total, double *bottom);
 int tb_real_pressure(tb_ctx *ctx, tb_solve_stats *st);
 /* Pressure_project_velocity (Pressure.c:280-439, Pressure.h:10) */
 int tb_project(tb_ctx *ctx, double dt);
-/* Pressure_compute_velocity_divergence (Pressure.c:462-586, Pressure.h:12) */
+/* Pressure_compute_velocity_divergence (Pressure.c:462-586, Pressure.h:12);
+ * the same pass scans u, v, w and c for NaN / Inf on the non-BOUNDARY cells
+ * (Debugger_check_nan, Debugger.c:286-324) and returns -1 with the cell in
+ * ijk and in tb_last_error when it finds one (tb_step then fails too) */
 int tb_div_max(tb_ctx *ctx, double *div_max, int32_t *ijk);
 /* Velocity_store_old_data + Conc_store_old_data (Velocity.c:297-302,
  * Velocity.h:6; Conc.c:997-1003, Conc.h:31) for u, v, w, c */
@@ -315,7 +333,9 @@ int tb_set_helmholtz_rtol(tb_ctx *ctx, double rtol);
 #define TB_HELM_BICGSTAB 1
 int tb_set_helmholtz_method(tb_ctx *ctx, int method);
 /* mixed Poisson mode: smallest residual reduction one FP32 inner solve is
- * asked for (default 1e-5) and its iteration cap (default 40) */
+ * asked for (default 1e-5) and its iteration cap (default 40); a cap of 0
+ * leaves the refinement passes without FP32 work, so every mixed solve
+ * hands over to FP64 GMG-PCG at once (st->fallback bit 1) */
 int tb_set_mixed_inner(tb_ctx *ctx, double reduction_floor, int maxit);
 /* learned mode: number of previous solutions (1..6, default 6) whose
  * A-norm-optimal combination, with the network's direction when one is

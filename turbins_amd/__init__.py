@@ -57,6 +57,7 @@ def lib():
     L.tb_poisson_rhs.argtypes = [P, C.c_double]
     L.tb_poisson_apply.argtypes = [P, C.c_int, C.c_int]
     L.tb_poisson_solve.argtypes = [P, C.POINTER(abi.SolveStats)]
+    L.tb_poisson_solve_guess.argtypes = [P, C.c_int, C.POINTER(abi.SolveStats)]
     L.tb_project.argtypes = [P, C.c_double]
     L.tb_div_max.argtypes = [P, C.POINTER(C.c_double), P]
     L.tb_store_old.argtypes = [P]
@@ -179,6 +180,9 @@ class Solver:
         self.NT = int(np.prod(self.shape))
 
     def _ok(self, rc, what):
+        # status of the last call: 0 ok, > 0 finished without convergence
+        # or with a fallback taken (the stats say which)
+        self.last_rc = rc
         if rc < 0:
             raise TBError(f"{what}: {self.L.tb_last_error(self.h).decode()}")
         return rc
@@ -270,6 +274,14 @@ class Solver:
     def poisson_solve(self):
         st = abi.SolveStats()
         self._ok(self.L.tb_poisson_solve(self.h, C.byref(st)), "tb_poisson_solve")
+        return st
+
+    def poisson_solve_guess(self, guess):
+        """Pressure_solve from a caller-supplied initial guess (uploaded to
+        SCRATCH0); the previous PHI is kept if its residual is smaller."""
+        self.upload("SCRATCH0", guess)
+        st = abi.SolveStats()
+        self._ok(self.L.tb_poisson_solve_guess(self.h, abi.F["SCRATCH0"], C.byref(st)), "tb_poisson_solve_guess")
         return st
 
     def diagnostics(self):

@@ -237,7 +237,8 @@ struct tb_ctx {
     int ncx, ncy, ncz;
     Dims d;
     hipStream_t s = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;       // each solve's own timing
+    hipEvent_t step_ev[2] = {nullptr, nullptr};    // tb_step's timing
     int64_t bytes = 0;
 
     // host copies of the 1-D grid arrays, and their device copies
@@ -308,6 +309,7 @@ struct tb_ctx {
     double guess_r[GUESS_HIST] = {};
     int guess_nr = 0;
     bool phi_pinned = false;   // the phi field is the output of the last Poisson solve (corner exactly 0)
+    bool phi_is_hist0 = false; // ... and that solve stored it as phi_hist[0]
     int guess_hist = GUESS_DEFAULT; // solutions in the Galerkin initial guess (tb_set_guess_history)
     tb_net *net = nullptr;       // attached learned guess (not owned)
     tb_net *net_record = nullptr;    // snapshot sink after every solve (not owned)
@@ -384,5 +386,5 @@ double tb_elapsed_ms(tb_ctx *c);
 
 // every entry point that writes a caller-chosen field reports it here
 inline void tb_field_written(tb_ctx *c, int f) {
-    if (f == TB_F_PHI) c->phi_pinned = false;
+    if (f == TB_F_PHI) c->phi_pinned = c->phi_is_hist0 = false;
 }

@@ -92,7 +92,8 @@ extern "C" tb_ctx *tb_create(const tb_config *cfg) {
     c->d.plane = (long)c->d.px * c->d.NY;
     c->d.ntot = c->d.plane * c->d.NZ;
     if (hipStreamCreate(&c->s) != hipSuccess || hipEventCreate(&c->ev0) != hipSuccess ||
-        hipEventCreate(&c->ev1) != hipSuccess) {
+        hipEventCreate(&c->ev1) != hipSuccess || hipEventCreate(&c->step_ev[0]) != hipSuccess ||
+        hipEventCreate(&c->step_ev[1]) != hipSuccess) {
         delete c;
         return nullptr;
     }
@@ -160,6 +161,8 @@ extern "C" void tb_destroy(tb_ctx *c) {
         if (c->net_ev[e]) hipEventDestroy(c->net_ev[e]);
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
+    for (int e = 0; e < 2; e++)
+        if (c->step_ev[e]) hipEventDestroy(c->step_ev[e]);
     if (c->s) hipStreamDestroy(c->s);
     delete c;
 }

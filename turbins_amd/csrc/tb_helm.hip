@@ -868,6 +868,9 @@ extern "C" int tb_helmholtz_solve(tb_ctx *c, int q, double dt, tb_solve_stats *s
     double rtol = c->cfg.helm_rtol;
     int maxit = c->cfg.helm_maxit;
     TB_CHECK(c, hipEventRecord(c->ev0, c->s));
+    // 1 when the Gauss-Seidel sweeps stopped contracting (or ran out of
+    // pairs) and BiCGStab finished the solve
+    int handover = 0;
     if (c->helm_method == TB_HELM_GS) {
         int gits = 0;
         double g0 = 0.0, gres = 0.0, gbn = 1.0;
@@ -888,6 +891,7 @@ extern "C" int tb_helmholtz_solve(tb_ctx *c, int q, double dt, tb_solve_stats *s
             return 0;
         }
         // sweeps stalled: BiCGStab continues from the current x
+        handover = 1;
     }
     if (helm_apply(c, H, x, t, true, 0) < 0) return -1;
     hipLaunchKernelGGL(k_bicg_init, dim3(VB), dim3(VT), 0, c->s, n, b, t, r, rh, p, v, c->red);
@@ -933,7 +937,7 @@ extern "C" int tb_helmholtz_solve(tb_ctx *c, int q, double dt, tb_solve_stats *s
     if (st) {
         st->iterations = it;
         st->converged = tres <= rtol;
-        st->fallback = 0;
+        st->fallback = handover;
         st->vcycles = 0;
         st->res0 = res0;
         st->res = tres;

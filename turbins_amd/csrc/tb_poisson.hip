@@ -11,7 +11,9 @@
 // compatible right-hand side (sum b = 0, which the divergence RHS satisfies
 // up to rounding) that is the unique solution of the pinned system.
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <vector>
 
@@ -60,11 +62,6 @@ extern "C" int tb_real_pressure_rhs(tb_ctx *c) {
                        t);
     TB_LAUNCH_CHECK(c);
     return 0;
-}
-
-extern "C" int tb_real_pressure(tb_ctx *c, tb_solve_stats *st) {
-    if (tb_real_pressure_rhs(c)) return -1;
-    return tb_poisson_solve(c, st);
 }
 
 // ---------------------------------------------------------------------------
@@ -1666,7 +1663,7 @@ static int cg32_inner(tb_ctx *c, const KTiles &kt, double r0n, double tol_rel, i
 // rtol, at maxit inner iterations, or when an outer pass reduces the FP64
 // residual by less than half (stalled: the caller falls back to FP64 PCG).
 extern "C" int tb_set_mixed_inner(tb_ctx *c, double reduction_floor, int maxit) {
-    if (!c || !(reduction_floor > 0.0 && reduction_floor < 1.0) || maxit < 1) {
+    if (!c || !(reduction_floor > 0.0 && reduction_floor < 1.0) || maxit < 0) {
         if (c) c->err = "tb_set_mixed_inner: bad argument";
         return -1;
     }
@@ -1698,6 +1695,11 @@ static int mixed_ir(tb_ctx *c, const PoisCtx &P, double *x, const double *b, dou
         if (residual(c, P, true, x, b, r, L0.b32, rr, bb)) return -1;
     }
     res = sqrt(rr) / bn;
+    if (c->ir_maxit == 0 && res > rtol) {
+        // no FP32 work allowed: the first pass would leave x as it is
+        stalled = true;
+        return 0;
+    }
     while (res > rtol && its < maxit) {
         const double prev = res;
         double want = 0.5 * rtol / res;
@@ -2026,7 +2028,18 @@ extern "C" int tb_net_gate(tb_ctx *c, int *open, int *trials, double *saved_per_
     return 0;
 }
 
-extern "C" int tb_poisson_solve(tb_ctx *c, tb_solve_stats *st) {
+// One Poisson solve.  guess_f >= 0: the caller's initial guess in that
+// field; output: the solve of Pressure_compute_real_pressure, which leaves
+// the solution history, the Galerkin matrix, the snapshot store and the
+// network's judgement as they are.
+//
+// A guess (the caller's, or the learned mode's Galerkin combination) is
+// measured against the solve's plain start, the field as the previous solve
+// left it (the reference's initial guess, Solver.c:256): it is kept when its
+// residual is not larger; otherwise the previous field is restored and FP64
+// GMG-PCG, the plain path, solves from it (fallback bit 2).  A mixed solve
+// that stagnates hands over to FP64 GMG-PCG as well (fallback bit 1).
+static int poisson_solve(tb_ctx *c, tb_solve_stats *st, int guess_f, bool output) {
     if (!c) return -1;
     if (tb_mg_build(c)) return -1;
     PoisCtx P = pois_ctx(c);
@@ -2042,28 +2055,48 @@ extern "C" int tb_poisson_solve(tb_ctx *c, tb_solve_stats *st) {
     memset(&loc, 0, sizeof(loc));
     int its = 0, vcy = 0;
     double res = 0.0;
-    // Learned mode with a stored Galerkin matrix: the guess replaces x first;
-    // its residual pass also gives ||b||, and the guess is kept when it is
-    // better than the zero vector (relative residual < 1; the previous
-    // solution, the only other candidate, is in its span and typically
-    // worse than zero, this phi being an increment).  Otherwise x returns
-    // to the previous solution, measured as in the other modes.
-    bool guess_resid = false;   // lev[0].b32 holds the residual of the accepted guess
-    double rr_guess = 0.0;
-    if (mode == TB_POIS_LEARNED_IR && c->n_hist >= 1 && c->gram_n >= 1) {
-        double bbg = 0.0;
+    bool plain = mode == TB_POIS_FP64_PCG;
+    c->net_used = false;
+    const bool learned = !output && guess_f < 0 && mode == TB_POIS_LEARNED_IR && c->n_hist >= 1 && c->gram_n >= 1;
+    // guess_resid: x holds an accepted guess and rr its residual, whose FP32
+    // copy (mixed) or FP64 vector kv[0] (plain) the method starts from
+    bool guess_resid = false;
+    if (learned || guess_f >= 0) {
+        // the plain start: the previous solution, which the learned mode
+        // keeps as phi_hist[0]; any other field is saved (pinned) in kv[7]
+        const double *prev = c->phi_is_hist0 ? c->phi_hist[0] : c->kv[7];
+        if (!c->phi_is_hist0) {
+            TB_CHECK(c, hipMemcpyAsync(c->kv[7], x, sizeof(double) * c->d.ntot, hipMemcpyDeviceToDevice, c->s));
+            if (!c->phi_pinned && shift_corner(c, P, c->kv[7])) return -1;
+        }
+        double rr_g = 0.0, bb_g = 0.0;
         int ndir = 0;
-        if (learned_guess(c, P, x, b, rr_guess, bbg, ndir)) return -1;
-        if (ndir > 0 && bbg > 0.0 && rr_guess < bbg) {
-            guess_resid = true;
-            bb = bbg;
-            rr = rr_guess;
+        if (learned) {
+            if (learned_guess(c, P, x, b, rr_g, bb_g, ndir)) return -1;
         } else {
-            loc.fallback = 2;
-            // keep the previous solution (still in phi_hist[0], pinned)
-            if (ndir > 0)
-                TB_CHECK(c, hipMemcpyAsync(x, c->phi_hist[0], sizeof(double) * c->d.ntot, hipMemcpyDeviceToDevice,
-                                           c->s));
+            TB_CHECK(c, hipMemcpyAsync(x, c->f[guess_f], sizeof(double) * c->d.ntot, hipMemcpyDeviceToDevice, c->s));
+            if (shift_corner(c, P, x)) return -1;
+            if (residual(c, P, !plain, x, b, plain ? c->kv[0] : nullptr, plain ? nullptr : c->lev[0].b32, rr_g, bb_g))
+                return -1;
+            ndir = 1;
+        }
+        if (ndir > 0 && bb_g > 0.0) {
+            // residual norm of the plain start (nothing written)
+            double rr_p = 0.0, bb_p = 0.0;
+            if (residual(c, P, true, prev, b, nullptr, nullptr, rr_p, bb_p)) return -1;
+            if (rr_g <= rr_p) {
+                guess_resid = true;
+                rr = rr_g;
+                bb = bb_g;
+            } else {
+                loc.fallback = 2;
+                plain = true;
+            }
+        }
+        if (!guess_resid && ndir > 0) {
+            // back to the previous solution (pinned)
+            TB_CHECK(c, hipMemcpyAsync(x, prev, sizeof(double) * c->d.ntot, hipMemcpyDeviceToDevice, c->s));
+            c->phi_pinned = true;
         }
     }
     double rr_top = 0.0;
@@ -2074,24 +2107,24 @@ extern "C" int tb_poisson_solve(tb_ctx *c, tb_solve_stats *st) {
         // this residual is also the first one of the method that follows:
         // FP64 PCG takes r (kv[0]), the mixed solve its FP32 copy (x is
         // pinned, so the Neumann and the pinned residual are the same vector)
-        double *r_top = (mode == TB_POIS_FP64_PCG) ? c->kv[0] : nullptr;
-        float *r32_top = (mode != TB_POIS_FP64_PCG) ? c->lev[0].b32 : nullptr;
-        if (residual(c, P, mode != TB_POIS_FP64_PCG, x, b, r_top, r32_top, rr, bb)) return -1;
-        rr_top = rr;
+        double *r_top = plain ? c->kv[0] : nullptr;
+        float *r32_top = plain ? nullptr : c->lev[0].b32;
+        if (residual(c, P, !plain, x, b, r_top, r32_top, rr, bb)) return -1;
     }
+    rr_top = rr;
     c->phi_pinned = false;
+    c->phi_is_hist0 = false;
     double bn = sqrt(bb);
     if (bn == 0.0) {
         TB_CHECK(c, hipMemsetAsync(x, 0, sizeof(double) * c->d.ntot, c->s));
         loc.converged = 1;
     } else {
         loc.res0 = sqrt(rr) / bn;
-        if (mode == TB_POIS_FP64_PCG) {
+        if (plain) {
             if (pcg64(c, P, x, b, bn, rtol, maxit, its, vcy, res, &rr_top)) return -1;
         } else {
             bool stalled = false;
-            const double *rr_known = guess_resid ? &rr_guess : &rr_top;
-            if (mixed_ir(c, P, x, b, bn, rtol, maxit, its, vcy, res, stalled, rr_known)) return -1;
+            if (mixed_ir(c, P, x, b, bn, rtol, maxit, its, vcy, res, stalled, &rr_top)) return -1;
             if (res > rtol) {
                 loc.fallback |= 1;   // plain FP64 GMG-PCG takes over
                 if (pcg64(c, P, x, b, bn, rtol, maxit, its, vcy, res, nullptr)) return -1;
@@ -2108,14 +2141,17 @@ extern "C" int tb_poisson_solve(tb_ctx *c, tb_solve_stats *st) {
     // pass, or all zero)
     c->phi_pinned = true;
     // history of solutions for the initial guess and the network's inputs,
-    // newest first: the pointers
-    // rotate, one copy per solve; kept only where something reads it
-    if (c->cfg.pois_mode == TB_POIS_LEARNED_IR || c->net || c->net_record) {
+    // newest first: the pointers rotate, one copy per solve; kept only where
+    // something reads it, and not for output solves
+    if (output) {
+        // the history and its Galerkin matrix stay as they are
+    } else if (c->cfg.pois_mode == TB_POIS_LEARNED_IR || c->net || c->net_record) {
         double *oldest = c->phi_hist[GUESS_HIST - 1];
         for (int k = GUESS_HIST - 1; k > 0; k--) c->phi_hist[k] = c->phi_hist[k - 1];
         c->phi_hist[0] = oldest;
         TB_CHECK(c, hipMemcpyAsync(c->phi_hist[0], x, sizeof(double) * c->d.ntot, hipMemcpyDeviceToDevice, c->s));
         if (c->n_hist < GUESS_HIST) c->n_hist++;
+        c->phi_is_hist0 = true;
         // the last residual pass of every solve above ran on the final x
         // (x = 0 when b = 0) and left x . b in S_RES + 2
         if (c->cfg.pois_mode == TB_POIS_LEARNED_IR) gram_push(c, bn == 0.0 ? 0.0 : c->hscal[S_RES + 2]);
@@ -2123,7 +2159,7 @@ extern "C" int tb_poisson_solve(tb_ctx *c, tb_solve_stats *st) {
     } else {
         c->n_hist = c->gram_n = c->guess_nr = 0;
     }
-    if (c->net_record && c->n_hist >= 3 && tb_net_record_one(c)) return -1;
+    if (!output && c->net_record && c->n_hist >= 3 && tb_net_record_one(c)) return -1;
     TB_CHECK(c, hipEventRecord(c->ev1, c->s));
     if (tb_sync(c)) return -1;
     loc.iterations = its;
@@ -2132,6 +2168,21 @@ extern "C" int tb_poisson_solve(tb_ctx *c, tb_solve_stats *st) {
     loc.ms = tb_elapsed_ms(c);
     if (st) *st = loc;
     return loc.converged ? 0 : 1;
+}
+
+extern "C" int tb_poisson_solve(tb_ctx *c, tb_solve_stats *st) { return poisson_solve(c, st, -1, false); }
+
+extern "C" int tb_poisson_solve_guess(tb_ctx *c, int f_guess, tb_solve_stats *st) {
+    if (!c || f_guess < 0 || f_guess >= TB_F_COUNT || f_guess == TB_F_PHI || f_guess == TB_F_PRHS) {
+        if (c) c->err = "tb_poisson_solve_guess: bad field";
+        return -1;
+    }
+    return poisson_solve(c, st, f_guess, false);
+}
+
+extern "C" int tb_real_pressure(tb_ctx *c, tb_solve_stats *st) {
+    if (tb_real_pressure_rhs(c)) return -1;
+    return poisson_solve(c, st, -1, true);
 }
 
 // ---------------------------------------------------------------------------
@@ -2184,12 +2235,15 @@ extern "C" int tb_project(tb_ctx *c, double dt) {
 }
 
 // Pressure_compute_velocity_divergence (Pressure.c:462-586): |div| per cell
-// and the maximum with the last (k, j, i)-ordered cell on ties.
+// and the maximum with the last (k, j, i)-ordered cell on ties.  The same
+// pass is the NaN / Inf scan of Debugger_check_nan (Debugger.c:286-324) over
+// the non-BOUNDARY cells: a cell whose divergence or concentration is not
+// finite ranks as +infinity, so the maximum reports it with its index.
 __global__ void __launch_bounds__(256) k_div(Dims d, GridDev g, const double *__restrict__ u,
                                              const double *__restrict__ v, const double *__restrict__ w,
-                                             const uint8_t *__restrict__ sc, double *__restrict__ div,
-                                             double *pmax, long *pidx, double a_dEta, double a_dXi, double a_dPhi,
-                                             Tiles t) {
+                                             const double *__restrict__ cc, const uint8_t *__restrict__ sc,
+                                             double *__restrict__ div, double *pmax, long *pidx, double a_dEta,
+                                             double a_dXi, double a_dPhi, Tiles t) {
     __shared__ double sv[256];
     __shared__ long si[256];
     int tid = threadIdx.y * blockDim.x + threadIdx.x;
@@ -2199,14 +2253,17 @@ __global__ void __launch_bounds__(256) k_div(Dims d, GridDev g, const double *__
     if (tile_point(t, i, j, k)) {
         long id = gidx(d, i, j, k);
         double dc = 0.0;
+        bool finite = true;
         if (sc[id] != TB_BOUNDARY) {
             double dudx = (u[id + 1] - u[id]) * a_dEta * g.mxc[i];
             double dvdy = (v[id + d.px] - v[id]) * a_dXi * g.myc[j];
             double dwdz = (w[id + d.plane] - w[id]) * a_dPhi * g.mzc[k];
             dc = fabs(dudx + dvdy + dwdz);
+            // false for NaN and +-Inf
+            finite = dc <= DBL_MAX && fabs(cc[id]) <= DBL_MAX;
         }
         div[id] = dc;
-        best = dc;
+        best = finite ? dc : INFINITY;
         bidx = ((long)k * d.NY + j) * d.NX + i;
     }
     sv[tid] = best;
@@ -2257,7 +2314,8 @@ extern "C" int tb_div_max(tb_ctx *c, double *div_max, int32_t *ijk) {
     double *pm = c->red;
     long *pi = reinterpret_cast<long *>(c->red + RED_MAX_BLOCKS);
     hipLaunchKernelGGL(k_div, tiles_grid(t), tiles_block(), 0, c->s, c->d, c->g, c->f[TB_F_U], c->f[TB_F_V],
-                       c->f[TB_F_W], c->st[3], c->f[TB_F_DIV], pm, pi, 1.0 / c->dEta, 1.0 / c->dXi, 1.0 / c->dPhi, t);
+                       c->f[TB_F_W], c->f[TB_F_C], c->st[3], c->f[TB_F_DIV], pm, pi, 1.0 / c->dEta, 1.0 / c->dXi,
+                       1.0 / c->dPhi, t);
     hipLaunchKernelGGL(k_div_final, dim3(1), dim3(1024), 0, c->s, pm, pi, nb, c->scal + S_TMP0,
                        reinterpret_cast<long *>(c->scal + S_TMP1));
     TB_LAUNCH_CHECK(c);
@@ -2265,11 +2323,19 @@ extern "C" int tb_div_max(tb_ctx *c, double *div_max, int32_t *ijk) {
     const double h = c->hscal[S_TMP0];
     long hi;
     memcpy(&hi, &c->hscal[S_TMP1], sizeof(long));
+    const int32_t ci = (int32_t)(hi % c->d.NX), cj = (int32_t)((hi / c->d.NX) % c->d.NY),
+                  ck = (int32_t)(hi / ((long)c->d.NX * c->d.NY));
     if (div_max) *div_max = h;
     if (ijk) {
-        ijk[0] = (int32_t)(hi % c->d.NX);
-        ijk[1] = (int32_t)((hi / c->d.NX) % c->d.NY);
-        ijk[2] = (int32_t)(hi / ((long)c->d.NX * c->d.NY));
+        ijk[0] = ci;
+        ijk[1] = cj;
+        ijk[2] = ck;
+    }
+    if (!(h <= DBL_MAX)) {
+        char buf[160];
+        snprintf(buf, sizeof(buf), "NaN or Inf in u, v, w or c at cell (i,j,k)=(%d,%d,%d)", ci, cj, ck);
+        c->err = buf;
+        return -1;
     }
     return 0;
 }

@@ -13,9 +13,7 @@ extern "C" int tb_step(tb_ctx *c, double dt, tb_step_stats *st) {
     if (!c || !(dt > 0.0)) { if (c) c->err = "tb_step: bad argument"; return -1; }
     tb_step_stats loc;
     memset(&loc, 0, sizeof(loc));
-    hipEvent_t e0, e1;
-    TB_CHECK(c, hipEventCreate(&e0));
-    TB_CHECK(c, hipEventCreate(&e1));
+    hipEvent_t e0 = c->step_ev[0], e1 = c->step_ev[1];
     TB_CHECK(c, hipEventRecord(e0, c->s));
     int worst = 0;
     if (tb_store_old(c)) return -1;
@@ -40,9 +38,7 @@ extern "C" int tb_step(tb_ctx *c, double dt, tb_step_stats *st) {
     TB_CHECK(c, hipEventRecord(e1, c->s));
     TB_CHECK(c, hipEventSynchronize(e1));
     float ms = 0.f;
-    hipEventElapsedTime(&ms, e0, e1);
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    TB_CHECK(c, hipEventElapsedTime(&ms, e0, e1));
     loc.div_max = dm;
     loc.ms = ms;
     if (st) *st = loc;
