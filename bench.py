@@ -16,12 +16,16 @@ The JSON line's `value` is the headline path's timesteps/s.  Inputs are
 resident in HBM before the timed region; each library step ends with a
 device synchronisation, so the wall clock brackets exactly K steps.
 
+Both paths start from one snapshot taken after a few untimed FP64 steps
+(past the seeded start's transient) and run W untimed then K timed steps
+from it, so they time the same stretch of the trajectory.
+
 `roofline` reports the instrumented kernel with the largest total time in
 a few further headline-path steps run with HIP events on (outside the timed
 region): algorithmic bytes per launch / average launch time
 from HIP events on the library's stream.  `cpu_baseline` times the FP64
 oracle (the reference's algorithm: unpreconditioned BiCGStab, GMG-PCG
-Poisson) for one step of the same workload on the host cores.
+Poisson) on the same workload from the same snapshot on the host cores.
 """
 import argparse
 import json
@@ -57,7 +61,23 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run_path(g, c, steps, warmup, mode, rtol):
+STATE = ["U", "V", "W", "C", "DPDX", "DPDY", "DPDZ", "PHI"]
+
+
+def snapshot(g):
+    return {n: g.download(n) for n in STATE}
+
+
+def restore(g, snap):
+    # uploading PHI also empties the learned mode's solution history
+    for n in STATE:
+        g.upload(n, snap[n])
+
+
+def run_path(g, c, snap, steps, warmup, mode, rtol):
+    """W untimed and K timed steps from the snapshot: both paths run the same
+    stretch of the trajectory (steps W+1 .. W+K after the snapshot)."""
+    restore(g, snap)
     g.set_poisson_mode(mode, rtol)
     for _ in range(warmup):
         g.step(c["dt"])
@@ -143,30 +163,65 @@ def roofline(prof, stats, c):
             "bytes_per_launch": int(per_launch), "avg_launch_us": round(avg_s * 1e6, 2), "launches": n}
 
 
-def cpu_baseline(c, g, threads):
-    """One SSP-RK2 step of the same workload on the FP64 oracle, with the
-    device's geometry installed (the setup chain is not part of a step)."""
+def _omp_threads(n):
+    """Thread count of the oracle's OpenMP loops (libgomp, loaded with it)."""
+    import ctypes
+    ctypes.CDLL("libgomp.so.1").omp_set_num_threads(int(n))
+
+
+def cpu_baseline(c, g, snap, threads, warmup, reps):
+    """The same workload on the FP64 oracle (the reference's algorithm:
+    unpreconditioned BiCGStab Helmholtz, GMG-PCG Poisson), from the bench's
+    snapshot, with the device's geometry installed (the setup chain is not
+    part of a step) and the matrices assembled before the clock starts:
+    `warmup` untimed and `reps` timed SSP-RK2 steps on `threads` threads
+    (median per step), then one RK stage on one thread."""
+    # OpenMP reads the binding when it starts, i.e. with the oracle's first
+    # parallel loop
+    os.environ.setdefault("OMP_PROC_BIND", "close")
     import oracle
     from turbins_amd import workload as W
-    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+    _omp_threads(threads)
     cfg = W.config_for(c, pois_rtol=1e-8)
     o = oracle.Oracle(cfg)
     for q in range(4):
         o.set_geometry(q, g.flags(q), g.ib_table(q))
-    for name in ["U", "V", "W", "C", "DPDX", "DPDY", "DPDZ", "PHI"]:
-        o.upload(name, g.download(name))
+    for name in STATE:
+        o.upload(name, snap[name])
     x = np.zeros(o.NT)
     t = time.perf_counter()
-    for q in range(4):                  # assemble the matrices outside the timed step
+    for q in range(4):
         o.helmholtz_apply(q, c["dt"], x)
     o.poisson_apply(x)
     o.gmg_levels()
     log(f"cpu baseline: assembly {time.perf_counter() - t:.1f}s")
+    for _ in range(warmup):
+        o.step(c["dt"])
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        o.step(c["dt"])
+        times.append(time.perf_counter() - t0)
+    log("cpu baseline: " + ", ".join(f"{v:.2f}" for v in times) + f" s per step on {threads} threads")
+    # one stage of ora_step's sequence on one thread
+    _omp_threads(1)
+    dt = c["dt"]
     t0 = time.perf_counter()
-    st = o.step(c["dt"])
-    dt_s = time.perf_counter() - t0
+    o.store_old()
+    for q in range(4):
+        o.eno_conv(q)
+    for q in range(4):
+        o.set_rhs(q, dt)
+    for q in range(4):
+        o.helmholtz_solve(q, dt, rtol=cfg.helm_rtol, maxit=cfg.helm_maxit)
+    o.poisson_rhs(dt)
+    o.poisson_solve(rtol=cfg.pois_rtol, maxit=cfg.pois_maxit)
+    o.project(dt)
+    stage_1 = time.perf_counter() - t0
+    _omp_threads(threads)
+    log(f"cpu baseline: one RK stage on 1 thread {stage_1:.1f}s")
     o.close()
-    return 1.0 / dt_s, dt_s, st
+    return float(np.median(times)), times, stage_1
 
 
 def main():
@@ -178,7 +233,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch the V-cycle kernels one by one (for tracing tools that do not follow HIP graphs)")
+    ap.add_argument("--settle", type=int, default=4,
+                    help="untimed FP64 steps from the seeded start before the snapshot both paths run from")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
+    ap.add_argument("--cpu-warmup", type=int, default=1)
+    ap.add_argument("--cpu-reps", type=int, default=3)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
@@ -198,7 +257,13 @@ def main():
     for k, v in W.initial_fields(c, flags).items():
         g.upload(k, v)
 
-    t_plain, st_plain = run_path(g, c, args.steps, args.warmup, abi.POIS_FP64_PCG, 1e-8)
+    # past the seeded start's transient (its non-solenoidal part decays over
+    # the first steps), then one snapshot for both paths
+    g.set_poisson_mode(abi.POIS_FP64_PCG, 1e-8)
+    for _ in range(args.settle):
+        g.step(c["dt"])
+    snap = snapshot(g)
+    t_plain, st_plain = run_path(g, c, snap, args.steps, args.warmup, abi.POIS_FP64_PCG, 1e-8)
     plain_sps = args.steps / t_plain
     from turbins_amd import Net
     net = None
@@ -208,7 +273,7 @@ def main():
         g.attach_net(net)
     learned = net is not None
     mode = abi.POIS_LEARNED_IR
-    t_head, st_head = run_path(g, c, args.steps, args.warmup, mode, 1e-8)
+    t_head, st_head = run_path(g, c, snap, args.steps, args.warmup, mode, 1e-8)
     head_sps = args.steps / t_head
     prof, st_prof = profile_path(g, c, max(2, min(args.steps, 4)))
     gate = g.net_gate() if learned else None
@@ -287,12 +352,17 @@ def main():
         "roofline": rl,
     }
     if not args.no_cpu:
-        sps, secs, st = cpu_baseline(c, g, args.cpu_threads)
-        log(f"cpu baseline: {secs:.1f}s per step")
+        secs, times, stage_1 = cpu_baseline(c, g, snap, args.cpu_threads, args.cpu_warmup, args.cpu_reps)
         result["cpu_baseline"] = {
-            "value": round(sps, 6), "unit": "timesteps/s", "cores": args.cpu_threads, "kind": "port",
-            "sample": f"1 SSP-RK2 step of {args.workload} ({c['nx']}x{c['ny']}x{c['nz']}) on the FP64 C oracle "
-                      "(unpreconditioned BiCGStab Helmholtz, GMG-PCG Poisson, OpenMP), device geometry installed",
+            "value": round(1.0 / secs, 6), "unit": "timesteps/s", "cores": args.cpu_threads, "kind": "port",
+            "sample": f"{args.workload} ({c['nx']}x{c['ny']}x{c['nz']}) on the FP64 C oracle (unpreconditioned "
+                      f"BiCGStab Helmholtz, GMG-PCG Poisson, OpenMP, OMP_PROC_BIND={os.environ.get('OMP_PROC_BIND', 'unset')}) "
+                      f"from the bench's snapshot, device geometry installed: {args.cpu_warmup} untimed and "
+                      f"{args.cpu_reps} timed SSP-RK2 steps on {args.cpu_threads} threads, median per step; "
+                      "one_core: one RK stage on one thread, doubled",
+            "reps_s": [round(v, 3) for v in times],
+            "one_core": round(1.0 / (2.0 * stage_1), 6),
+            "host_cpus": os.cpu_count(),
         }
     print(json.dumps(result), flush=True)
 

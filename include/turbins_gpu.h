@@ -189,6 +189,14 @@ int tb_download_ib_table(tb_ctx *ctx, int q, tb_ib_node *out, int capacity);
 /* pseudo-time iterations taken by Levelset_reinitialize (Levelset.c:140) */
 int tb_reinit_iterations(tb_ctx *ctx);
 
+/* grid coordinates and metrics (Grid_generate_mesh_coordinates,
+ * Grid_compute_metric_coefficients, Grid.c:470-708: grid->xc ... metric_zw);
+ * which: 0 xc 1 yc 2 zc 3 xu 4 yu 5 zu 6 xv 7 yv 8 zv 9 xw 10 yw 11 zw
+ * 12 dx_c 13 dy_c 14 dz_c 15 metric_xc 16 metric_yc 17 metric_zc
+ * 18 metric_xu 19 metric_yv 20 metric_zw.  Returns the length (nx+1, ny+1
+ * or nz+1) or -1. */
+int tb_grid_array(tb_ctx *ctx, int which, double *out, int capacity);
+
 /* fields ------------------------------------------------------------------ */
 /* host layout = the reference's DA natural order (padded, x fastest, FP64);
  * replaces the DAVecGetArray access to the G_data Vecs (e.g. Velocity.c:88) */
@@ -205,6 +213,9 @@ int tb_download_field(tb_ctx *ctx, int f, double *host);
 #define TB_IO_NOT_A_VEC (-3)   /* wrong class id, negative length or bad record */
 #define TB_IO_SIZE (-4)        /* length larger than the buffer / the field */
 #define TB_IO_SHORT (-5)       /* short read or write */
+#define TB_IO_ZERO_STEP 1      /* warning: Resume_primary.dat's dt or dt_old reads as 0 (its
+                                  six decimals lose any step below 5e-7); the values are
+                                  still returned */
 int tb_petsc_write_array(const char *path, const double *a, int64_t n);
 /* returns N (<= capacity) or a TB_IO_ code */
 int64_t tb_petsc_read_array(const char *path, double *a, int64_t capacity);
@@ -219,6 +230,7 @@ typedef struct {
     int32_t iter, output_iter;
 } tb_resume_info;
 int tb_write_resume_primary(const char *dir, const tb_resume_info *info);
+/* 0, TB_IO_ZERO_STEP (record read, a step is 0) or a negative TB_IO_ code */
 int tb_read_resume_primary(const char *dir, tb_resume_info *info);
 /* the restart set of Resume_write_data / Resume_read_data with the files of
  * Resume_set_data (Resume.c:172-244): Resume_u, _v, _w, _dpdx, _dpdy, _dpdz,
@@ -226,6 +238,7 @@ int tb_read_resume_primary(const char *dir, tb_resume_info *info);
  * written by the reference only on request, is not: the geometry is set up
  * again from the topography) */
 int tb_write_resume(tb_ctx *ctx, const char *dir, const tb_resume_info *info);
+/* 0, TB_IO_ZERO_STEP (fields read; dt or dt_old is 0) or -1 */
 int tb_read_resume(tb_ctx *ctx, const char *dir, tb_resume_info *info);
 
 /* per-operation mirrors of the reference entry points ---------------------- */
@@ -316,6 +329,17 @@ int tb_store_old(tb_ctx *ctx);
 /* Velocity_rk_average + Conc_rk_average (Velocity.c:2047-2055,
  * Velocity.h:19; Conc.c:1007-1017, Conc.h:19): q = old_frac*q_old + new_frac*q */
 int tb_rk_average(tb_ctx *ctx, double old_frac, double new_frac);
+
+/* CFL time step of the host driver (the reference's parameters cfl_method,
+ * alpha = 0.5 and default_dt, Input.c:421-423; its rule lived in the
+ * unpublished driver, README.md:26): over the FLUID cells of the
+ * concentration grid, with the cell-centre velocities of Velocity_cell_center
+ * (Velocity.c:1476-1602), rate = |u_c|/dx + |v_c|/dy + |w_c|/dz and
+ * dt = min(dt_max, alpha / max rate) (dt_max when the velocity is 0).
+ * rate (may be NULL) receives the maximum; NaN / Inf velocities fail.  The
+ * Helmholtz rows take dt per call, which is the diagonal the reference's
+ * Velocity_modify_diagonal(dt, dt_old) accumulates (+1/dt - 1/dt_old). */
+int tb_cfl_dt(tb_ctx *ctx, double alpha, double dt_max, double *dt, double *rate);
 
 /* composite SSP-RK2 step: the per-step sequence of the unpublished driver
  * (README.md:26), reconstructed from the TVD_RK2 scheme (definitions.h:148-149) */

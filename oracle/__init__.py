@@ -63,6 +63,8 @@ def lib():
         L.ora_store_old.argtypes = [P]
         L.ora_rk_average.argtypes = [P, C.c_double, C.c_double]
         L.ora_step.argtypes = [P, C.c_double, C.POINTER(abi.StepStats)]
+        L.ora_cfl_dt.restype = C.c_double
+        L.ora_cfl_dt.argtypes = [P, C.c_double, C.c_double, C.POINTER(C.c_double)]
         L.ora_gmg_levels.argtypes = [P, P, C.c_int]
         L.ora_helmholtz_matrix.argtypes = [P, C.c_int, C.c_double, P, P, P]
         L.ora_poisson_matrix.argtypes = [P, P, P, P]
@@ -221,6 +223,12 @@ class Oracle:
         st = abi.StepStats()
         self.L.ora_step(self.h, dt, C.byref(st))
         return st
+
+    def cfl_dt(self, alpha, dt_max):
+        """(dt, rate): the CFL step of the host driver's rule."""
+        r = C.c_double()
+        dt = self.L.ora_cfl_dt(self.h, alpha, dt_max, C.byref(r))
+        return dt, r.value
 
     def _csr(self, cnt, col, val):
         import scipy.sparse as sp

@@ -961,6 +961,28 @@ int ora_store_old(ora_ctx *o) {
     return 0;
 }
 
+/* CFL time step.  The reference's parameters are cfl_method, alpha and
+ * default_dt (Input.c:421-423); its rule lived in the unpublished driver.
+ * Restated rule (the same as the device's): over FLUID cells of the
+ * concentration grid, with the cell-centre velocities of
+ * Velocity_cell_center (Velocity.c:1476-1602),
+ * rate = |u_c|/dx + |v_c|/dy + |w_c|/dz; dt = min(dt_max, alpha / max rate). */
+double ora_cfl_dt(ora_ctx *o, double alpha, double dt_max, double *rate) {
+    cell_center(o);
+    double m = 0.0;
+    for (int k = 0; k < o->NZ; k++)
+        for (int j = 0; j < o->NY; j++)
+            for (int i = 0; i < o->NX; i++) {
+                size_t id = ora_idx(o, i, j, k);
+                if (o->st[TB_Q_C][id] != TB_FLUID) continue;
+                double r = fabs(o->ubc[id]) / o->dx_c[i] + fabs(o->vbc[id]) / o->dy_c[j] +
+                           fabs(o->wbc[id]) / o->dz_c[k];
+                if (r != r || r > m) m = r;
+            }
+    if (rate) *rate = m;
+    return (m > 0.0 && alpha / m < dt_max) ? alpha / m : dt_max;
+}
+
 /* VecAXPBY(y, a, b, x): y = a*x + b*y  (Velocity.c:2047-2055, Conc.c:1007-1017) */
 int ora_rk_average(ora_ctx *o, double old_frac, double new_frac) {
     for (int q = 0; q < 4; q++) {

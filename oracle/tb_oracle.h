@@ -66,6 +66,7 @@ double ora_div_max(ora_ctx *o, int32_t *ijk);
 int ora_store_old(ora_ctx *o);
 int ora_rk_average(ora_ctx *o, double old_frac, double new_frac);
 int ora_step(ora_ctx *o, double dt, tb_step_stats *st);
+double ora_cfl_dt(ora_ctx *o, double alpha, double dt_max, double *rate);
 
 /* GMG hierarchy description (for tests): returns number of levels and fills
  * dims[3*l..3*l+2] */

@@ -76,10 +76,22 @@ def test_resume_primary_record(tmp_path):
     assert text == PB.primary_text(12.3456789, 1e-3, 2.5e-4, 4000, 17)
     assert text.splitlines()[1] == "12.345679"     # the reference's %lf: six decimals
     got = T.read_resume_primary(tmp_path)
-    assert got == {"time": 12.345679, "dt": 0.001, "dt_old": 0.00025, "iteration": 4000, "output_iter": 17}
+    assert got == {"time": 12.345679, "dt": 0.001, "dt_old": 0.00025, "iteration": 4000, "output_iter": 17,
+                   "zero_step": False}
     (tmp_path / "Resume_primary.dat").write_text("time=\n1.0\ndt=\n0.1\n")
     with pytest.raises(OSError):
         T.read_resume_primary(tmp_path)
     os.remove(tmp_path / "Resume_primary.dat")
     with pytest.raises(OSError, match="open"):
         T.read_resume_primary(tmp_path)
+
+
+def test_resume_primary_zero_step_warning(tmp_path):
+    """Resume_primary.dat keeps the reference's %lf (six decimals,
+    Resume.c:272): a dt of 1e-7 is written as 0.000000 and read back as 0,
+    which the reader reports (zero_step) instead of passing silently."""
+    T.write_resume_primary(tmp_path, 1.0, 1e-7, 1e-3, 3, 0)
+    got = T.read_resume_primary(tmp_path)
+    assert got["dt"] == 0.0 and got["dt_old"] == 1e-3 and got["zero_step"]
+    T.write_resume_primary(tmp_path, 1.0, 2e-3, 1e-3, 3, 0)
+    assert not T.read_resume_primary(tmp_path)["zero_step"]

@@ -567,3 +567,25 @@ def test_projection_rotational_pressure_gradient_closed_form():
         np.testing.assert_allclose(got[m], want[name].ravel()[m], rtol=1e-12, atol=1e-12)
         assert np.all(got[~m] == 0.5), name
     o.close()
+
+
+# ---------------------------------------------------------------------------
+# CFL time step of the host driver (the reference keeps alpha and default_dt,
+# Input.c:421-423; the rule is the driver's, restated the same in the oracle)
+
+def test_cfl_time_step_closed_form():
+    """A uniform velocity (U, V, W) on a box without topography: every cell
+    has |U|/dx + |V|/dy + |W|/dz, so dt = alpha / that rate, and the cap
+    dt_max applies when the rate is small or zero."""
+    cfg = _cfg(16, 8, 8, 4.0, 1.0, 2.0)
+    o = _oracle(cfg)
+    for name, val in (("U", 0.5), ("V", -0.25), ("W", 0.125)):
+        o.upload(name, np.full(o.NT, val))
+    dt, rate = o.cfl_dt(0.5, 10.0)
+    want = 0.5 / 0.25 + 0.25 / 0.125 + 0.125 / 0.25
+    assert rate == want and dt == 0.5 / want
+    assert o.cfl_dt(0.5, 0.01)[0] == 0.01
+    for name in "UVW":
+        o.upload(name, np.zeros(o.NT))
+    assert o.cfl_dt(0.5, 0.02) == (0.02, 0.0)
+    o.close()

@@ -46,6 +46,7 @@ def lib():
     L.tb_ib_count.argtypes = [P, C.c_int]
     L.tb_download_ib_table.argtypes = [P, C.c_int, P, C.c_int]
     L.tb_reinit_iterations.argtypes = [P]
+    L.tb_grid_array.argtypes = [P, C.c_int, P, C.c_int]
     L.tb_upload_field.argtypes = [P, C.c_int, P]
     L.tb_download_field.argtypes = [P, C.c_int, P]
     L.tb_eno_conv.argtypes = [P, C.c_int]
@@ -63,6 +64,7 @@ def lib():
     L.tb_store_old.argtypes = [P]
     L.tb_rk_average.argtypes = [P, C.c_double, C.c_double]
     L.tb_step.argtypes = [P, C.c_double, C.POINTER(abi.StepStats)]
+    L.tb_cfl_dt.argtypes = [P, C.c_double, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.tb_set_poisson_mode.argtypes = [P, C.c_int, C.c_double]
     L.tb_set_helmholtz_rtol.argtypes = [P, C.c_double]
     L.tb_set_mixed_inner.argtypes = [P, C.c_double, C.c_int]
@@ -157,14 +159,18 @@ def write_resume_primary(directory, time, dt, dt_old, iteration, output_iter):
 
 
 def read_resume_primary(directory):
+    """The Resume_primary.dat record; zero_step says that dt or dt_old read
+    as 0 (the record's six decimals lose any step below 5e-7)."""
     info = abi.ResumeInfo()
-    _io_check(lib().tb_read_resume_primary(os.fsencode(directory), C.byref(info)), "read_resume_primary", directory)
+    rc = _io_check(lib().tb_read_resume_primary(os.fsencode(directory), C.byref(info)), "read_resume_primary",
+                   directory)
     return {"time": info.time, "dt": info.dt, "dt_old": info.dt_old, "iteration": info.iter,
-            "output_iter": info.output_iter}
+            "output_iter": info.output_iter, "zero_step": rc == 1}
 
 
 class TBError(RuntimeError):
     pass
+
 
 
 class Solver:
@@ -223,6 +229,15 @@ class Solver:
         arr = (abi.IBNode * max(n, 1))()
         self._ok(self.L.tb_download_ib_table(self.h, q, C.cast(arr, C.c_void_p), n), "tb_download_ib_table")
         return list(arr)[:n]
+
+    GRID_ARRAYS = ["xc", "yc", "zc", "xu", "yu", "zu", "xv", "yv", "zv", "xw", "yw", "zw",
+                   "dx_c", "dy_c", "dz_c", "mxc", "myc", "mzc", "mxu", "myv", "mzw"]
+
+    def grid(self, name):
+        """One 1-D grid array (coordinates, spacings, metrics) of the context."""
+        out = np.zeros(max(self.shape) + 1)
+        n = self._ok(self.L.tb_grid_array(self.h, self.GRID_ARRAYS.index(name), _ptr(out), out.size), "tb_grid_array")
+        return out[:n].copy()
 
     def upload(self, name, a):
         a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
@@ -331,6 +346,13 @@ class Solver:
         st = abi.StepStats()
         self._ok(self.L.tb_step(self.h, dt, C.byref(st)), "tb_step")
         return st
+
+    def cfl_dt(self, alpha, dt_max):
+        """(dt, rate): dt = min(dt_max, alpha / max over FLUID cells of
+        |u_c|/dx + |v_c|/dy + |w_c|/dz)."""
+        dt, r = C.c_double(), C.c_double()
+        self._ok(self.L.tb_cfl_dt(self.h, alpha, dt_max, C.byref(dt), C.byref(r)), "tb_cfl_dt")
+        return dt.value, r.value
 
     def set_poisson_mode(self, mode, rtol):
         return self._ok(self.L.tb_set_poisson_mode(self.h, mode, rtol), "tb_set_poisson_mode")

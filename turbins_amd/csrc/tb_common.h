@@ -75,6 +75,19 @@ __device__ inline bool tile_point(const Tiles &t, int &i, int &j, int &k) {
     return i < t.nx && j < t.ny;
 }
 
+// The same for virtual block vb of a loop over the tiles (vb < per_xcd * 8).
+__device__ inline bool tile_point_at(const Tiles &t, long vb, int &i, int &j, int &k) {
+    long lt = (vb % NXCD) * t.per_xcd + vb / NXCD;
+    if (lt >= t.n) return false;
+    int tx = (int)(lt % t.tx);
+    long r = lt / t.tx;
+    int ty = (int)(r % t.ty);
+    k = (int)(r / t.ty);
+    i = tx * TILE_X + threadIdx.x;
+    j = ty * TILE_Y + threadIdx.y;
+    return i < t.nx && j < t.ny;
+}
+
 // Same tiles, numbered with k fastest: the blocks an XCD runs back to back
 // cover one 64 x 4 column plane after plane, so a stencil reaching several
 // planes in z finds them in that XCD's L2 (the x-fastest order above revisits

@@ -177,6 +177,21 @@ extern "C" int tb_device_bytes(tb_ctx *c, int64_t *bytes) {
 
 void tb_free_ib(tb_ctx *c, int q) { free_ib(c, q); }
 
+// The host copy of one 1-D grid array (the order of tb_grid_array's which).
+extern "C" int tb_grid_array(tb_ctx *c, int which, double *out, int capacity) {
+    if (!c || which < 0 || which >= 21 || !out) {
+        if (c) c->err = "tb_grid_array: bad argument";
+        return -1;
+    }
+    const std::vector<double> &a = c->hgrid[which];
+    if ((size_t)capacity < a.size()) {
+        c->err = "tb_grid_array: capacity too small";
+        return -1;
+    }
+    memcpy(out, a.data(), sizeof(double) * a.size());
+    return (int)a.size();
+}
+
 // ---------------------------------------------------------------------------
 extern "C" int tb_upload_field(tb_ctx *c, int f, const double *host) {
     if (!c || f < 0 || f >= TB_F_COUNT || !host) { if (c) c->err = "tb_upload_field: bad argument"; return -1; }

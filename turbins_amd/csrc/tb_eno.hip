@@ -13,8 +13,8 @@
 // registers.  All arithmetic keeps the reference's order (no contraction),
 // so results are bit-identical to the line-by-line evaluation.
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
-
 #include <cstring>
 
 #include "tb_common.h"
@@ -712,5 +712,82 @@ extern "C" int tb_sedimentation_rate(tb_ctx *c, double *total, double *bottom) {
     for (long a = 0; a < n; a++) s += h[a];
     if (total) *total = s;
     if (bottom) memcpy(bottom, h.data(), sizeof(double) * n);
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// CFL time step.  The reference keeps its parameters (cfl_method, alpha = 0.5,
+// default_dt, Input.c:421-423) but its rule lived in the unpublished driver;
+// the rule here: over the FLUID cells of the concentration grid, with the
+// cell-centre velocities of Velocity_cell_center (Velocity.c:1476-1602),
+// rate = |u_c|/dx + |v_c|/dy + |w_c|/dz, and dt = min(dt_max, alpha / max rate).
+// Per-workgroup maxima, then one workgroup takes the maximum of those (a
+// maximum does not depend on the order, so the result is exact); a NaN
+// anywhere wins every comparison and is reported.
+__device__ inline double max_nan(double a, double b) { return (a != a || a > b) ? a : b; }
+
+__global__ void __launch_bounds__(256) k_cfl(Dims d, GridDev g, const double *__restrict__ u,
+                                             const double *__restrict__ v, const double *__restrict__ w,
+                                             const uint8_t *__restrict__ sc, double *part, Tiles t) {
+    __shared__ double sh[256];
+    const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+    double r = 0.0;
+    int i, j, k;
+    if (tile_point(t, i, j, k)) {
+        const long id = gidx(d, i, j, k);
+        if (sc[id] == TB_FLUID) {
+            const double ubc = interp2d(u[id], g.xu[i], u[id + 1], g.xu[i + 1], g.xc[i]);
+            const double vbc = interp2d(v[id], g.yv[j], v[id + d.px], g.yv[j + 1], g.yc[j]);
+            const double wbc = interp2d(w[id], g.zw[k], w[id + d.plane], g.zw[k + 1], g.zc[k]);
+            r = fabs(ubc) / g.dxc[i] + fabs(vbc) / g.dyc[j] + fabs(wbc) / g.dzc[k];
+        }
+    }
+    sh[tid] = r;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (tid < s) sh[tid] = max_nan(sh[tid], sh[tid + s]);
+        __syncthreads();
+    }
+    if (tid == 0) part[blockIdx.x] = sh[0];
+}
+
+__global__ void k_max_final(const double *part, long n, double *out) {
+    __shared__ double sh[1024];
+    double m = 0.0;
+    for (long b = threadIdx.x; b < n; b += blockDim.x) m = max_nan(m, part[b]);
+    sh[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = 512; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] = max_nan(sh[threadIdx.x], sh[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = sh[0];
+}
+
+extern "C" int tb_cfl_dt(tb_ctx *c, double alpha, double dt_max, double *dt, double *rate) {
+    if (!c || !dt || !(alpha > 0.0) || !(dt_max > 0.0)) {
+        if (c) c->err = "tb_cfl_dt: bad argument";
+        return -1;
+    }
+    const Tiles t = make_tiles(c->ncx, c->ncy, c->ncz);
+    const dim3 grid = tiles_grid(t);
+    if ((long)grid.x > RED_MAX_BLOCKS) {
+        c->err = "tb_cfl_dt: grid too large for the reduction buffer";
+        return -1;
+    }
+    hipLaunchKernelGGL(k_cfl, grid, tiles_block(), 0, c->s, c->d, c->g, (const double *)c->f[TB_F_U],
+                       (const double *)c->f[TB_F_V], (const double *)c->f[TB_F_W], (const uint8_t *)c->st[3], c->red,
+                       t);
+    hipLaunchKernelGGL(k_max_final, dim3(1), dim3(1024), 0, c->s, (const double *)c->red, (long)grid.x,
+                       c->scal + S_TMP0);
+    TB_LAUNCH_CHECK(c);
+    if (tb_read_scalars(c, S_TMP0, 1)) return -1;
+    const double m = c->hscal[S_TMP0];
+    if (!(m <= DBL_MAX)) {
+        c->err = "tb_cfl_dt: NaN or Inf velocity";
+        return -1;
+    }
+    if (rate) *rate = m;
+    *dt = (m > 0.0 && alpha / m < dt_max) ? alpha / m : dt_max;
     return 0;
 }

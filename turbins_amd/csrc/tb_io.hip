@@ -126,7 +126,8 @@ extern "C" int tb_read_resume_primary(const char *dir, tb_resume_info *info) {
     for (int k = 0; k < 5; k++)
         if (strcmp(label[k], want[k]) != 0) return TB_IO_NOT_A_VEC;
     *info = r;
-    return 0;
+    // the record's six decimals turn a step below 5e-7 into 0: say so
+    return (r.dt == 0.0 || r.dt_old == 0.0) ? TB_IO_ZERO_STEP : 0;
 }
 
 static const char *io_message(int64_t rc) {
@@ -200,5 +201,6 @@ extern "C" int tb_read_resume(tb_ctx *c, const char *dir, tb_resume_info *info) 
     // a restart begins a new trajectory: the learned guess forgets the
     // solutions of the old one
     c->n_hist = c->gram_n = c->guess_nr = 0;
-    return 0;
+    c->phi_is_hist0 = false;
+    return rc;   // TB_IO_ZERO_STEP passes through as a warning
 }

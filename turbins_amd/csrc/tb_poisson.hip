@@ -2239,6 +2239,8 @@ extern "C" int tb_project(tb_ctx *c, double dt) {
 // pass is the NaN / Inf scan of Debugger_check_nan (Debugger.c:286-324) over
 // the non-BOUNDARY cells: a cell whose divergence or concentration is not
 // finite ranks as +infinity, so the maximum reports it with its index.
+// Each workgroup walks the tiles blockIdx.x, blockIdx.x + gridDim.x, ...
+// (one tile when the grid covers them all); div may be null.
 __global__ void __launch_bounds__(256) k_div(Dims d, GridDev g, const double *__restrict__ u,
                                              const double *__restrict__ v, const double *__restrict__ w,
                                              const double *__restrict__ cc, const uint8_t *__restrict__ sc,
@@ -2249,8 +2251,10 @@ __global__ void __launch_bounds__(256) k_div(Dims d, GridDev g, const double *__
     int tid = threadIdx.y * blockDim.x + threadIdx.x;
     double best = -1.0;
     long bidx = -1;
-    int i, j, k;
-    if (tile_point(t, i, j, k)) {
+    const long nvb = t.per_xcd * NXCD;
+    for (long vb = blockIdx.x; vb < nvb; vb += gridDim.x) {
+        int i, j, k;
+        if (!tile_point_at(t, vb, i, j, k)) continue;
         long id = gidx(d, i, j, k);
         double dc = 0.0;
         bool finite = true;
@@ -2262,9 +2266,10 @@ __global__ void __launch_bounds__(256) k_div(Dims d, GridDev g, const double *__
             // false for NaN and +-Inf
             finite = dc <= DBL_MAX && fabs(cc[id]) <= DBL_MAX;
         }
-        div[id] = dc;
-        best = finite ? dc : INFINITY;
-        bidx = ((long)k * d.NY + j) * d.NX + i;
+        if (div) div[id] = dc;
+        const double key = finite ? dc : INFINITY;
+        const long idx = ((long)k * d.NY + j) * d.NX + i;
+        if (key > best || (key == best && idx > bidx)) { best = key; bidx = idx; }
     }
     sv[tid] = best;
     si[tid] = bidx;
