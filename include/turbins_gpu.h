@@ -323,6 +323,36 @@ int tb_project(tb_ctx *ctx, double dt);
  * (Debugger_check_nan, Debugger.c:286-324) and returns -1 with the cell in
  * ijk and in tb_last_error when it finds one (tb_step then fails too) */
 int tb_div_max(tb_ctx *ctx, double *div_max, int32_t *ijk);
+/* Allocation-free divergence check.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory -- code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc.  The same
+ * kernels as tb_div_max on caller-owned device buffers, no context: |div u|
+ * per cell into div (device, may be NULL), the maximum into *out_max and its
+ * cell index (k*(ny+1) + j)*(nx+1) + i into *out_index (both device
+ * pointers), enqueued on `stream` (a hipStream_t; NULL: the default stream)
+ * and not waited for.  A NaN / Inf in u, v, w or c on a non-BOUNDARY cell
+ * makes *out_max +infinity with that cell's index.  Fields are FP64 in the
+ * layout of the library's fields with row pitch box->pitch >= nx+1 elements
+ * (x fastest, then y, z); flags_c are the concentration grid's status bytes
+ * in the same layout; mxc, myc, mzc are the metric arrays (Grid.c:595-708)
+ * of nx+1, ny+1, nz+1 doubles on the device.  Workspace: a 32 KiB device
+ * array inside the library (one per GPU, held until the process ends); calls
+ * that overlap on one GPU race for it.  Boxes of more than 2^31 points are
+ * rejected.  Calls no hipMalloc / hipFree.  Returns 0 or a TB_NOALLOC_
+ * code. */
+typedef struct tb_box {
+    int32_t nx, ny, nz;         /* physical cells */
+    int32_t pitch;              /* row pitch of every field, in elements */
+    double deta, dxi, dphi;     /* computational spacings 1/nx, 1/ny, 1/nz (grid->dEta ...) */
+} tb_box;
+#define TB_NOALLOC_BAD_ARGUMENT (-1)
+#define TB_NOALLOC_TOO_LARGE (-2)     /* more than 2^31 points */
+#define TB_NOALLOC_NO_SCRATCH (-3)    /* no current device, or its workspace address unavailable */
+#define TB_NOALLOC_LAUNCH (-4)        /* kernel launch failed */
+int tb_div_max_noalloc(const tb_box *box, const double *mxc, const double *myc, const double *mzc,
+                       const double *u, const double *v, const double *w, const double *c,
+                       const uint8_t *flags_c, double *div, double *out_max, int64_t *out_index,
+                       void *stream);
 /* Velocity_store_old_data + Conc_store_old_data (Velocity.c:297-302,
  * Velocity.h:6; Conc.c:997-1003, Conc.h:31) for u, v, w, c */
 int tb_store_old(tb_ctx *ctx);

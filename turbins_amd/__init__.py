@@ -86,6 +86,7 @@ def lib():
     L.tb_read_resume_primary.argtypes = [C.c_char_p, C.POINTER(abi.ResumeInfo)]
     L.tb_write_resume.argtypes = [P, C.c_char_p, C.POINTER(abi.ResumeInfo)]
     L.tb_read_resume.argtypes = [P, C.c_char_p, C.POINTER(abi.ResumeInfo)]
+    L.tb_div_max_noalloc.argtypes = [C.POINTER(abi.Box)] + [P] * 11 + [P]
     L.tb_profile_enable.argtypes = [P, C.c_int]
     L.tb_profile_read.argtypes = [P, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
     L.tb_net_create.restype = P
@@ -171,6 +172,15 @@ def read_resume_primary(directory):
 class TBError(RuntimeError):
     pass
 
+
+def div_max_noalloc(box, mxc, myc, mzc, u, v, w, c, flags_c, div, out_max, out_index, stream=None):
+    """tb_div_max_noalloc on device addresses (ints): enqueues the
+    divergence check on `stream` (a hipStream_t address, None: default)."""
+    rc = lib().tb_div_max_noalloc(C.byref(box), mxc, myc, mzc, u, v, w, c, flags_c, div, out_max, out_index,
+                                  stream)
+    if rc < 0:
+        raise TBError(f"tb_div_max_noalloc: {abi.NOALLOC_ERRORS.get(int(rc), rc)}")
+    return rc
 
 
 class Solver:

@@ -56,6 +56,18 @@ class IBNode(C.Structure):
     ]
 
 
+class Box(C.Structure):
+    """tb_box: a field layout for the context-free entry points."""
+    _fields_ = [
+        ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("pitch", C.c_int32),
+        ("deta", C.c_double), ("dxi", C.c_double), ("dphi", C.c_double),
+    ]
+
+
+NOALLOC_ERRORS = {-1: "bad argument", -2: "box larger than 2^31 points", -3: "no device workspace",
+                  -4: "kernel launch failed"}
+
+
 class SolveStats(C.Structure):
     _fields_ = [
         ("iterations", C.c_int32), ("converged", C.c_int32),

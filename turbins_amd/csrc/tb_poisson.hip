@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "tb_common.h"
@@ -2342,5 +2343,74 @@ extern "C" int tb_div_max(tb_ctx *c, double *div_max, int32_t *ijk) {
         c->err = buf;
         return -1;
     }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// tb_div_max_noalloc: the same kernels on caller-owned device buffers, with
+// the per-workgroup partials in a file-scope device array instead of memory
+// the library allocates.  A fixed grid of NOALLOC_BLOCKS workgroups walks the
+// tiles, so the partials take 2 * NOALLOC_BLOCKS slots whatever the box.
+// Each GPU holds its own copy of the array until the process ends; every
+// call on a GPU overwrites the slots its workgroups own before the final
+// kernel reads them (nothing needs clearing), and calls that overlap on one
+// GPU race for them.
+constexpr int NOALLOC_BLOCKS = 2048;
+constexpr long NOALLOC_MAX_POINTS = 1L << 31;
+static __device__ __align__(256) double scratch[2 * NOALLOC_BLOCKS];   // maxima, then indices (bits of a long)
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex noalloc_mu;
+static double *noalloc_ptr[NOALLOC_MAX_DEVICES];   // scratch's address on each device
+
+// address of scratch on the current device, looked up on its first call
+static int noalloc_scratch(double **p) {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= NOALLOC_MAX_DEVICES) return -1;
+    std::lock_guard<std::mutex> lock(noalloc_mu);
+    if (!noalloc_ptr[dev]) {
+        void *q = nullptr;
+        if (hipGetSymbolAddress(&q, HIP_SYMBOL(scratch)) != hipSuccess || !q) return -1;
+        noalloc_ptr[dev] = static_cast<double *>(q);
+    }
+    *p = noalloc_ptr[dev];
+    return 0;
+}
+
+extern "C" int tb_div_max_noalloc(const tb_box *box, const double *mxc, const double *myc, const double *mzc,
+                                  const double *u, const double *v, const double *w, const double *c,
+                                  const uint8_t *flags_c, double *div, double *out_max, int64_t *out_index,
+                                  void *stream) {
+    if (!box || !mxc || !myc || !mzc || !u || !v || !w || !c || !flags_c || !out_max || !out_index)
+        return TB_NOALLOC_BAD_ARGUMENT;
+    if (box->nx < 1 || box->ny < 1 || box->nz < 1 || box->pitch < box->nx + 1 || !(box->deta > 0.0) ||
+        !(box->dxi > 0.0) || !(box->dphi > 0.0))
+        return TB_NOALLOC_BAD_ARGUMENT;
+    Dims d;
+    d.NX = box->nx + 1;
+    d.NY = box->ny + 1;
+    d.NZ = box->nz + 1;
+    d.px = box->pitch;
+    d.plane = (long)d.px * d.NY;
+    d.ntot = d.plane * d.NZ;
+    if (d.ntot > NOALLOC_MAX_POINTS) return TB_NOALLOC_TOO_LARGE;
+    double *scr = nullptr;
+    if (noalloc_scratch(&scr)) return TB_NOALLOC_NO_SCRATCH;
+    GridDev g;
+    memset(&g, 0, sizeof(g));
+    g.mxc = mxc;
+    g.myc = myc;
+    g.mzc = mzc;
+    const Tiles t = make_tiles(d.NX, d.NY, d.NZ);
+    const long nvb = t.per_xcd * NXCD;
+    const int nb = nvb < NOALLOC_BLOCKS ? (int)nvb : NOALLOC_BLOCKS;
+    double *pm = scr;
+    long *pi = reinterpret_cast<long *>(scr + NOALLOC_BLOCKS);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(k_div, dim3(nb), tiles_block(), 0, st, d, g, u, v, w, c, flags_c, div, pm, pi,
+                       1.0 / box->deta, 1.0 / box->dxi, 1.0 / box->dphi, t);
+    hipLaunchKernelGGL(k_div_final, dim3(1), dim3(1024), 0, st, (const double *)pm, (const long *)pi, (long)nb,
+                       out_max, reinterpret_cast<long *>(out_index));
+    if (hipGetLastError() != hipSuccess) return TB_NOALLOC_LAUNCH;
     return 0;
 }
