@@ -233,9 +233,7 @@ struct MomArgs {
 __device__ inline bool vel_row(int s) { return s == TB_FLUID || s == TB_BUFFER_SOLID; }
 
 template <int R, bool RHS>
-__global__ void __launch_bounds__(256) k_eno_momentum(MomArgs a, Tiles t) {
-    int i, j, k;
-    if (!tile_point_kfast(t, i, j, k)) return;
+__device__ inline void mom_point(const MomArgs &a, int i, int j, int k) {
     const Dims &d = a.d;
     const GridDev &g = a.g;
     long id = gidx(d, i, j, k);
@@ -299,6 +297,13 @@ __global__ void __launch_bounds__(256) k_eno_momentum(MomArgs a, Tiles t) {
     }
 }
 
+template <int R, bool RHS>
+__global__ void __launch_bounds__(256) k_eno_momentum(MomArgs a, Tiles t) {
+    int i, j, k;
+    if (!tile_point_kfast(t, i, j, k)) return;
+    mom_point<R, RHS>(a, i, j, k);
+}
+
 struct ConcArgs {
     Dims d;
     GridDev g;
@@ -314,9 +319,7 @@ struct ConcArgs {
 };
 
 template <int R, bool RHS>
-__global__ void __launch_bounds__(256) k_eno_conc(ConcArgs a, Tiles t) {
-    int i, j, k;
-    if (!tile_point_kfast(t, i, j, k)) return;
+__device__ inline void conc_point(const ConcArgs &a, int i, int j, int k) {
     const Dims &d = a.d;
     const GridDev &g = a.g;
     long id = gidx(d, i, j, k);
@@ -334,6 +337,13 @@ __global__ void __launch_bounds__(256) k_eno_conc(ConcArgs a, Tiles t) {
     }
     if (!RHS || a.store_conv) st_stream(&a.cc[id], r);
     if constexpr (RHS) st_stream(&a.rc[id], s == TB_FLUID ? a.c[id] * a.a_dt - r : 0.0);
+}
+
+template <int R, bool RHS>
+__global__ void __launch_bounds__(256) k_eno_conc(ConcArgs a, Tiles t) {
+    int i, j, k;
+    if (!tile_point_kfast(t, i, j, k)) return;
+    conc_point<R, RHS>(a, i, j, k);
 }
 
 // True when every first gap fl(p[m+1] - p[m]) of [lo, hi] equals one power
@@ -437,10 +447,277 @@ static int eno_weights(tb_ctx *c) {
     return 0;
 }
 
+// ---------------------------------------------------------------------------
+// Second-order ENO of u, v, w and c in one k-marching pass (the workloads'
+// order) on power-of-two uniform grids.  A block owns a 64 x 4 tile of (i, j)
+// columns and walks EF_KC planes of it.  The planes k-1, k, k+1 of u, v, w, c
+// sit in an LDS ring with a two-point x / y halo, so the x and y windows, the
+// z neighbours k +- 1 and the transposes read LDS; each thread keeps k - 2 and
+// k + 2 of its own column in registers.  It computes the nodes at least
+// EF_SHELL from every box face, whose windows all lie inside the uniform
+// interior (the exact-reciprocal form of eno_uniform); k_eno_shell computes
+// the rest with the per-point code (wall ghosts, general form).  Both do the
+// operations of eno_line_point in the same order, so every value is
+// bit-identical to the per-point kernels.
+constexpr int EF_TX = 64, EF_TY = 4, EF_H = 2, EF_KC = 16;
+constexpr int EF_SX = EF_TX + 2 * EF_H, EF_SY = EF_TY + 2 * EF_H;
+constexpr int EF_PLANE = EF_SX * EF_SY;
+constexpr int EF_THREADS = EF_TX * EF_TY;
+constexpr int EF_STAGE = (EF_PLANE + EF_THREADS - 1) / EF_THREADS;
+
+struct EfTiles {
+    int tx, ty, kc;     // tiles per axis from i, j = 0, k chunks from k = EF_SHELL
+    long n, per_xcd;
+};
+
+static EfTiles make_ef_tiles(const Dims &d, int shell) {
+    EfTiles t;
+    t.tx = (d.NX - shell + EF_TX - 1) / EF_TX;
+    t.ty = (d.NY - shell + EF_TY - 1) / EF_TY;
+    t.kc = (d.NZ - 2 * shell + EF_KC - 1) / EF_KC;
+    t.n = (long)t.tx * t.ty * t.kc;
+    t.per_xcd = (t.n + NXCD - 1) / NXCD;
+    return t;
+}
+
+// eno_uniform<2> on five values already at hand (w[2] is the node)
+__device__ inline double eno_u5(double w0, double w1, double w2, double w3, double w4, double rh, double h,
+                                double vel) {
+    const double D1_0 = (w1 - w0) * rh, D1_1 = (w2 - w1) * rh, D1_2 = (w3 - w2) * rh, D1_3 = (w4 - w3) * rh;
+    const bool left = vel > 0.0;
+    double dQ = left ? D1_1 : D1_2;
+    const double r2h = 0.5 * rh;
+    const double D2_0 = (D1_1 - D1_0) * r2h, D2_1 = (D1_2 - D1_1) * r2h, D2_2 = (D1_3 - D1_2) * r2h;
+    const double d2k = left ? D2_0 : D2_1;
+    const double d2k1 = left ? D2_1 : D2_2;
+    const double c = (fabs(d2k) <= fabs(d2k1)) ? d2k : d2k1;
+    dQ += c * (left ? h : -h);
+    return vel * dQ;
+}
+
+// Everything k_eno_fused2 reads, flat.
+struct FusedArgs {
+    Dims d;
+    const double *u, *v, *w, *c;
+    const uint8_t *su, *sv, *sw, *sc;
+    const double *dpx, *dpy, *dpz;
+    double *cu, *cv, *cw, *cc, *ru, *rv, *rw, *rc;
+    const double *W;          // transpose weight tables (W_COUNT x wm)
+    int wm;
+    const double *xu, *xc, *yv, *yc, *zw, *zc;
+    double a_dt, buoy, v_settle;
+    int particle, store_conv;
+    // uniform fast path: bit 3*q + ax set when axis ax of quantity q is a
+    // power-of-two uniform line of spacing h[ax]; hi[ax] the last interior
+    // index (the first is 1 on the quantity's own axis, else 0)
+    int unimask;
+    int hi[3];
+    double h[3], rh[3];
+};
+
+// the fused kernel's nodes: at least EF_SHELL from every box face, where
+// every window of every quantity lies inside its line's interior span
+constexpr int EF_SHELL = 3;
+
+template <bool RHS>
+__global__ void __launch_bounds__(EF_THREADS) k_eno_fused2(FusedArgs A, EfTiles T) {
+    // [ring slot][field u, v, w, c][plane with halo]
+    __shared__ double S[3][4][EF_PLANE];
+    const Dims &d = A.d;
+    const long b = blockIdx.x;
+    const long lt = (b % NXCD) * T.per_xcd + b / NXCD;
+    if (lt >= T.n) return;                     // whole block: no barrier below is shared
+    const int tyi = (int)(lt % T.ty);
+    const long r1 = lt / T.ty;
+    const int txi = (int)(r1 % T.tx);
+    const int kci = (int)(r1 / T.tx);
+    const int i0 = txi * EF_TX, j0 = tyi * EF_TY;
+    const int k0 = EF_SHELL + kci * EF_KC;
+    const int k1 = (k0 + EF_KC < d.NZ - EF_SHELL) ? k0 + EF_KC : d.NZ - EF_SHELL;
+    const int tx = threadIdx.x, ty = threadIdx.y, t = ty * EF_TX + tx;
+    const int i = i0 + tx, j = j0 + ty;
+    const bool act = i >= EF_SHELL && i < d.NX - EF_SHELL && j >= EF_SHELL && j < d.NY - EF_SHELL;
+
+    // staging offsets of this thread's halo points (clamped into the box;
+    // a clamped point is never read by an active node)
+    int soff[EF_STAGE];
+#pragma unroll
+    for (int e = 0; e < EF_STAGE; e++) {
+        const int p = t + e * EF_THREADS;
+        const int r = p / EF_SX, cc = p - r * EF_SX;
+        int ii = i0 - EF_H + cc, jj = j0 - EF_H + r;
+        ii = ii < 0 ? 0 : (ii > d.NX - 1 ? d.NX - 1 : ii);
+        jj = jj < 0 ? 0 : (jj > d.NY - 1 ? d.NY - 1 : jj);
+        soff[e] = p < EF_PLANE ? jj * d.px + ii : -1;
+    }
+    auto stage = [&](int m) {
+        const long base = (long)m * d.plane;
+        double *P = &S[(m + 3) % 3][0][0];
+#pragma unroll
+        for (int e = 0; e < EF_STAGE; e++) {
+            const int p = t + e * EF_THREADS;
+            if (soff[e] >= 0) {
+                const long q = base + soff[e];
+                P[p] = A.u[q];
+                P[EF_PLANE + p] = A.v[q];
+                P[2 * EF_PLANE + p] = A.w[q];
+                P[3 * EF_PLANE + p] = A.c[q];
+            }
+        }
+    };
+    // z windows of this thread's column: k-2 and k+2 in registers, k-1 .. k+1
+    // from the staged planes
+    const int ic = i < d.NX - 1 ? i : d.NX - 2, jc = j < d.NY - 1 ? j : d.NY - 2;
+    const long col = (long)jc * d.px + ic;
+    double zm2[4], zp2[4];
+    {
+        const long a2 = (long)(k0 - 2) * d.plane + col, p2 = a2 + 4 * d.plane;
+        zm2[0] = A.u[a2]; zm2[1] = A.v[a2]; zm2[2] = A.w[a2]; zm2[3] = A.c[a2];
+        zp2[0] = A.u[p2]; zp2[1] = A.v[p2]; zp2[2] = A.w[p2]; zp2[3] = A.c[p2];
+    }
+    stage(k0 - 1);
+    stage(k0);
+    stage(k0 + 1);
+    const int o = (ty + EF_H) * EF_SX + tx + EF_H;   // this thread's cell in a staged plane
+    const double *W = A.W;
+    const int wm = A.wm;
+    const double hx = A.h[0], hy = A.h[1], hz = A.h[2], rx = A.rh[0], ry = A.rh[1], rz = A.rh[2];
+    for (int k = k0; k < k1; k++) {
+        __syncthreads();
+        if (act) {
+            const double *Ubk = &S[(k + 2) % 3][0][0], *Vbk = &S[(k + 2) % 3][1][0];
+            const double *Uk = &S[k % 3][0][0], *Vk = &S[k % 3][1][0], *Wk = &S[k % 3][2][0],
+                         *Ck = &S[k % 3][3][0];
+            const double *Uf = &S[(k + 1) % 3][0][0], *Vf = &S[(k + 1) % 3][1][0], *Wf = &S[(k + 1) % 3][2][0],
+                         *Cf = &S[(k + 1) % 3][3][0];
+            const double *Wbk = &S[(k + 2) % 3][2][0], *Cbk = &S[(k + 2) % 3][3][0];
+            const long id = gidx(d, i, j, k);
+            const double uc = Uk[o], vc = Vk[o], wc = Wk[o], cc = Ck[o];
+            // u node
+            {
+                double r = 0.0;
+                const int s = A.su[id];
+                if (s != TB_BOUNDARY) {
+                    const double vu = interp4w(Vk[o - 1], vc, Vk[o + EF_SX - 1], Vk[o + EF_SX], W[W_VU_X * wm + i],
+                                               W[W_VU_Y * wm + j]);
+                    const double wu = interp4w(Wk[o - 1], wc, Wf[o - 1], Wf[o], W[W_WU_X * wm + i],
+                                               W[W_WU_Z * wm + k]);
+                    r = eno_u5(Uk[o - 2], Uk[o - 1], uc, Uk[o + 1], Uk[o + 2], rx, hx, uc);
+                    r += eno_u5(Uk[o - 2 * EF_SX], Uk[o - EF_SX], uc, Uk[o + EF_SX], Uk[o + 2 * EF_SX], ry, hy, vu);
+                    r += eno_u5(zm2[0], Ubk[o], uc, Uf[o], zp2[0], rz, hz, wu);
+                }
+                if (!RHS || A.store_conv) st_stream(&A.cu[id], r);
+                if constexpr (RHS) st_stream(&A.ru[id], vel_row(s) ? uc * A.a_dt - r - A.dpx[id] : 0.0);
+            }
+            // v node
+            {
+                double r = 0.0;
+                const int s = A.sv[id];
+                if (s != TB_BOUNDARY) {
+                    const double uv = interp4w(Uk[o - EF_SX], Uk[o - EF_SX + 1], uc, Uk[o + 1], W[W_UV_X * wm + i],
+                                               W[W_UV_Y * wm + j]);
+                    const double wv = interp4w(Wk[o - EF_SX], Wf[o - EF_SX], wc, Wf[o], W[W_WV_Z * wm + k],
+                                               W[W_WV_Y * wm + j]);
+                    r = eno_u5(Vk[o - 2], Vk[o - 1], vc, Vk[o + 1], Vk[o + 2], rx, hx, uv);
+                    r += eno_u5(Vk[o - 2 * EF_SX], Vk[o - EF_SX], vc, Vk[o + EF_SX], Vk[o + 2 * EF_SX], ry, hy, vc);
+                    r += eno_u5(zm2[1], Vbk[o], vc, Vf[o], zp2[1], rz, hz, wv);
+                }
+                if (!RHS || A.store_conv) st_stream(&A.cv[id], r);
+                if constexpr (RHS) {
+                    double rh = 0.0;
+                    if (vel_row(s)) {
+                        const double c_term = interp2d(Ck[o - EF_SX], A.yc[j - 1], cc, A.yc[j], A.yv[j]);
+                        rh = vc * A.a_dt - r - A.dpy[id] - A.buoy * c_term;
+                    }
+                    st_stream(&A.rv[id], rh);
+                }
+            }
+            // w node
+            {
+                double r = 0.0;
+                const int s = A.sw[id];
+                if (s != TB_BOUNDARY) {
+                    const double uw = interp4w(Ubk[o], Ubk[o + 1], uc, Uk[o + 1], W[W_UW_X * wm + i],
+                                               W[W_UW_Z * wm + k]);
+                    const double vw = interp4w(Vbk[o], vc, Vbk[o + EF_SX], Vk[o + EF_SX], W[W_VW_Z * wm + k],
+                                               W[W_VW_Y * wm + j]);
+                    r = eno_u5(Wk[o - 2], Wk[o - 1], wc, Wk[o + 1], Wk[o + 2], rx, hx, uw);
+                    r += eno_u5(Wk[o - 2 * EF_SX], Wk[o - EF_SX], wc, Wk[o + EF_SX], Wk[o + 2 * EF_SX], ry, hy, vw);
+                    r += eno_u5(zm2[2], Wbk[o], wc, Wf[o], zp2[2], rz, hz, wc);
+                }
+                if (!RHS || A.store_conv) st_stream(&A.cw[id], r);
+                if constexpr (RHS) st_stream(&A.rw[id], vel_row(s) ? wc * A.a_dt - r - A.dpz[id] : 0.0);
+            }
+            // concentration cell
+            {
+                double r = 0.0;
+                const int s = A.sc[id];
+                if (s != TB_BOUNDARY) {
+                    // Velocity_cell_center (Velocity.c:1476-1602) at this cell
+                    const double ubc = interp2d(uc, A.xu[i], Uk[o + 1], A.xu[i + 1], A.xc[i]);
+                    double vbc = interp2d(vc, A.yv[j], Vk[o + EF_SX], A.yv[j + 1], A.yc[j]);
+                    const double wbc = interp2d(wc, A.zw[k], Wf[o], A.zw[k + 1], A.zc[k]);
+                    if (A.particle) vbc = vbc + A.v_settle;   // Conc_set_particle_v_velocity (Conc.c:734-761)
+                    r = eno_u5(Ck[o - 2], Ck[o - 1], cc, Ck[o + 1], Ck[o + 2], rx, hx, ubc);
+                    r += eno_u5(Ck[o - 2 * EF_SX], Ck[o - EF_SX], cc, Ck[o + EF_SX], Ck[o + 2 * EF_SX], ry, hy, vbc);
+                    r += eno_u5(zm2[3], Cbk[o], cc, Cf[o], zp2[3], rz, hz, wbc);
+                }
+                if (!RHS || A.store_conv) st_stream(&A.cc[id], r);
+                if constexpr (RHS) st_stream(&A.rc[id], s == TB_FLUID ? cc * A.a_dt - r : 0.0);
+            }
+            // the column's z window moves up: k-1 becomes k-2, k+3 arrives
+            zm2[0] = Ubk[o]; zm2[1] = Vbk[o]; zm2[2] = Wbk[o]; zm2[3] = Cbk[o];
+        }
+        __syncthreads();
+        // the slot of plane k - 1 is free: plane k + 2 goes there
+        if (k + 2 < d.NZ) stage(k + 2);
+        if (act && k + 1 < k1) {
+            const long p3 = (long)(k + 3) * d.plane + col;
+            zp2[0] = A.u[p3]; zp2[1] = A.v[p3]; zp2[2] = A.w[p3]; zp2[3] = A.c[p3];
+        }
+    }
+}
+
+// the nodes within EF_SHELL of a box face (the fused kernel leaves them):
+// each with the per-point code, over the x slabs, then the y slabs between
+// them, then the z slabs between those
+template <bool RHS>
+__global__ void __launch_bounds__(256) k_eno_shell(MomArgs a, ConcArgs ca, long nx_slab, long ny_slab, long nz_slab) {
+    const Dims &d = a.d;
+    long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    const int H = EF_SHELL;
+    int i, j, k;
+    if (e < nx_slab) {
+        const int side = (int)(e % (2 * H));
+        i = side < H ? side : d.NX - 2 * H + side;
+        const long r = e / (2 * H);
+        j = (int)(r % d.NY);
+        k = (int)(r / d.NY);
+    } else if ((e -= nx_slab) < ny_slab) {
+        const int w = d.NX - 2 * H;
+        i = H + (int)(e % w);
+        const long r = e / w;
+        const int side = (int)(r % (2 * H));
+        j = side < H ? side : d.NY - 2 * H + side;
+        k = (int)(r / (2 * H));
+    } else if ((e -= ny_slab) < nz_slab) {
+        const int w = d.NX - 2 * H, h = d.NY - 2 * H;
+        i = H + (int)(e % w);
+        const long r = e / w;
+        j = H + (int)(r % h);
+        const int side = (int)(r / h);
+        k = side < H ? side : d.NZ - 2 * H + side;
+    } else {
+        return;
+    }
+    mom_point<2, RHS>(a, i, j, k);
+    conc_point<2, RHS>(ca, i, j, k);
+}
+
+
 // rhs_dt > 0: the four right-hand sides of step dt in the same pass, the
 // convective terms stored only if store_conv
-static int eno_momentum(tb_ctx *c, double rhs_dt, bool store_conv) {
-    if (eno_weights(c)) return -1;
+static MomArgs mom_args(tb_ctx *c, double rhs_dt, bool store_conv) {
     MomArgs a;
     a.d = c->d; a.g = c->g;
     a.W = c->eno_w;
@@ -454,6 +731,12 @@ static int eno_momentum(tb_ctx *c, double rhs_dt, bool store_conv) {
     a.a_dt = rhs_dt > 0.0 ? 1.0 / rhs_dt : 0.0;
     a.buoy = c->cfg.buoyancy;
     a.store_conv = store_conv ? 1 : 0;
+    return a;
+}
+
+static int eno_momentum(tb_ctx *c, double rhs_dt, bool store_conv) {
+    if (eno_weights(c)) return -1;
+    const MomArgs a = mom_args(c, rhs_dt, store_conv);
     Tiles t = make_tiles(c->d.NX, c->d.NY, c->d.NZ);
     const dim3 grid = tiles_grid(t), blk = tiles_block();
     const bool rhs = rhs_dt > 0.0;
@@ -479,7 +762,7 @@ static int eno_momentum(tb_ctx *c, double rhs_dt, bool store_conv) {
 
 int tb_eno_momentum(tb_ctx *c) { return eno_momentum(c, 0.0, true); }
 
-static int eno_conc(tb_ctx *c, double rhs_dt, bool store_conv) {
+static ConcArgs conc_args(tb_ctx *c, double rhs_dt, bool store_conv) {
     ConcArgs a;
     a.d = c->d; a.g = c->g;
     a.u = c->f[TB_F_U]; a.v = c->f[TB_F_V]; a.w = c->f[TB_F_W]; a.c = c->f[TB_F_C];
@@ -491,6 +774,11 @@ static int eno_conc(tb_ctx *c, double rhs_dt, bool store_conv) {
     a.rc = c->f[TB_F_RHS_C];
     a.a_dt = rhs_dt > 0.0 ? 1.0 / rhs_dt : 0.0;
     a.store_conv = store_conv ? 1 : 0;
+    return a;
+}
+
+static int eno_conc(tb_ctx *c, double rhs_dt, bool store_conv) {
+    const ConcArgs a = conc_args(c, rhs_dt, store_conv);
     Tiles t = make_tiles(c->d.NX, c->d.NY, c->d.NZ);
     const dim3 grid = tiles_grid(t), blk = tiles_block();
     const bool rhs = rhs_dt > 0.0;
@@ -522,7 +810,73 @@ int tb_eno_conc(tb_ctx *c) { return eno_conc(c, 0.0, true); }
 // the convective term in registers), without re-reading the convective terms
 // and fields.  tb_step passes store_conv = false (nothing downstream reads
 // the convective terms: the TB_F_CONV_* fields keep their last values).
+// Second order: u, v, w and c in one k-marching pass (k_eno_fused2) plus
+// the zeros of the last planes
+static int eno_fused(tb_ctx *c, double rhs_dt, bool store_conv) {
+    if (eno_weights(c)) return -1;
+    const MomArgs m = mom_args(c, rhs_dt, store_conv);
+    const ConcArgs ca = conc_args(c, rhs_dt, store_conv);
+    FusedArgs A;
+    A.d = c->d;
+    A.u = m.u; A.v = m.v; A.w = m.w; A.c = ca.c;
+    A.su = m.su; A.sv = m.sv; A.sw = m.sw; A.sc = ca.sc;
+    A.dpx = m.dpx; A.dpy = m.dpy; A.dpz = m.dpz;
+    A.cu = m.cu; A.cv = m.cv; A.cw = m.cw; A.cc = ca.cc;
+    A.ru = m.ru; A.rv = m.rv; A.rw = m.rw; A.rc = ca.rc;
+    A.W = m.W;
+    A.wm = m.Wmax;
+    A.xu = c->g.xu; A.xc = c->g.xc; A.yv = c->g.yv; A.yc = c->g.yc; A.zw = c->g.zw; A.zc = c->g.zc;
+    A.a_dt = m.a_dt;
+    A.buoy = m.buoy;
+    A.v_settle = ca.v_settle;
+    A.particle = ca.particle;
+    A.store_conv = m.store_conv;
+    // the fast path needs one spacing per axis: a quantity whose uniform
+    // spacing differs from the x / y / z one of u, v, w, c's first uniform
+    // line takes the general form
+    A.unimask = 0;
+    for (int ax = 0; ax < 3; ax++) {
+        A.h[ax] = 0.0;
+        A.rh[ax] = 0.0;
+        A.hi[ax] = ax == 0 ? c->d.NX - 2 : (ax == 1 ? c->d.NY - 2 : c->d.NZ - 2);
+        for (int q = 0; q < 4; q++) {
+            const LineSpec L = (q < 3 ? m.L[q] : ca.L).ax[ax];
+            if (!L.uni || L.lo != (q == ax ? 1 : 0) || L.hi != A.hi[ax]) continue;
+            if (A.h[ax] == 0.0) { A.h[ax] = L.h; A.rh[ax] = L.rh; }
+            if (L.h == A.h[ax] && L.rh == A.rh[ax]) A.unimask |= 1 << (3 * q + ax);
+        }
+    }
+    const Dims &d = c->d;
+    const bool rhs = rhs_dt > 0.0;
+    if (A.unimask != 0xFFF || d.NX < 2 * EF_SHELL + 2 || d.NY < 2 * EF_SHELL + 2 || d.NZ < 2 * EF_SHELL + 2)
+        return 1;   // not this kernel's grid: the caller runs the per-point kernels
+    const EfTiles T = make_ef_tiles(d, EF_SHELL);
+    const dim3 grid((unsigned)(T.per_xcd * NXCD)), blk(EF_TX, EF_TY);
+    if (c->prof_on) tb_prof_begin(c, TB_K_ENO_MOM);
+    if (rhs) hipLaunchKernelGGL(k_eno_fused2<true>, grid, blk, 0, c->s, A, T);
+    else hipLaunchKernelGGL(k_eno_fused2<false>, grid, blk, 0, c->s, A, T);
+    if (c->prof_on) tb_prof_end(c, TB_K_ENO_MOM);
+    const long H = EF_SHELL;
+    const long nxs = 2 * H * d.NY * d.NZ, nys = (d.NX - 2 * H) * 2 * H * d.NZ,
+               nzs = (d.NX - 2 * H) * (d.NY - 2 * H) * 2 * H;
+    const long np = nxs + nys + nzs;
+    if (c->prof_on) tb_prof_begin(c, TB_K_ENO_CONC);
+    if (rhs)
+        hipLaunchKernelGGL(k_eno_shell<true>, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, c->s, m, ca, nxs,
+                           nys, nzs);
+    else
+        hipLaunchKernelGGL(k_eno_shell<false>, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, c->s, m, ca, nxs,
+                           nys, nzs);
+    if (c->prof_on) tb_prof_end(c, TB_K_ENO_CONC);
+    TB_LAUNCH_CHECK(c);
+    return 0;
+}
+
 int tb_eno_rhs_async(tb_ctx *c, double dt, bool store_conv) {
+    if (c->cfg.eno_order == 2) {
+        const int rc = eno_fused(c, dt, store_conv);
+        if (rc <= 0) return rc;
+    }
     if (eno_momentum(c, dt, store_conv)) return -1;
     return eno_conc(c, dt, store_conv);
 }
@@ -537,8 +891,13 @@ extern "C" int tb_eno_conv_rhs(tb_ctx *c, double dt) {
 extern "C" int tb_eno_conv(tb_ctx *c, int q) {
     if (!c || q < -1 || q > 3) { if (c) c->err = "tb_eno_conv: bad quantity"; return -1; }
     int rc = 0;
-    if (q != TB_Q_C) rc = tb_eno_momentum(c);   // u, v, w are produced together
-    if (rc == 0 && (q == TB_Q_C || q == -1)) rc = tb_eno_conc(c);
+    if (q == -1 && c->cfg.eno_order == 2) rc = eno_fused(c, 0.0, true);
+    if (rc < 0) return rc;
+    if (!(q == -1 && c->cfg.eno_order == 2) || rc > 0) {
+        rc = 0;
+        if (q != TB_Q_C) rc = tb_eno_momentum(c);   // u, v, w are produced together
+        if (rc == 0 && (q == TB_Q_C || q == -1)) rc = tb_eno_conc(c);
+    }
     if (rc == 0) TB_CHECK(c, hipStreamSynchronize(c->s));
     return rc;
 }
