@@ -371,8 +371,102 @@ int tb_rk_average(tb_ctx *ctx, double old_frac, double new_frac);
  * Velocity_modify_diagonal(dt, dt_old) accumulates (+1/dt - 1/dt_old). */
 int tb_cfl_dt(tb_ctx *ctx, double alpha, double dt_max, double *dt, double *rate);
 
+/* Concentration fields (params->NConc and the per-field arrays Pe, V_s0,
+ * Conc_alpha, Phi, conc_type read in Input.c; one Concentration object per
+ * field, Conc_create, Conc.c:22-217).  A context starts with one field described by
+ * tb_config (pe, conc_type, v_settle). */
+#define TB_NCONC_MAX 4
+typedef struct tb_conc_field {
+    int32_t type;               /* TB_SALINITY or TB_PARTICLE (params->conc_type[i]) */
+    int32_t reserved;
+    double pe;                  /* Peclet number (params->Pe[i]) */
+    double v_settle;            /* settling speed as used, negative = downward
+                                   (params->V_s0[i] after Input.c:480 flips it) */
+    double alpha;               /* weight in the total concentration (params->Conc_alpha[i]) */
+    double phi_max;             /* particle volume fraction at c = 1 (params->Phi[i]) */
+} tb_conc_field;
+/* Sets nconc (1..TB_NCONC_MAX) fields; fields[0] replaces tb_config's pe,
+ * conc_type and v_settle.  Fields 1.. are allocated (zero) or freed as
+ * nconc grows or shrinks; field 0 keeps its data.  hindered != 0 is
+ * params->hindered_settling (Input.c:369): a particle field is then advected
+ * with v + V_s0 (1 - c phi_max)^4 on FLUID cells and v elsewhere
+ * (Conc_compute_particle_settling_speed, Conc_settling_speed_function and
+ * Conc_set_particle_v_velocity, Conc.c:734-836; the power is taken as two
+ * squarings, within 2 ulp of pow()), instead of v + V_s0 everywhere.  The
+ * no-flux top wall of a sedimenting field keeps |V_s0| (Conc.c:1156-1181).
+ * Selects field 0. */
+int tb_set_conc_fields(tb_ctx *ctx, int nconc, const tb_conc_field *fields, int hindered);
+/* Makes field n the one that the TB_Q_C operators (tb_eno_conv, tb_set_rhs,
+ * tb_helmholtz_*, tb_diagnostics' mass, tb_stokes_dissipation,
+ * tb_sedimentation_rate) and the transfers of TB_F_C, TB_F_CONV_C and
+ * TB_F_RHS_C act on: the reference passes c[iconc] to each Conc_* call. */
+int tb_select_conc(tb_ctx *ctx, int n);
+/* Conc_compute_total_concentration (Conc.c:1291-1314): with more than one
+ * field, c_total = alpha_0 c_0, then c_total += alpha_i c_i for i = 1..; the
+ * v right-hand side then takes its buoyancy term from c_total instead of c_0
+ * (Velocity.c:1798-1815).  A no-op for one field.  tb_step calls it at the
+ * start of each stage. */
+int tb_total_concentration(tb_ctx *ctx);
+
+/* Output-stage diagnostics (Output_flow_properties, Output.c:2062-2216).
+ * Column arrays are (nz+1) x (nx+1) doubles, x fastest; an output pointer
+ * may be NULL.  Sums over cells run in a fixed order of their own (FP64,
+ * relative differences from the reference's serial loop of order 1e-15 per
+ * 1e6 cells); column values and sums over columns follow the reference's
+ * operations and order. */
+/* Extract_viscous_dissipation_rate (Extract.c:453-725), without inflow or
+ * outflow: (1/Re) sum over FLUID cells of u lap(u) dV_u + v lap(v) dV_v +
+ * w lap(w) dV_w, with the wall ghosts the reference builds from Boundary.h
+ * (no-slip -u, free-slip +u; the wall-normal component 0) */
+int tb_viscous_dissipation(tb_ctx *ctx, double *rate);
+/* Extract_kinetic_potential_conversion_rate (Extract.c:727-802): sum over
+ * FLUID cells of -v_c c_total dV (the total of tb_total_concentration,
+ * recomputed here, with several fields; field 0 with one) */
+int tb_kinetic_potential_conversion(tb_ctx *ctx, double *rate);
+/* Conc_compute_potential_energies (Conc.c:1934-2058) of the selected field */
+typedef struct tb_energies {
+    double ep;                  /* alpha sum c dV (y_c - y_v[0]) over FLUID cells */
+    double ep_passive;          /* particles: sum deposit dx dz (y_c[j_s] - y_v[0]) */
+    double stokes;              /* |V_s0| alpha sum c dV (particles) */
+    double conversion;          /* alpha sum v_c c dV */
+    double ep_passive_rate;     /* ep_passive |V_s0| */
+    double ep_rate;             /* conversion + ep_passive_rate - stokes */
+} tb_energies;
+int tb_potential_energies(tb_ctx *ctx, tb_energies *out);
+/* Extract_wall_shear (Extract.c:18-196), or with quadratic != 0
+ * Extract_wall_shear_quadratic (Extract.c:197-452): per (i, k) column of
+ * the bottom (the interface cell of Grid_set_interface_y_index,
+ * Grid.c:1972-2009) the shear stress and the tangential velocity.  At an
+ * IMMERSED cell the velocity is interpolated trilinearly
+ * (MyMath_do_trilinear_inter, MyMath.c:234-360) at image_point1 = control
+ * point + n sqrt(3) dy_c (Immersed.c:1135-1143), and for the quadratic fit
+ * through the control point also at image_point2 (twice as far); at a FLUID
+ * cell of row 0 the cell-centre u and w over the half-cell height.  Columns
+ * without an interface cell give 0. */
+int tb_wall_shear(tb_ctx *ctx, int quadratic, double *tau, double *u_tan, double *v_tan, double *w_tan);
+/* Conc_compute_ave_height_x (Conc.c:1524-1608) of the selected field: nx+1
+ * values h[i] = 1/(Ly Lz) sum over the FLUID cells of plane i of c dy dz */
+int tb_ave_height_x(tb_ctx *ctx, double *h);
+/* Conc_find_front_location (Conc.c:1610-1673, front_limit 0.01 at
+ * Output.c:2139): the front from the span-averaged heights of every field
+ * weighted by alpha; -1 when no crossing is found */
+int tb_front_location(tb_ctx *ctx, double front_limit, double *x_front);
+/* Conc_integrate_deposited_height (Conc.c:1675-1749) of the selected field,
+ * after a step: at each column's interface cell the deposit grows by
+ * dt |V_s0| weight (c + c_old)/2, c_old the field at the start of the step
+ * (0 growth for salinity) */
+int tb_deposit_integrate(tb_ctx *ctx, double weight, double dt);
+/* the selected field's deposit (W_deposit_height, Conc.c:1751-1793), and
+ * setting it (a restart, or 0 to reset) */
+int tb_deposit_height(tb_ctx *ctx, double *h);
+int tb_deposit_set(tb_ctx *ctx, const double *h);
+/* Conc_dump_particles (Conc.c:1840-1933) of the selected field: the deposit
+ * plus weight times the column integral of the suspended c dy */
+int tb_deposit_dumped(tb_ctx *ctx, double weight, double *h);
+
 /* composite SSP-RK2 step: the per-step sequence of the unpublished driver
- * (README.md:26), reconstructed from the TVD_RK2 scheme (definitions.h:148-149) */
+ * (README.md:26), reconstructed from the TVD_RK2 scheme (definitions.h:148-149);
+ * every concentration field is stepped */
 int tb_step(tb_ctx *ctx, double dt, tb_step_stats *st);
 
 /* solver controls (no reference counterpart: the reference fixes these in

@@ -65,6 +65,25 @@ def lib():
         L.ora_step.argtypes = [P, C.c_double, C.POINTER(abi.StepStats)]
         L.ora_cfl_dt.restype = C.c_double
         L.ora_cfl_dt.argtypes = [P, C.c_double, C.c_double, C.POINTER(C.c_double)]
+        L.ora_set_conc_fields.argtypes = [P, C.c_int, P, C.c_int]
+        L.ora_select_conc.argtypes = [P, C.c_int]
+        L.ora_total_concentration.argtypes = [P]
+        L.ora_hindered_speed.restype = C.c_double
+        L.ora_interface_y_index.argtypes = [P, C.c_int, C.c_int]
+        L.ora_viscous_dissipation.restype = C.c_double
+        L.ora_viscous_dissipation.argtypes = [P]
+        L.ora_kinetic_potential_conversion.restype = C.c_double
+        L.ora_kinetic_potential_conversion.argtypes = [P]
+        L.ora_potential_energies.argtypes = [P, P]
+        L.ora_ave_height_x.argtypes = [P, P]
+        L.ora_front_location.restype = C.c_double
+        L.ora_front_location.argtypes = [P, C.c_double]
+        L.ora_deposit_integrate.argtypes = [P, C.c_double, C.c_double]
+        L.ora_deposit_get.argtypes = [P, P]
+        L.ora_deposit_set.argtypes = [P, P]
+        L.ora_deposit_dumped.argtypes = [P, C.c_double, P]
+        L.ora_wall_shear.argtypes = [P, C.c_int, P, P, P, P]
+        L.ora_hindered_speed.argtypes = [C.c_double, C.c_double, C.c_double]
         L.ora_gmg_levels.argtypes = [P, P, C.c_int]
         L.ora_helmholtz_matrix.argtypes = [P, C.c_int, C.c_double, P, P, P]
         L.ora_poisson_matrix.argtypes = [P, P, P, P]
@@ -224,6 +243,64 @@ class Oracle:
         self.L.ora_step(self.h, dt, C.byref(st))
         return st
 
+    def set_conc_fields(self, fields, hindered=False):
+        arr = abi.conc_fields(fields)
+        assert self.L.ora_set_conc_fields(self.h, len(fields), C.cast(arr, C.c_void_p), 1 if hindered else 0) == 0
+
+    def select_conc(self, n):
+        assert self.L.ora_select_conc(self.h, n) == 0
+
+    def total_concentration(self):
+        return self.L.ora_total_concentration(self.h)
+
+    def _cols(self):
+        return np.zeros((self.cfg.nz + 1, self.cfg.nx + 1))
+
+    def interface_y_index(self, i, k):
+        return self.L.ora_interface_y_index(self.h, i, k)
+
+    def viscous_dissipation(self):
+        return self.L.ora_viscous_dissipation(self.h)
+
+    def kinetic_potential_conversion(self):
+        return self.L.ora_kinetic_potential_conversion(self.h)
+
+    def potential_energies(self):
+        out = np.zeros(6)
+        self.L.ora_potential_energies(self.h, _ptr(out))
+        return dict(zip(("ep", "ep_passive", "stokes", "conversion", "ep_passive_rate", "ep_rate"), out.tolist()))
+
+    def wall_shear(self, quadratic=False):
+        out = {n: self._cols() for n in ("tau", "u_tan", "v_tan", "w_tan")}
+        self.L.ora_wall_shear(self.h, 1 if quadratic else 0, _ptr(out["tau"]), _ptr(out["u_tan"]),
+                              _ptr(out["v_tan"]), _ptr(out["w_tan"]))
+        return out
+
+    def ave_height_x(self):
+        h = np.zeros(self.cfg.nx + 1)
+        self.L.ora_ave_height_x(self.h, _ptr(h))
+        return h
+
+    def front_location(self, limit=0.01):
+        return self.L.ora_front_location(self.h, limit)
+
+    def deposit_integrate(self, weight, dt):
+        return self.L.ora_deposit_integrate(self.h, weight, dt)
+
+    def deposit_height(self):
+        h = self._cols()
+        self.L.ora_deposit_get(self.h, _ptr(h))
+        return h
+
+    def deposit_set(self, h):
+        h = np.ascontiguousarray(h, dtype=np.float64).reshape(self.cfg.nz + 1, self.cfg.nx + 1)
+        self.L.ora_deposit_set(self.h, _ptr(h))
+
+    def deposit_dumped(self, weight):
+        h = self._cols()
+        self.L.ora_deposit_dumped(self.h, weight, _ptr(h))
+        return h
+
     def cfl_dt(self, alpha, dt_max):
         """(dt, rate): the CFL step of the host driver's rule."""
         r = C.c_double()
@@ -272,6 +349,11 @@ def eno_line(order, D0, pos, vel):
     out = np.zeros_like(D0)
     lib().ora_eno_line(order, D0.size, _ptr(D0), _ptr(pos), _ptr(vel), _ptr(out))
     return out
+
+
+def hindered_speed(conc, phi_max, v_s0):
+    """Conc_settling_speed_function as restated: V_s0 (1 - c phi_max)^4."""
+    return lib().ora_hindered_speed(conc, phi_max, v_s0)
 
 
 def gauss_jordan(a):

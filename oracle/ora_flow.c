@@ -25,6 +25,11 @@ ora_ctx *ora_create(const tb_config *cfg) {
     for (int f = 0; f < TB_F_COUNT; f++) o->f[f] = dalloc(o->NT);
     o->tr1 = dalloc(o->NT); o->tr2 = dalloc(o->NT);
     o->ubc = dalloc(o->NT); o->vbc = dalloc(o->NT); o->wbc = dalloc(o->NT);
+    o->nconc = 1;
+    o->cs[0].c = o->f[TB_F_C]; o->cs[0].conv = o->f[TB_F_CONV_C];
+    o->cs[0].rhs = o->f[TB_F_RHS_C]; o->cs[0].old = o->old[3];
+    o->cs[0].p.type = cfg->conc_type; o->cs[0].p.pe = cfg->pe; o->cs[0].p.v_settle = cfg->v_settle;
+    o->cs[0].p.alpha = 1.0; o->cs[0].p.phi_max = 0.0;
     ora_setup_geometry(o, NULL);
     return o;
 }
@@ -34,6 +39,13 @@ void ora_destroy(ora_ctx *o) {
     double *g[] = {o->xc, o->yc, o->zc, o->xu, o->yu, o->zu, o->xv, o->yv, o->zv, o->xw, o->yw, o->zw,
                    o->dx_c, o->dy_c, o->dz_c, o->mxc, o->myc, o->mzc, o->mxu, o->myv, o->mzw};
     for (size_t a = 0; a < sizeof(g) / sizeof(g[0]); a++) free(g[a]);
+    o->f[TB_F_C] = o->cs[0].c; o->f[TB_F_CONV_C] = o->cs[0].conv;
+    o->f[TB_F_RHS_C] = o->cs[0].rhs; o->old[3] = o->cs[0].old;
+    for (int n = 1; n < TB_NCONC_MAX; n++) {
+        free(o->cs[n].c); free(o->cs[n].conv); free(o->cs[n].rhs); free(o->cs[n].old);
+    }
+    for (int n = 0; n < TB_NCONC_MAX; n++) free(o->cs[n].dep);
+    free(o->ctot);
     for (int q = 0; q < 4; q++) { free(o->st[q]); free(o->old[q]); free(o->ib[q]); ora_mat_free(&o->helm[q]); }
     for (int f = 0; f < TB_F_COUNT; f++) free(o->f[f]);
     free(o->tr1); free(o->tr2); free(o->ubc); free(o->vbc); free(o->wbc);
@@ -254,9 +266,20 @@ int ora_eno_conv(ora_ctx *o, int q) {
         const double *vadv = o->vbc;
         double *vp = NULL;
         if (o->cfg.conc_type == TB_PARTICLE) {
-            /* Conc_set_particle_v_velocity (Conc.c:734-761): VecShift */
+            /* Conc_set_particle_v_velocity (Conc.c:734-761): VecShift by
+             * V_s0, or with hindered settling VecWAXPY with the field of
+             * Conc_compute_particle_settling_speed (Conc.c:764-812): the
+             * hindered speed on FLUID cells, 0 elsewhere */
+            const double *cd = o->f[TB_F_C];
+            double phi = o->cs[o->conc_sel].p.phi_max, vs = o->cfg.v_settle;
             vp = dalloc(o->NT);
-            for (size_t id = 0; id < o->NT; id++) vp[id] = o->vbc[id] + o->cfg.v_settle;
+            for (size_t id = 0; id < o->NT; id++) {
+                if (!o->hindered) vp[id] = o->vbc[id] + vs;
+                else {
+                    double v_settl = (o->st[TB_Q_C][id] == TB_FLUID) ? ora_hindered_speed(cd[id], phi, vs) : 0.0;
+                    vp[id] = o->vbc[id] + v_settl;
+                }
+            }
             vadv = vp;
         }
         eno_direction(o, q, 0, o->ubc, conv, 1);
@@ -283,7 +306,8 @@ int ora_set_rhs(ora_ctx *o, int q, double dt) {
     if (q < 0 || q > 3) return -1;
     const double *data = o->f[TB_F_U + q], *conv = o->f[TB_F_CONV_U + q];
     const double *dp = (q < 3) ? o->f[TB_F_DPDX + q] : NULL;
-    const double *c = o->f[TB_F_C];
+    /* Velocity.c:1798-1815: c_total with several fields, else c[0] */
+    const double *c = (o->nconc > 1) ? o->ctot : o->cs[0].c;
     double *rhs = o->f[TB_F_RHS_U + q];
     double a_dt = 1.0 / dt;
     double buoy = o->cfg.buoyancy;
@@ -548,10 +572,22 @@ int ora_helmholtz_gs_sweeps(ora_ctx *o, int q, double dt, const double *b, doubl
     return 0;
 }
 
+/* a . b as DOT_BLOCKS block sums added in block order: the same value
+ * whatever the number of threads or their finishing order (an OpenMP
+ * reduction combines the threads' partials in arrival order) */
+#define DOT_BLOCKS 256
 static double dot(const double *a, const double *b, size_t n) {
+    double part[DOT_BLOCKS];
+    size_t per = (n + DOT_BLOCKS - 1) / DOT_BLOCKS;
+#pragma omp parallel for schedule(static)
+    for (int blk = 0; blk < DOT_BLOCKS; blk++) {
+        size_t i0 = (size_t)blk * per, i1 = i0 + per < n ? i0 + per : n;
+        double s = 0.0;
+        for (size_t i = i0; i < i1; i++) s += a[i] * b[i];
+        part[blk] = s;
+    }
     double s = 0.0;
-#pragma omp parallel for reduction(+ : s) schedule(static)
-    for (size_t i = 0; i < n; i++) s += a[i] * b[i];
+    for (int blk = 0; blk < DOT_BLOCKS; blk++) s += part[blk];
     return s;
 }
 
@@ -957,7 +993,8 @@ double ora_div_max(ora_ctx *o, int32_t *ijk) {
 }
 
 int ora_store_old(ora_ctx *o) {
-    for (int q = 0; q < 4; q++) memcpy(o->old[q], o->f[TB_F_U + q], sizeof(double) * o->NT);
+    for (int q = 0; q < 3; q++) memcpy(o->old[q], o->f[TB_F_U + q], sizeof(double) * o->NT);
+    for (int n = 0; n < o->nconc; n++) memcpy(o->cs[n].old, o->cs[n].c, sizeof(double) * o->NT);
     return 0;
 }
 
@@ -985,9 +1022,9 @@ double ora_cfl_dt(ora_ctx *o, double alpha, double dt_max, double *rate) {
 
 /* VecAXPBY(y, a, b, x): y = a*x + b*y  (Velocity.c:2047-2055, Conc.c:1007-1017) */
 int ora_rk_average(ora_ctx *o, double old_frac, double new_frac) {
-    for (int q = 0; q < 4; q++) {
-        double *y = o->f[TB_F_U + q];
-        const double *x = o->old[q];
+    for (int q = 0; q < 3 + o->nconc; q++) {
+        double *y = (q < 3) ? o->f[TB_F_U + q] : o->cs[q - 3].c;
+        const double *x = (q < 3) ? o->old[q] : o->cs[q - 3].old;
 #pragma omp parallel for schedule(static)
         for (size_t i = 0; i < o->NT; i++) y[i] = old_frac * x[i] + new_frac * y[i];
     }
@@ -1001,12 +1038,27 @@ int ora_rk_average(ora_ctx *o, double old_frac, double new_frac) {
 int ora_step(ora_ctx *o, double dt, tb_step_stats *st) {
     if (st) memset(st, 0, sizeof(*st));
     ora_store_old(o);
+    int sel = o->conc_sel;
     for (int stage = 0; stage < 2; stage++) {
-        for (int q = 0; q < 4; q++) ora_eno_conv(o, q);
-        for (int q = 0; q < 4; q++) ora_set_rhs(o, q, dt);
-        for (int q = 0; q < 4; q++)
+        ora_total_concentration(o);
+        for (int q = 0; q < 3; q++) ora_eno_conv(o, q);
+        for (int n = 0; n < o->nconc; n++) {
+            ora_select_conc(o, n);
+            ora_eno_conv(o, TB_Q_C);
+            ora_set_rhs(o, TB_Q_C, dt);
+        }
+        ora_select_conc(o, sel);
+        for (int q = 0; q < 3; q++) ora_set_rhs(o, q, dt);
+        for (int q = 0; q < 3; q++)
             ora_helmholtz_solve(o, q, dt, o->cfg.helm_rtol, o->cfg.helm_maxit,
                                 st ? &st->helm[stage][q] : NULL);
+        for (int n = 0; n < o->nconc; n++) {
+            tb_solve_stats cst;
+            ora_select_conc(o, n);
+            ora_helmholtz_solve(o, TB_Q_C, dt, o->cfg.helm_rtol, o->cfg.helm_maxit, &cst);
+            if (st && n == 0) st->helm[stage][TB_Q_C] = cst;
+        }
+        ora_select_conc(o, sel);
         ora_poisson_rhs(o, dt);
         ora_poisson_solve(o, o->cfg.pois_rtol, o->cfg.pois_maxit, st ? &st->pois[stage] : NULL);
         ora_project(o, dt);
@@ -1014,5 +1066,63 @@ int ora_step(ora_ctx *o, double dt, tb_step_stats *st) {
     ora_rk_average(o, 0.5, 0.5);
     double dm = ora_div_max(o, NULL);
     if (st) st->div_max = dm;
+    return 0;
+}
+
+/* ---------------------------------------------------------------------- */
+/* concentration fields                                                     */
+
+/* Conc_settling_speed_function (Conc.c:815-836): V_s0 (1 - c phi_max)^n with
+ * n = 4 (the Richardson-Zaki form the reference returns), the power taken as
+ * two squarings */
+double ora_hindered_speed(double conc, double phi_max, double v_s0) {
+    double t = 1.0 - conc * phi_max;
+    double t2 = t * t;
+    return v_s0 * (t2 * t2);
+}
+
+int ora_select_conc(ora_ctx *o, int n) {
+    if (n < 0 || n >= o->nconc) return -1;
+    const ora_conc *s = &o->cs[n];
+    o->f[TB_F_C] = s->c; o->f[TB_F_CONV_C] = s->conv; o->f[TB_F_RHS_C] = s->rhs; o->old[3] = s->old;
+    if (o->cfg.pe != s->p.pe || o->cfg.conc_type != s->p.type || o->cfg.v_settle != s->p.v_settle)
+        o->helm_dt[TB_Q_C] = 0.0;   /* the concentration rows depend on these */
+    o->cfg.pe = s->p.pe;
+    o->cfg.conc_type = s->p.type;
+    o->cfg.v_settle = s->p.v_settle;
+    o->conc_sel = n;
+    return 0;
+}
+
+int ora_set_conc_fields(ora_ctx *o, int nconc, const tb_conc_field *fields, int hindered) {
+    if (nconc < 1 || nconc > TB_NCONC_MAX || !fields) return -1;
+    ora_select_conc(o, 0);
+    for (int n = 1; n < TB_NCONC_MAX; n++) {
+        ora_conc *s = &o->cs[n];
+        if (n < nconc && !s->c) {
+            s->c = dalloc(o->NT); s->conv = dalloc(o->NT); s->rhs = dalloc(o->NT); s->old = dalloc(o->NT);
+        } else if (n >= nconc && s->c) {
+            free(s->c); free(s->conv); free(s->rhs); free(s->old); free(s->dep);
+            s->c = s->conv = s->rhs = s->old = s->dep = NULL;
+        }
+    }
+    if (nconc > 1 && !o->ctot) o->ctot = dalloc(o->NT);
+    if (nconc == 1) { free(o->ctot); o->ctot = NULL; }
+    for (int n = 0; n < nconc; n++) o->cs[n].p = fields[n];
+    o->nconc = nconc;
+    o->hindered = hindered != 0;
+    return ora_select_conc(o, 0);
+}
+
+/* Conc_compute_total_concentration (Conc.c:1291-1314): VecCopy + VecScale by
+ * alpha_0, then VecAXPY alpha_i c_i for i = 1.. */
+int ora_total_concentration(ora_ctx *o) {
+    if (o->nconc == 1) return 0;
+    for (size_t id = 0; id < o->NT; id++) {
+        double t = o->cs[0].c[id];
+        t = t * o->cs[0].p.alpha;
+        for (int n = 1; n < o->nconc; n++) t = t + o->cs[n].p.alpha * o->cs[n].c[id];
+        o->ctot[id] = t;
+    }
     return 0;
 }

@@ -32,6 +32,13 @@ typedef struct {
     double *x, *b, *r;        /* compact cell arrays */
 } ora_level;
 
+/* one concentration field (Conc_create, Conc.c:22-217) */
+typedef struct {
+    double *c, *conv, *rhs, *old;
+    double *dep;              /* deposited height per (i, k) column, x fastest */
+    tb_conc_field p;
+} ora_conc;
+
 struct ora_ctx {
     tb_config cfg;
     int ncx, ncy, ncz;
@@ -50,6 +57,11 @@ struct ora_ctx {
 
     double *f[TB_F_COUNT];
     double *old[4];
+    /* concentration fields; the selected one is viewed through f[TB_F_C],
+     * f[TB_F_CONV_C], f[TB_F_RHS_C], old[3] and cfg.pe / conc_type / v_settle */
+    ora_conc cs[TB_NCONC_MAX];
+    int nconc, conc_sel, hindered;
+    double *ctot;
     double *tr1, *tr2;         /* transposed velocities (grid->G_dummy1/2) */
     double *ubc, *vbc, *wbc;   /* cell-centre velocities (G_data_bc) */
 

@@ -66,7 +66,28 @@ double ora_div_max(ora_ctx *o, int32_t *ijk);
 int ora_store_old(ora_ctx *o);
 int ora_rk_average(ora_ctx *o, double old_frac, double new_frac);
 int ora_step(ora_ctx *o, double dt, tb_step_stats *st);
+/* concentration fields: tb_set_conc_fields / tb_select_conc /
+ * tb_total_concentration (Conc.c:22-217, 734-836, 1291-1314) */
+int ora_set_conc_fields(ora_ctx *o, int nconc, const tb_conc_field *fields, int hindered);
+int ora_select_conc(ora_ctx *o, int n);
+int ora_total_concentration(ora_ctx *o);
+/* Conc_settling_speed_function (Conc.c:815-836) as restated: V_s0 (1 - c phi)^4 */
+double ora_hindered_speed(double conc, double phi_max, double v_s0);
 double ora_cfl_dt(ora_ctx *o, double alpha, double dt_max, double *rate);
+
+/* output-stage diagnostics (ora_diag.c); per-column arrays are
+ * (NZ+1) x (NX+1), x fastest */
+int ora_interface_y_index(const ora_ctx *o, int i, int k);
+double ora_viscous_dissipation(ora_ctx *o);
+double ora_kinetic_potential_conversion(ora_ctx *o);
+int ora_potential_energies(ora_ctx *o, double *out);
+int ora_ave_height_x(ora_ctx *o, double *h);
+double ora_front_location(ora_ctx *o, double limit);
+int ora_deposit_integrate(ora_ctx *o, double weight, double dt);
+int ora_deposit_get(ora_ctx *o, double *out);
+int ora_deposit_set(ora_ctx *o, const double *in);
+int ora_deposit_dumped(ora_ctx *o, double weight, double *out);
+int ora_wall_shear(ora_ctx *o, int quadratic, double *tau, double *ut, double *vt, double *wt);
 
 /* GMG hierarchy description (for tests): returns number of levels and fills
  * dims[3*l..3*l+2] */

@@ -65,6 +65,19 @@ def lib():
     L.tb_rk_average.argtypes = [P, C.c_double, C.c_double]
     L.tb_step.argtypes = [P, C.c_double, C.POINTER(abi.StepStats)]
     L.tb_cfl_dt.argtypes = [P, C.c_double, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.tb_set_conc_fields.argtypes = [P, C.c_int, P, C.c_int]
+    L.tb_viscous_dissipation.argtypes = [P, C.POINTER(C.c_double)]
+    L.tb_kinetic_potential_conversion.argtypes = [P, C.POINTER(C.c_double)]
+    L.tb_potential_energies.argtypes = [P, C.POINTER(abi.Energies)]
+    L.tb_wall_shear.argtypes = [P, C.c_int, P, P, P, P]
+    L.tb_ave_height_x.argtypes = [P, P]
+    L.tb_front_location.argtypes = [P, C.c_double, C.POINTER(C.c_double)]
+    L.tb_deposit_integrate.argtypes = [P, C.c_double, C.c_double]
+    L.tb_deposit_height.argtypes = [P, P]
+    L.tb_deposit_set.argtypes = [P, P]
+    L.tb_deposit_dumped.argtypes = [P, C.c_double, P]
+    L.tb_select_conc.argtypes = [P, C.c_int]
+    L.tb_total_concentration.argtypes = [P]
     L.tb_set_poisson_mode.argtypes = [P, C.c_int, C.c_double]
     L.tb_set_helmholtz_rtol.argtypes = [P, C.c_double]
     L.tb_set_mixed_inner.argtypes = [P, C.c_double, C.c_int]
@@ -363,6 +376,76 @@ class Solver:
         dt, r = C.c_double(), C.c_double()
         self._ok(self.L.tb_cfl_dt(self.h, alpha, dt_max, C.byref(dt), C.byref(r)), "tb_cfl_dt")
         return dt.value, r.value
+
+    def set_conc_fields(self, fields, hindered=False):
+        """Concentration fields from dicts (type, pe, v_settle, alpha,
+        phi_max); field 0 replaces the config's; hindered is
+        params->hindered_settling.  Selects field 0."""
+        arr = abi.conc_fields(fields)
+        return self._ok(self.L.tb_set_conc_fields(self.h, len(fields), C.cast(arr, C.c_void_p), 1 if hindered else 0),
+                        "tb_set_conc_fields")
+
+    def select_conc(self, n):
+        """Field n becomes the one the concentration operators and the C,
+        CONV_C, RHS_C transfers act on (the reference's c[iconc])."""
+        return self._ok(self.L.tb_select_conc(self.h, n), "tb_select_conc")
+
+    def total_concentration(self):
+        """Conc_compute_total_concentration: the v buoyancy source of
+        several fields, sum of alpha_i c_i."""
+        return self._ok(self.L.tb_total_concentration(self.h), "tb_total_concentration")
+
+    # output-stage diagnostics (column arrays have shape (nz+1, nx+1))
+    def _cols(self):
+        return np.zeros((self.cfg.nz + 1, self.cfg.nx + 1))
+
+    def viscous_dissipation(self):
+        r = C.c_double()
+        self._ok(self.L.tb_viscous_dissipation(self.h, C.byref(r)), "tb_viscous_dissipation")
+        return r.value
+
+    def kinetic_potential_conversion(self):
+        r = C.c_double()
+        self._ok(self.L.tb_kinetic_potential_conversion(self.h, C.byref(r)), "tb_kinetic_potential_conversion")
+        return r.value
+
+    def potential_energies(self):
+        e = abi.Energies()
+        self._ok(self.L.tb_potential_energies(self.h, C.byref(e)), "tb_potential_energies")
+        return {n: getattr(e, n) for n, _ in abi.Energies._fields_}
+
+    def wall_shear(self, quadratic=False):
+        out = {n: self._cols() for n in ("tau", "u_tan", "v_tan", "w_tan")}
+        self._ok(self.L.tb_wall_shear(self.h, 1 if quadratic else 0, _ptr(out["tau"]), _ptr(out["u_tan"]),
+                                      _ptr(out["v_tan"]), _ptr(out["w_tan"])), "tb_wall_shear")
+        return out
+
+    def ave_height_x(self):
+        h = np.zeros(self.cfg.nx + 1)
+        self._ok(self.L.tb_ave_height_x(self.h, _ptr(h)), "tb_ave_height_x")
+        return h
+
+    def front_location(self, limit=0.01):
+        x = C.c_double()
+        self._ok(self.L.tb_front_location(self.h, limit, C.byref(x)), "tb_front_location")
+        return x.value
+
+    def deposit_integrate(self, weight, dt):
+        return self._ok(self.L.tb_deposit_integrate(self.h, weight, dt), "tb_deposit_integrate")
+
+    def deposit_height(self):
+        h = self._cols()
+        self._ok(self.L.tb_deposit_height(self.h, _ptr(h)), "tb_deposit_height")
+        return h
+
+    def deposit_set(self, h):
+        h = np.ascontiguousarray(h, dtype=np.float64).reshape(self.cfg.nz + 1, self.cfg.nx + 1)
+        return self._ok(self.L.tb_deposit_set(self.h, _ptr(h)), "tb_deposit_set")
+
+    def deposit_dumped(self, weight):
+        h = self._cols()
+        self._ok(self.L.tb_deposit_dumped(self.h, weight, _ptr(h)), "tb_deposit_dumped")
+        return h
 
     def set_poisson_mode(self, mode, rtol):
         return self._ok(self.L.tb_set_poisson_mode(self.h, mode, rtol), "tb_set_poisson_mode")

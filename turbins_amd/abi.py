@@ -68,6 +68,51 @@ NOALLOC_ERRORS = {-1: "bad argument", -2: "box larger than 2^31 points", -3: "no
                   -4: "kernel launch failed"}
 
 
+NCONC_MAX = 4
+
+
+class ConcField(C.Structure):
+    """tb_conc_field: one concentration field (params->conc_type[i], Pe[i],
+    V_s0[i] as used, Conc_alpha[i], Phi[i])."""
+    _fields_ = [
+        ("type", C.c_int32), ("reserved", C.c_int32),
+        ("pe", C.c_double), ("v_settle", C.c_double), ("alpha", C.c_double), ("phi_max", C.c_double),
+    ]
+
+
+def conc_fields(fields):
+    """ctypes array of tb_conc_field from dicts with keys type, pe,
+    v_settle, alpha (default 1) and phi_max (default 0)."""
+    arr = (ConcField * len(fields))()
+    for a, f in zip(arr, fields):
+        a.type = int(f["type"])
+        a.pe = float(f["pe"])
+        a.v_settle = float(f.get("v_settle", 0.0))
+        a.alpha = float(f.get("alpha", 1.0))
+        a.phi_max = float(f.get("phi_max", 0.0))
+    return arr
+
+
+def multigrain_properties(alpha, phi0, v_s0):
+    """The host-side parameter rescaling of several grain sizes
+    (RECALCULATE_MULTIGRAIN_PROPERTIES, Input.c:453-481): with
+    E_i = alpha_i / alpha_0 and E_t = sum E_i, alpha_i <- E_i / E_t (the
+    weights of the total concentration, summing to 1), Phi_i <- E_i Phi_0,
+    and every settling speed V_s0[i] changes sign (negative = downward).
+    Returns (alpha, phi, v_settle) lists."""
+    a0 = float(alpha[0])
+    et = 0.0
+    for a in alpha:
+        et += a / a0
+    e = [a / a0 for a in alpha]
+    return [ei / et for ei in e], [ei * phi0 for ei in e], [-v for v in v_s0]
+
+
+class Energies(C.Structure):
+    """tb_energies: Conc_compute_potential_energies of one field."""
+    _fields_ = [(n, C.c_double) for n in ("ep", "ep_passive", "stokes", "conversion", "ep_passive_rate", "ep_rate")]
+
+
 class SolveStats(C.Structure):
     _fields_ = [
         ("iterations", C.c_int32), ("converged", C.c_int32),

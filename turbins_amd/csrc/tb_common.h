@@ -31,6 +31,14 @@ struct GridDev {
     const double *mxc, *myc, *mzc, *mxu, *myv, *mzw;
 };
 
+// MyMath_interpolate_quantity, two points (MyMath.c:24-41): linear in x,
+// the first value where the two abscissae are closer than 1e-8
+__host__ __device__ inline double interp2d(double q0, double x0, double q1, double x1, double xi) {
+    double m = (q1 - q0) / (x1 - x0);
+    if (fabs(x1 - x0) < 1.0e-8) return q0;
+    return m * (xi - x0) + q0;
+}
+
 // ---------------------------------------------------------------------------
 // Launch geometry for point-wise sweeps: tiles of 64 x 4 points in one k
 // plane.  Workgroups are handed to the 8 XCDs round robin (block b runs on
@@ -243,6 +251,16 @@ struct Level {
 };
 
 // ---------------------------------------------------------------------------
+// One concentration field (a Concentration object, Conc.c:22-217): its data,
+// convective term, right-hand side and stage-start copy, its parameters, and
+// the Gauss-Seidel pair count and contraction its last Helmholtz solve took.
+struct ConcSlot {
+    double *c = nullptr, *conv = nullptr, *rhs = nullptr, *old = nullptr;
+    double *dep = nullptr;       // deposited height per (i, k) column, x fastest (allocated on first use)
+    tb_conc_field p{};
+    int sweeps = 0;
+    double rate = 0.0;
+};
 
 struct tb_ctx {
     tb_config cfg;
@@ -263,6 +281,15 @@ struct tb_ctx {
     double *f[TB_F_COUNT] = {};
     double *old[4] = {};
     uint8_t *st[4] = {};
+
+    // concentration fields; the selected one's buffers are f[TB_F_C],
+    // f[TB_F_CONV_C], f[TB_F_RHS_C] and old[3], its parameters cfg.pe,
+    // cfg.conc_type and cfg.v_settle (tb_conc_use)
+    ConcSlot cs[TB_NCONC_MAX];
+    int nconc = 1;
+    int conc_sel = 0;
+    int hindered = 0;            // params->hindered_settling
+    double *ctot = nullptr;      // total concentration (nconc > 1)
 
     // immersed boundary: node tables and the scaled matrix rows
     int nib[4] = {0, 0, 0, 0};
@@ -396,6 +423,43 @@ int tb_vcycle32(tb_ctx *c);
 int tb_net_predict(tb_ctx *c, double *out, int correction_only);
 int tb_net_record_one(tb_ctx *c);
 double tb_elapsed_ms(tb_ctx *c);
+int tb_conc_use(tb_ctx *c, int n);
+void tb_free_deposits(tb_ctx *c, int first);
+int tb_ib_rows(tb_ctx *c, int q);
+
+// The selected field's views are the live pointers (a Helmholtz solve may
+// hand the field a new buffer, trading it with a scratch vector): copies
+// them back into its slot; every reader of the slot table calls it first.
+inline void tb_conc_sync(tb_ctx *c) {
+    ConcSlot &s = c->cs[c->conc_sel];
+    s.c = c->f[TB_F_C];
+    s.conv = c->f[TB_F_CONV_C];
+    s.rhs = c->f[TB_F_RHS_C];
+    s.old = c->old[TB_Q_C];
+}
+
+// the source of the buoyancy term of the v right-hand side
+// (Velocity.c:1798-1815): c_total with several fields, else field 0
+inline const double *tb_buoyancy_source(tb_ctx *c) {
+    tb_conc_sync(c);
+    return c->nconc > 1 ? c->ctot : c->cs[0].c;
+}
+
+// Conc_settling_speed_function (Conc.c:815-836), the Richardson-Zaki form
+// V_s0 (1 - c phi_max)^4 with the power taken as two squarings
+__host__ __device__ inline double hindered_speed(double conc, double phi_max, double v_s0) {
+    const double t = 1.0 - conc * phi_max;
+    const double t2 = t * t;
+    return v_s0 * (t2 * t2);
+}
+
+// Conc_set_particle_v_velocity (Conc.c:734-761): the v velocity that
+// advects a particle field at a cell, from the fluid's cell-centre v
+__host__ __device__ inline double particle_v(double vbc, int status, double conc, int hindered, double phi_max,
+                                            double v_s0) {
+    if (!hindered) return vbc + v_s0;
+    return vbc + (status == TB_FLUID ? hindered_speed(conc, phi_max, v_s0) : 0.0);
+}
 
 // every entry point that writes a caller-chosen field reports it here
 inline void tb_field_written(tb_ctx *c, int f) {

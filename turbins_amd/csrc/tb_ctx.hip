@@ -124,6 +124,15 @@ extern "C" tb_ctx *tb_create(const tb_config *cfg) {
     ok = ok && hipHostMalloc((void **)&c->hscal, sizeof(double) * S_COUNT, hipHostMallocCoherent) == hipSuccess;
     ok = ok && hipHostMalloc((void **)&c->hseq, sizeof(unsigned long long), hipHostMallocCoherent) == hipSuccess;
     if (ok) *c->hseq = 0;
+    if (ok) {
+        ConcSlot &s0 = c->cs[0];
+        s0.c = c->f[TB_F_C]; s0.conv = c->f[TB_F_CONV_C]; s0.rhs = c->f[TB_F_RHS_C]; s0.old = c->old[3];
+        s0.p.type = c->cfg.conc_type;
+        s0.p.pe = c->cfg.pe;
+        s0.p.v_settle = c->cfg.v_settle;
+        s0.p.alpha = 1.0;
+        s0.p.phi_max = 0.0;
+    }
     ok = ok && hipStreamSynchronize(c->s) == hipSuccess;
     if (!ok) {
         tb_destroy(c);
@@ -148,6 +157,16 @@ extern "C" void tb_destroy(tb_ctx *c) {
     tb_mg_free(c);
     hipFree(c->dgrid);
     hipFree(c->eno_w);
+    if (c->cs[0].c) {   // field 0's buffers back in place; the others freed here
+        tb_conc_sync(c);
+        c->f[TB_F_C] = c->cs[0].c; c->f[TB_F_CONV_C] = c->cs[0].conv;
+        c->f[TB_F_RHS_C] = c->cs[0].rhs; c->old[3] = c->cs[0].old;
+    }
+    for (int n = 1; n < TB_NCONC_MAX; n++) {
+        hipFree(c->cs[n].c); hipFree(c->cs[n].conv); hipFree(c->cs[n].rhs); hipFree(c->cs[n].old);
+    }
+    hipFree(c->ctot);
+    tb_free_deposits(c, 0);
     for (int f = 0; f < TB_F_COUNT; f++) hipFree(c->f[f]);
     for (int q = 0; q < 4; q++) { hipFree(c->old[q]); hipFree(c->st[q]); free_ib(c, q); }
     for (int a = 0; a < 8; a++) hipFree(c->kv[a]);
@@ -410,20 +429,147 @@ __global__ void k_axpby4(double *y0, double *y1, double *y2, double *y3, const d
     y3[i] = a * x3[i] + b * y3[i];
 }
 
+__global__ void k_axpby1(double *y, const double *x, double a, double b, long n) {
+    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    y[i] = a * x[i] + b * y[i];
+}
+
 extern "C" int tb_store_old(tb_ctx *c) {
     if (!c) return -1;
-    for (int q = 0; q < 4; q++)
-        TB_CHECK(c, hipMemcpyAsync(c->old[q], c->f[TB_F_U + q], sizeof(double) * c->d.ntot,
-                                   hipMemcpyDeviceToDevice, c->s));
+    const size_t bytes = sizeof(double) * c->d.ntot;
+    tb_conc_sync(c);
+    for (int q = 0; q < 3; q++)
+        TB_CHECK(c, hipMemcpyAsync(c->old[q], c->f[TB_F_U + q], bytes, hipMemcpyDeviceToDevice, c->s));
+    for (int n = 0; n < c->nconc; n++)
+        TB_CHECK(c, hipMemcpyAsync(c->cs[n].old, c->cs[n].c, bytes, hipMemcpyDeviceToDevice, c->s));
     return 0;
 }
 
 extern "C" int tb_rk_average(tb_ctx *c, double old_frac, double new_frac) {
     if (!c) return -1;
     long n = c->d.ntot;
-    hipLaunchKernelGGL(k_axpby4, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->s,
-                       c->f[TB_F_U], c->f[TB_F_V], c->f[TB_F_W], c->f[TB_F_C],
-                       c->old[0], c->old[1], c->old[2], c->old[3], old_frac, new_frac, n);
+    const dim3 grid((unsigned)((n + 255) / 256));
+    tb_conc_sync(c);
+    hipLaunchKernelGGL(k_axpby4, grid, dim3(256), 0, c->s, c->f[TB_F_U], c->f[TB_F_V], c->f[TB_F_W], c->cs[0].c,
+                       c->old[0], c->old[1], c->old[2], c->cs[0].old, old_frac, new_frac, n);
+    for (int m = 1; m < c->nconc; m++)
+        hipLaunchKernelGGL(k_axpby1, grid, dim3(256), 0, c->s, c->cs[m].c, (const double *)c->cs[m].old, old_frac,
+                           new_frac, n);
+    TB_LAUNCH_CHECK(c);
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// concentration fields
+
+// Points the selected-field views (f[TB_F_C], f[TB_F_CONV_C], f[TB_F_RHS_C],
+// old[3], cfg.pe / conc_type / v_settle and the Gauss-Seidel history of q = 3)
+// at field n; the IMMERSED rows of the concentration grid are scaled by
+// ap_solid, which depends on Pe, so they are rebuilt when Pe changes.
+int tb_conc_use(tb_ctx *c, int n) {
+    tb_conc_sync(c);
+    ConcSlot &was = c->cs[c->conc_sel];
+    was.sweeps = c->helm_sweeps[TB_Q_C];
+    was.rate = c->helm_rate[TB_Q_C];
+    const ConcSlot &s = c->cs[n];
+    const bool pe_changed = c->cfg.pe != s.p.pe;
+    c->f[TB_F_C] = s.c; c->f[TB_F_CONV_C] = s.conv; c->f[TB_F_RHS_C] = s.rhs; c->old[TB_Q_C] = s.old;
+    c->cfg.pe = s.p.pe;
+    c->cfg.conc_type = s.p.type;
+    c->cfg.v_settle = s.p.v_settle;
+    c->helm_sweeps[TB_Q_C] = s.sweeps;
+    c->helm_rate[TB_Q_C] = s.rate;
+    c->conc_sel = n;
+    if (pe_changed && c->nib[TB_Q_C] > 0) return tb_ib_rows(c, TB_Q_C);
+    return 0;
+}
+
+static void dev_release(tb_ctx *c, double *&p) {
+    if (!p) return;
+    hipFree(p);
+    p = nullptr;
+    c->bytes -= (int64_t)(sizeof(double) * (size_t)c->d.ntot);
+}
+
+extern "C" int tb_set_conc_fields(tb_ctx *c, int nconc, const tb_conc_field *fields, int hindered) {
+    if (!c || nconc < 1 || nconc > TB_NCONC_MAX || !fields) {
+        if (c) c->err = "tb_set_conc_fields: bad argument";
+        return -1;
+    }
+    for (int n = 0; n < nconc; n++) {
+        const tb_conc_field &p = fields[n];
+        if ((p.type != TB_SALINITY && p.type != TB_PARTICLE) || !(p.pe > 0.0) || !std::isfinite(p.pe) ||
+            !std::isfinite(p.v_settle) || !std::isfinite(p.alpha) || !std::isfinite(p.phi_max)) {
+            c->err = "tb_set_conc_fields: field " + std::to_string(n) + " has a bad type or parameter";
+            return -1;
+        }
+    }
+    if (tb_conc_use(c, 0)) return -1;
+    TB_CHECK(c, hipStreamSynchronize(c->s));
+    const size_t fb = sizeof(double) * (size_t)c->d.ntot;
+    for (int n = 1; n < TB_NCONC_MAX; n++) {
+        ConcSlot &s = c->cs[n];
+        if (n < nconc) {
+            if (!s.c) {
+                if (dev_alloc(c, (void **)&s.c, fb) || dev_alloc(c, (void **)&s.conv, fb) ||
+                    dev_alloc(c, (void **)&s.rhs, fb) || dev_alloc(c, (void **)&s.old, fb))
+                    return -1;
+            }
+        } else {
+            dev_release(c, s.c); dev_release(c, s.conv); dev_release(c, s.rhs); dev_release(c, s.old);
+        }
+    }
+    tb_free_deposits(c, nconc);
+    if (nconc > 1 && !c->ctot) {
+        if (dev_alloc(c, (void **)&c->ctot, fb)) return -1;
+    } else if (nconc == 1) {
+        dev_release(c, c->ctot);
+    }
+    for (int n = 0; n < nconc; n++) {
+        c->cs[n].p = fields[n];
+        c->cs[n].sweeps = 0;
+        c->cs[n].rate = 0.0;
+    }
+    c->helm_sweeps[TB_Q_C] = 0;
+    c->helm_rate[TB_Q_C] = 0.0;
+    c->nconc = nconc;
+    c->hindered = hindered != 0;
+    if (tb_conc_use(c, 0)) return -1;   // field 0's (possibly new) parameters
+    TB_CHECK(c, hipStreamSynchronize(c->s));
+    return 0;
+}
+
+extern "C" int tb_select_conc(tb_ctx *c, int n) {
+    if (!c || n < 0 || n >= c->nconc) {
+        if (c) c->err = "tb_select_conc: no such concentration field";
+        return -1;
+    }
+    return tb_conc_use(c, n);
+}
+
+// c_total = alpha_0 c_0, then c_total += alpha_i c_i (VecCopy, VecScale and
+// VecAXPY in Conc_compute_total_concentration, Conc.c:1291-1314)
+__global__ void k_total_conc(double *t, const double *c0, const double *c1, const double *c2, const double *c3,
+                             double a0, double a1, double a2, double a3, int nconc, long n) {
+    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s = a0 * c0[i];
+    if (nconc > 1) s = s + a1 * c1[i];
+    if (nconc > 2) s = s + a2 * c2[i];
+    if (nconc > 3) s = s + a3 * c3[i];
+    t[i] = s;
+}
+
+extern "C" int tb_total_concentration(tb_ctx *c) {
+    if (!c) return -1;
+    if (c->nconc == 1) return 0;
+    tb_conc_sync(c);
+    const ConcSlot *s = c->cs;
+    const long n = c->d.ntot;
+    hipLaunchKernelGGL(k_total_conc, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->s, c->ctot,
+                       (const double *)s[0].c, (const double *)s[1].c, (const double *)s[2].c,
+                       (const double *)s[3].c, s[0].p.alpha, s[1].p.alpha, s[2].p.alpha, s[3].p.alpha, c->nconc, n);
     TB_LAUNCH_CHECK(c);
     return 0;
 }

@@ -19,12 +19,6 @@
 
 #include "tb_common.h"
 
-__device__ inline double interp2d(double q0, double x0, double q1, double x1, double xi) {
-    double m = (q1 - q0) / (x1 - x0);
-    if (fabs(x1 - x0) < 1.0e-8) return q0;
-    return m * (xi - x0) + q0;
-}
-
 // MyMath_interpolate_quantity, four points (MyMath.c:52-73)
 __device__ inline double interp4d(double q0, double q1, double q2, double q3, double X10, double X11, double X20,
                                   double X22, double x1, double x2) {
@@ -312,6 +306,8 @@ struct ConcArgs {
     double *cc;
     double v_settle;
     int particle;
+    int hindered;   // particle and params->hindered_settling
+    double phi_max;
     QLines L;
     double *rc;     // right-hand side in the same pass (RHS = true): the k_rhs_conc formula
     double a_dt;
@@ -330,7 +326,7 @@ __device__ inline void conc_point(const ConcArgs &a, int i, int j, int k) {
         double ubc = interp2d(a.u[id], g.xu[i], a.u[id + 1], g.xu[i + 1], g.xc[i]);
         double vbc = interp2d(a.v[id], g.yv[j], a.v[id + d.px], g.yv[j + 1], g.yc[j]);
         double wbc = interp2d(a.w[id], g.zw[k], a.w[id + d.plane], g.zw[k + 1], g.zc[k]);
-        if (a.particle) vbc = vbc + a.v_settle;   // Conc_set_particle_v_velocity (Conc.c:734-761)
+        if (a.particle) vbc = particle_v(vbc, s, a.c[id], a.hindered, a.phi_max, a.v_settle);
         r = eno_line_point<R>(a.L.ax[0], a.c, id, i, ubc);
         r += eno_line_point<R>(a.L.ax[1], a.c, id, j, vbc);
         r += eno_line_point<R>(a.L.ax[2], a.c, id, k, wbc);
@@ -505,8 +501,8 @@ struct FusedArgs {
     const double *W;          // transpose weight tables (W_COUNT x wm)
     int wm;
     const double *xu, *xc, *yv, *yc, *zw, *zc;
-    double a_dt, buoy, v_settle;
-    int particle, store_conv;
+    double a_dt, buoy, v_settle, phi_max;
+    int particle, hindered, store_conv;
     // uniform fast path: bit 3*q + ax set when axis ax of quantity q is a
     // power-of-two uniform line of spacing h[ax]; hi[ax] the last interior
     // index (the first is 1 on the quantity's own axis, else 0)
@@ -657,7 +653,7 @@ __global__ void __launch_bounds__(EF_THREADS) k_eno_fused2(FusedArgs A, EfTiles 
                     const double ubc = interp2d(uc, A.xu[i], Uk[o + 1], A.xu[i + 1], A.xc[i]);
                     double vbc = interp2d(vc, A.yv[j], Vk[o + EF_SX], A.yv[j + 1], A.yc[j]);
                     const double wbc = interp2d(wc, A.zw[k], Wf[o], A.zw[k + 1], A.zc[k]);
-                    if (A.particle) vbc = vbc + A.v_settle;   // Conc_set_particle_v_velocity (Conc.c:734-761)
+                    if (A.particle) vbc = particle_v(vbc, s, cc, A.hindered, A.phi_max, A.v_settle);
                     r = eno_u5(Ck[o - 2], Ck[o - 1], cc, Ck[o + 1], Ck[o + 2], rx, hx, ubc);
                     r += eno_u5(Ck[o - 2 * EF_SX], Ck[o - EF_SX], cc, Ck[o + EF_SX], Ck[o + 2 * EF_SX], ry, hy, vbc);
                     r += eno_u5(zm2[3], Cbk[o], cc, Cf[o], zp2[3], rz, hz, wbc);
@@ -726,7 +722,8 @@ static MomArgs mom_args(tb_ctx *c, double rhs_dt, bool store_conv) {
     a.su = c->st[0]; a.sv = c->st[1]; a.sw = c->st[2];
     a.cu = c->f[TB_F_CONV_U]; a.cv = c->f[TB_F_CONV_V]; a.cw = c->f[TB_F_CONV_W];
     for (int q = 0; q < 3; q++) a.L[q] = make_lines(c, q);
-    a.dpx = c->f[TB_F_DPDX]; a.dpy = c->f[TB_F_DPDY]; a.dpz = c->f[TB_F_DPDZ]; a.cfld = c->f[TB_F_C];
+    a.dpx = c->f[TB_F_DPDX]; a.dpy = c->f[TB_F_DPDY]; a.dpz = c->f[TB_F_DPDZ];
+    a.cfld = tb_buoyancy_source(c);
     a.ru = c->f[TB_F_RHS_U]; a.rv = c->f[TB_F_RHS_V]; a.rw = c->f[TB_F_RHS_W];
     a.a_dt = rhs_dt > 0.0 ? 1.0 / rhs_dt : 0.0;
     a.buoy = c->cfg.buoyancy;
@@ -770,6 +767,8 @@ static ConcArgs conc_args(tb_ctx *c, double rhs_dt, bool store_conv) {
     a.cc = c->f[TB_F_CONV_C];
     a.v_settle = c->cfg.v_settle;
     a.particle = (c->cfg.conc_type == TB_PARTICLE);
+    a.hindered = a.particle && c->hindered;
+    a.phi_max = c->cs[c->conc_sel].p.phi_max;
     a.L = make_lines(c, TB_Q_C);
     a.rc = c->f[TB_F_RHS_C];
     a.a_dt = rhs_dt > 0.0 ? 1.0 / rhs_dt : 0.0;
@@ -830,6 +829,8 @@ static int eno_fused(tb_ctx *c, double rhs_dt, bool store_conv) {
     A.buoy = m.buoy;
     A.v_settle = ca.v_settle;
     A.particle = ca.particle;
+    A.hindered = ca.hindered;
+    A.phi_max = ca.phi_max;
     A.store_conv = m.store_conv;
     // the fast path needs one spacing per axis: a quantity whose uniform
     // spacing differs from the x / y / z one of u, v, w, c's first uniform
@@ -848,6 +849,8 @@ static int eno_fused(tb_ctx *c, double rhs_dt, bool store_conv) {
     }
     const Dims &d = c->d;
     const bool rhs = rhs_dt > 0.0;
+    // several fields: the v buoyancy reads c_total, not the staged field
+    if (c->nconc > 1) return 1;
     if (A.unimask != 0xFFF || d.NX < 2 * EF_SHELL + 2 || d.NY < 2 * EF_SHELL + 2 || d.NZ < 2 * EF_SHELL + 2)
         return 1;   // not this kernel's grid: the caller runs the per-point kernels
     const EfTiles T = make_ef_tiles(d, EF_SHELL);
@@ -872,13 +875,19 @@ static int eno_fused(tb_ctx *c, double rhs_dt, bool store_conv) {
     return 0;
 }
 
+// every concentration field is computed (each Conc_* call of the reference
+// takes its c[iconc])
 int tb_eno_rhs_async(tb_ctx *c, double dt, bool store_conv) {
     if (c->cfg.eno_order == 2) {
         const int rc = eno_fused(c, dt, store_conv);
         if (rc <= 0) return rc;
     }
     if (eno_momentum(c, dt, store_conv)) return -1;
-    return eno_conc(c, dt, store_conv);
+    if (c->nconc == 1) return eno_conc(c, dt, store_conv);
+    const int sel = c->conc_sel;
+    for (int n = 0; n < c->nconc; n++)
+        if (tb_conc_use(c, n) || eno_conc(c, dt, store_conv)) return -1;
+    return tb_conc_use(c, sel);
 }
 
 extern "C" int tb_eno_conv_rhs(tb_ctx *c, double dt) {
@@ -943,7 +952,8 @@ extern "C" int tb_set_rhs(tb_ctx *c, int q, double dt) {
                            c->st[3], c->f[TB_F_RHS_C], a_dt, t);
     else
         hipLaunchKernelGGL(k_rhs_vel, tiles_grid(t), tiles_block(), 0, c->s, c->d, c->g, q, c->f[TB_F_U + q],
-                           c->f[TB_F_CONV_U + q], c->f[TB_F_DPDX + q], c->f[TB_F_C], c->st[q], c->f[TB_F_RHS_U + q],
+                           c->f[TB_F_CONV_U + q], c->f[TB_F_DPDX + q], tb_buoyancy_source(c), c->st[q],
+                           c->f[TB_F_RHS_U + q],
                            a_dt, c->cfg.buoyancy, t);
     TB_LAUNCH_CHECK(c);
     return 0;

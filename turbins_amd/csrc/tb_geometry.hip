@@ -505,6 +505,17 @@ static double ap_solid_for(tb_ctx *c, int q) {
     return -Cx * c->hgrid[15][0] - Cy * c->hgrid[16][0] - Cz * c->hgrid[17][0];
 }
 
+// the matrix rows of quantity q's IMMERSED nodes from its node table, for
+// the current Re / Pe
+int tb_ib_rows(tb_ctx *c, int q) {
+    const int n = c->nib[q];
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(k_ib_rows, dim3((n + 63) / 64), dim3(64), 0, c->s, c->d, (const tb_ib_node *)c->ib[q], n,
+                       ap_solid_for(c, q), c->ib_row[q], c->ib_cnt[q], c->ib_col[q], c->ib_val[q]);
+    TB_LAUNCH_CHECK(c);
+    return 0;
+}
+
 int tb_geometry_run(tb_ctx *c, const double *h_xz) {
     const Dims &d = c->d;
     Tiles all = make_tiles(d.NX, d.NY, d.NZ);
@@ -614,9 +625,8 @@ int tb_geometry_run(tb_ctx *c, const double *h_xz) {
         else { qc.x = c->g.xc; qc.y = c->g.yc; qc.z = c->g.zc; }
         hipLaunchKernelGGL(k_ib_nodes, dim3((n + 63) / 64), dim3(64), 0, c->s, d, c->g, q, qc,
                            c->f[TB_F_SDF_U + q], c->st[q], c->ib[q], n);
-        hipLaunchKernelGGL(k_ib_rows, dim3((n + 63) / 64), dim3(64), 0, c->s, d, c->ib[q], n, ap_solid_for(c, q),
-                           c->ib_row[q], c->ib_cnt[q], c->ib_col[q], c->ib_val[q]);
         TB_LAUNCH_CHECK(c);
+        if (tb_ib_rows(c, q)) return -1;
         TB_CHECK(c, hipStreamSynchronize(c->s));
     }
     return 0;

@@ -44,7 +44,8 @@ std::string join(const char *dir, const char *name) {
     return s + name;
 }
 
-// restart vectors: file name and field (Resume.c:186-238; one concentration)
+// restart vectors: file name and field (Resume.c:186-228); the
+// concentration fields follow as Resume_conc<i>.bin (Resume.c:229-238)
 struct ResumeVec {
     const char *name;
     int field;
@@ -52,8 +53,24 @@ struct ResumeVec {
 const ResumeVec RESUME_VECS[] = {
     {"Resume_u.bin", TB_F_U},       {"Resume_v.bin", TB_F_V},       {"Resume_w.bin", TB_F_W},
     {"Resume_dpdx.bin", TB_F_DPDX}, {"Resume_dpdy.bin", TB_F_DPDY}, {"Resume_dpdz.bin", TB_F_DPDZ},
-    {"Resume_conc0.bin", TB_F_C},
 };
+
+// saves (load = false) or loads every concentration field's Resume_conc<i>.bin;
+// the selected field is selected again afterwards
+int resume_conc(tb_ctx *c, const char *dir, bool load) {
+    const int sel = c->conc_sel;
+    for (int n = 0; n < c->nconc; n++) {
+        const std::string name = "Resume_conc" + std::to_string(n) + ".bin";
+        if (tb_conc_use(c, n)) return -1;
+        const int rc = load ? tb_load_field(c, TB_F_C, join(dir, name.c_str()).c_str())
+                            : tb_save_field(c, TB_F_C, join(dir, name.c_str()).c_str());
+        if (rc) {
+            tb_conc_use(c, sel);
+            return -1;
+        }
+    }
+    return tb_conc_use(c, sel);
+}
 
 }  // namespace
 
@@ -178,6 +195,7 @@ extern "C" int tb_write_resume(tb_ctx *c, const char *dir, const tb_resume_info 
     }
     for (const ResumeVec &v : RESUME_VECS)
         if (tb_save_field(c, v.field, join(dir, v.name).c_str())) return -1;
+    if (resume_conc(c, dir, false)) return -1;
     const int rc = tb_write_resume_primary(dir, info);
     if (rc < 0) {
         c->err = std::string("tb_write_resume Resume_primary.dat: ") + io_message(rc);
@@ -198,6 +216,7 @@ extern "C" int tb_read_resume(tb_ctx *c, const char *dir, tb_resume_info *info) 
     }
     for (const ResumeVec &v : RESUME_VECS)
         if (tb_load_field(c, v.field, join(dir, v.name).c_str())) return -1;
+    if (resume_conc(c, dir, true)) return -1;
     // a restart begins a new trajectory: the learned guess forgets the
     // solutions of the old one
     c->n_hist = c->gram_n = c->guess_nr = 0;

@@ -1,7 +1,8 @@
 // tb_step.hip -- the SSP-RK2 time step (the role of the reference's missing
 // driver, gvg.c): per stage, explicit terms of every field from the
-// stage-start state, then the four Helmholtz solves, then the Poisson solve
-// and the projection; after both stages the 0.5/0.5 RK average
+// stage-start state, then the Helmholtz solves (u, v, w and every
+// concentration field), then the Poisson solve and the projection; after
+// both stages the 0.5/0.5 RK average
 // (TVD_RK2, definitions.h:148-149) and the divergence check.
 #include <cstring>
 
@@ -17,15 +18,28 @@ extern "C" int tb_step(tb_ctx *c, double dt, tb_step_stats *st) {
     TB_CHECK(c, hipEventRecord(e0, c->s));
     int worst = 0;
     if (tb_store_old(c)) return -1;
+    const int sel = c->conc_sel;
     for (int stage = 0; stage < 2; stage++) {
-        // ENO and the four right-hand sides (tb_eno_conv + tb_set_rhs) in one
+        // c_total of the stage-start fields for the v buoyancy (several fields)
+        if (tb_total_concentration(c)) return -1;
+        // ENO and the right-hand sides (tb_eno_conv + tb_set_rhs) in one
         // pass; the convective terms stay in registers
         if (tb_eno_rhs_async(c, dt, false)) return -1;
-        for (int q = 0; q < 4; q++) {
+        for (int q = 0; q < 3; q++) {
             int rc = tb_helmholtz_solve(c, q, dt, &loc.helm[stage][q]);
             if (rc < 0) return -1;
             if (rc > worst) worst = rc;
         }
+        // every concentration field; the stats report field 0
+        for (int n = 0; n < c->nconc; n++) {
+            if (c->nconc > 1 && tb_conc_use(c, n)) return -1;
+            tb_solve_stats cst;
+            int rc = tb_helmholtz_solve(c, TB_Q_C, dt, &cst);
+            if (rc < 0) return -1;
+            if (rc > worst) worst = rc;
+            if (n == 0) loc.helm[stage][TB_Q_C] = cst;
+        }
+        if (c->nconc > 1 && tb_conc_use(c, sel)) return -1;
         if (tb_poisson_rhs(c, dt)) return -1;
         int rc = tb_poisson_solve(c, &loc.pois[stage]);
         if (rc < 0) return -1;
