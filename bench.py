@@ -335,6 +335,11 @@ def main():
                              + (" + network direction while its gate is open" if learned else "")
                              + ", FP32 CG + FP32 V-cycles inside FP64 iterative refinement",
             "learned_guess_loaded": learned,
+            # what the timed headline steps ran: the network's direction
+            # only while its run-time gate was open
+            "headline_effective": ("Galerkin + network + mixed" if gate and gate.get("open")
+                                   else "Galerkin + mixed (network gate closed)" if learned
+                                   else "Galerkin + mixed (no network loaded)"),
             "network_gate": gate,
             "plain_path": "FP64 GMG-PCG",
             "plain_timesteps_per_s": round(plain_sps, 4),

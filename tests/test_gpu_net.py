@@ -137,8 +137,15 @@ def test_trained_network_gain():
     prev, sub, lrn = res["previous"], res["subspace"], res["learned"]
     print("iterations previous / subspace / learned:", prev["iterations_mean"], sub["iterations_mean"],
           lrn["iterations_mean"], "gate", lrn["gate"])
+    # the gain the network adds beyond the Galerkin subspace of past
+    # solutions, reported as measured (it may be nothing: the gate then
+    # closes and the headline path is Galerkin + mixed)
+    gain = sub["iterations_mean"] - lrn["iterations_mean"]
+    print(f"iterations saved beyond the subspace guess: {gain:+.3f} per solve "
+          f"(subspace guess alone saves {prev['iterations_mean'] - sub['iterations_mean']:+.3f})")
     assert lrn["iterations_mean"] <= prev["iterations_mean"]
     assert lrn["res0_median"] <= prev["res0_median"]
+    assert gain >= -0.5          # the network's direction never costs a noticeable iteration
     g = lrn["gate"]
     assert g["trials"] == 4
     # time the saved iterations are worth, priced at the subspace run's own ms per iteration
