@@ -16,9 +16,14 @@ The JSON line's `value` is the headline path's timesteps/s.  Inputs are
 resident in HBM before the timed region; each library step ends with a
 device synchronisation, so the wall clock brackets exactly K steps.
 
-Both paths start from one snapshot taken after a few untimed FP64 steps
-(past the seeded start's transient) and run W untimed then K timed steps
-from it, so they time the same stretch of the trajectory.
+Both paths start from one snapshot taken after 30 untimed FP64 steps and
+run W untimed then K timed steps from it, so they time the same stretch of
+the trajectory.  The seeded start is not solenoidal; its divergence halves
+from step to step, and over the first ~25 steps the Poisson solves start
+from large residuals that a production run (hundreds of steps) does not
+see, so the snapshot is taken after them (at 512 x 256 x 128: headline
+39.4 / plain 30.2 timesteps/s from step 4, 45.2 / 33.5 from step 30, 45.8 /
+34.1 from step 60).
 
 `roofline` reports the instrumented kernel with the largest total time in
 a few further headline-path steps run with HIP events on (outside the timed
@@ -233,8 +238,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch the V-cycle kernels one by one (for tracing tools that do not follow HIP graphs)")
-    ap.add_argument("--settle", type=int, default=4,
-                    help="untimed FP64 steps from the seeded start before the snapshot both paths run from")
+    ap.add_argument("--settle", type=int, default=30,
+                    help="untimed FP64 steps from the seeded start before the snapshot both paths run from "
+                         "(the start's non-solenoidal transient decays over the first ~25 steps)")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
     ap.add_argument("--cpu-warmup", type=int, default=1)
     ap.add_argument("--cpu-reps", type=int, default=3)

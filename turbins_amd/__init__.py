@@ -31,9 +31,12 @@ def lib():
     global _LIB
     if _LIB is not None:
         return _LIB
-    if not os.path.exists(LIB_PATH):
-        raise RuntimeError(f"{LIB_PATH} is missing: run turbins_amd.build() first")
-    L = C.CDLL(LIB_PATH)
+    # TURBINS_GPU_LIB selects another build of the same sources (the
+    # host-sanitizer build of tests/test_sanitizers.py)
+    path = os.environ.get("TURBINS_GPU_LIB") or LIB_PATH
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: run turbins_amd.build() first")
+    L = C.CDLL(path)
     P = C.c_void_p
     L.tb_create.restype = P
     L.tb_create.argtypes = [C.POINTER(abi.Config)]
