@@ -350,3 +350,40 @@ def test_steps_match_oracle(name, steps):
     print("div max gpu/oracle", sg.div_max, so.div_max)
     g.close()
     o.close()
+
+
+@pytest.mark.parametrize("mode", [abi.POIS_FP64_PCG, abi.POIS_MIXED_IR])
+def test_poisson_on_a_grid_that_stops_halving(mode):
+    """60 x 30 x 15 cells over a bump: coarsening stops at 15 x 15 x 15
+    (3375 cells, too many for the dense coarse inverse), which takes 50 + 50
+    symmetric sweep pairs instead (the oracle's rule).  The solve reaches its
+    tolerance and the oracle's solution; three steps follow the oracle."""
+    c, cfg, g, o = _pair("bump_small", dims=(60, 30, 15), case_over=dict(lx=8.0), pois_rtol=1e-10, pois_mode=mode,
+                         helm_rtol=1e-13)
+    assert o.gmg_levels()[-1] == (15, 15, 15)
+    init = W.initial_fields(c, [o.flags(q) for q in range(4)])
+    for k, v in init.items():
+        g.upload(k, v)
+        o.upload(k, v)
+    dt = c["dt"]
+    g.poisson_rhs(dt)
+    o.poisson_rhs(dt)
+    sg = g.poisson_solve()
+    o.poisson_solve(rtol=1e-13)
+    err = _rel(g.download("PHI"), o.download("PHI"))
+    print("odd grid, mode", mode, "its", sg.iterations, "vcycles", sg.vcycles, "res", sg.res, "rel err", err)
+    assert sg.converged and sg.res <= 1e-10
+    assert err <= 1e-8
+    for k, v in init.items():
+        g.upload(k, v)
+        o.upload(k, v)
+    for _ in range(3):
+        st = g.step(dt)
+        assert all(p.converged for p in st.pois)
+        o.step(dt)
+    for nm in ["U", "V", "W", "C"]:
+        e = _rel(g.download(nm), o.download(nm))
+        print(nm, "rel L2 after 3 steps", e)
+        assert e <= 1e-6
+    g.close()
+    o.close()

@@ -1021,8 +1021,17 @@ int tb_mg_build(tb_ctx *c) {
         TB_CHECK(c, hipMemsetAsync(c->cg32[a], 0, sizeof(float) * L0.ntot, c->s));
         c->bytes += sizeof(float) * L0.ntot;
     }
+    // The coarsest level is solved by its dense inverse when it has at most
+    // 1024 cells.  Grid dimensions that stop halving (an odd number of
+    // cells) can leave a larger one: it then takes COARSE_SWEEPS symmetric
+    // sweep pairs from zero (the oracle's rule) and no LDS cycle is used.
+    Level &Lc = c->lev[nl - 1];
+    const long nc = (long)Lc.nx * Lc.ny * Lc.nz;
+    c->coarse_dense = nc <= 1024;
+    c->coarse_n = (int)(nc < (1L << 30) ? nc : (1L << 30));
     // first level of the LDS-resident small cycle, per precision
-    for (int a = 0; a < 2; a++) {
+    for (int a = 0; a < 2 && !c->coarse_dense; a++) c->small_l0[a] = nl;
+    for (int a = 0; a < 2 && c->coarse_dense; a++) {
         int l0 = nl - 1;
         while (l0 > 0) {
             const Level &P = c->lev[l0 - 1];
@@ -1038,12 +1047,14 @@ int tb_mg_build(tb_ctx *c) {
                                     SMALL_LDS_BYTES));
     TB_CHECK(c, hipFuncSetAttribute((const void *)k_small_cycle<double>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     SMALL_LDS_BYTES));
+    if (!c->coarse_dense) {
+        TB_CHECK(c, hipStreamSynchronize(c->s));
+        c->mg_built = true;
+        return 0;
+    }
     // dense inverse of (A_c + J/n) on the coarsest level (Gauss-Jordan,
     // partial pivoting), FP64 on the host
-    Level &Lc = c->lev[nl - 1];
-    int n = Lc.nx * Lc.ny * Lc.nz;
-    if (n > 1024) { c->err = "coarsest multigrid level too large (odd grid dimensions?)"; return -1; }
-    c->coarse_n = n;
+    const int n = (int)nc;
     std::vector<double> A((size_t)n * n), I((size_t)n * n, 0.0), e(n), col(n);
     for (int a = 0; a < n; a++) {
         std::fill(e.begin(), e.end(), 0.0);
@@ -1166,12 +1177,35 @@ static void resid_restrict(tb_ctx *c, int l, const T *x) {
                        (const T *)B.r, CB.b, call);
 }
 
+// symmetric sweep pairs from zero on a coarsest level too large for the
+// dense inverse (mg_build): forward pairs, then as many backward ones
+constexpr int COARSE_SWEEPS = 50;
+
 template <typename T>
 static int vcycle_level(tb_ctx *c, int l) {
     const Level &L = c->lev[l];
     MGBuf<T> B = bufs<T>(c, l);
     LV<T> lv = lv_of<T>(L);
     if (l == 0) c->vc_dot_blocks[sizeof(T) == 8 ? 1 : 0] = 0;
+    if (l == c->nlev - 1 && !c->coarse_dense) {
+        const Tiles half = make_tiles((L.nx + 1) / 2, L.ny, L.nz);
+        auto sweep = [&](int color, int zero_init) {
+            hipLaunchKernelGGL(k_rbgs<T>, tiles_grid(half), tiles_block(), 0, c->s, lv, B.x, (const T *)B.b, color,
+                               zero_init, half);
+        };
+        sweep(0, 1);
+        sweep(1, 0);
+        for (int s = 1; s < COARSE_SWEEPS; s++) {
+            sweep(0, 0);
+            sweep(1, 0);
+        }
+        for (int s = 0; s < COARSE_SWEEPS; s++) {
+            sweep(1, 0);
+            sweep(0, 0);
+        }
+        TB_LAUNCH_CHECK(c);
+        return 0;
+    }
     if (l == c->small_l0[sizeof(T) == 8 ? 1 : 0]) {
         SmallCycle<T> A;
         A.l0 = l;
