@@ -515,6 +515,15 @@ struct FusedArgs {
 // every window of every quantity lies inside its line's interior span
 constexpr int EF_SHELL = 3;
 
+// a value every lane of the wave holds (j is one per wave: a 64-wide row of
+// the 64 x 4 block), moved to scalar registers, bits unchanged
+__device__ inline double wave_uniform(double v) {
+    const unsigned long long bits = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)bits);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(bits >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 template <bool RHS>
 __global__ void __launch_bounds__(EF_THREADS) k_eno_fused2(FusedArgs A, EfTiles T) {
     // [ring slot][field u, v, w, c][plane with halo]
@@ -533,6 +542,17 @@ __global__ void __launch_bounds__(EF_THREADS) k_eno_fused2(FusedArgs A, EfTiles 
     const int tx = threadIdx.x, ty = threadIdx.y, t = ty * EF_TX + tx;
     const int i = i0 + tx, j = j0 + ty;
     const bool act = i >= EF_SHELL && i < d.NX - EF_SHELL && j >= EF_SHELL && j < d.NY - EF_SHELL;
+    // weights and coordinates that do not change with k (clamped so that
+    // threads outside the box read inside it; they are never used there)
+    const int ix = i < d.NX - 2 ? i : d.NX - 2;
+    const double wvu_x = A.W[W_VU_X * A.wm + ix], wwu_x = A.W[W_WU_X * A.wm + ix];
+    const double wuv_x = A.W[W_UV_X * A.wm + ix], wuw_x = A.W[W_UW_X * A.wm + ix];
+    const double xu0 = A.xu[ix], xu1 = A.xu[ix + 1], xc0 = A.xc[ix];
+    const int jy = j < 1 ? 1 : (j > d.NY - 2 ? d.NY - 2 : j);
+    const double wvu_y = wave_uniform(A.W[W_VU_Y * A.wm + jy]), wuv_y = wave_uniform(A.W[W_UV_Y * A.wm + jy]);
+    const double wwv_y = wave_uniform(A.W[W_WV_Y * A.wm + jy]), wvw_y = wave_uniform(A.W[W_VW_Y * A.wm + jy]);
+    const double yv0 = wave_uniform(A.yv[jy]), yv1 = wave_uniform(A.yv[jy + 1]);
+    const double yc0 = wave_uniform(A.yc[jy]), ycm = wave_uniform(A.yc[jy - 1]);
 
     // staging offsets of this thread's halo points (clamped into the box;
     // a clamped point is never read by an active node)
@@ -561,6 +581,35 @@ __global__ void __launch_bounds__(EF_THREADS) k_eno_fused2(FusedArgs A, EfTiles 
             }
         }
     };
+    // the same in two halves: plane m into registers one iteration ahead,
+    // then into its ring slot, so the loads overlap the passes between
+    double pf[EF_STAGE][4];
+    auto fetch = [&](int m) {
+        const long base = (long)m * d.plane;
+#pragma unroll
+        for (int e = 0; e < EF_STAGE; e++) {
+            if (soff[e] >= 0) {
+                const long q = base + soff[e];
+                pf[e][0] = A.u[q];
+                pf[e][1] = A.v[q];
+                pf[e][2] = A.w[q];
+                pf[e][3] = A.c[q];
+            }
+        }
+    };
+    auto put = [&](int m) {
+        double *P = &S[(m + 3) % 3][0][0];
+#pragma unroll
+        for (int e = 0; e < EF_STAGE; e++) {
+            const int p = t + e * EF_THREADS;
+            if (soff[e] >= 0) {
+                P[p] = pf[e][0];
+                P[EF_PLANE + p] = pf[e][1];
+                P[2 * EF_PLANE + p] = pf[e][2];
+                P[3 * EF_PLANE + p] = pf[e][3];
+            }
+        }
+    };
     // z windows of this thread's column: k-2 and k+2 in registers, k-1 .. k+1
     // from the staged planes
     const int ic = i < d.NX - 1 ? i : d.NX - 2, jc = j < d.NY - 1 ? j : d.NY - 2;
@@ -574,6 +623,7 @@ __global__ void __launch_bounds__(EF_THREADS) k_eno_fused2(FusedArgs A, EfTiles 
     stage(k0 - 1);
     stage(k0);
     stage(k0 + 1);
+    if (k0 + 2 < d.NZ) fetch(k0 + 2);
     const int o = (ty + EF_H) * EF_SX + tx + EF_H;   // this thread's cell in a staged plane
     const double *W = A.W;
     const int wm = A.wm;
@@ -594,9 +644,9 @@ __global__ void __launch_bounds__(EF_THREADS) k_eno_fused2(FusedArgs A, EfTiles 
                 double r = 0.0;
                 const int s = A.su[id];
                 if (s != TB_BOUNDARY) {
-                    const double vu = interp4w(Vk[o - 1], vc, Vk[o + EF_SX - 1], Vk[o + EF_SX], W[W_VU_X * wm + i],
-                                               W[W_VU_Y * wm + j]);
-                    const double wu = interp4w(Wk[o - 1], wc, Wf[o - 1], Wf[o], W[W_WU_X * wm + i],
+                    const double vu = interp4w(Vk[o - 1], vc, Vk[o + EF_SX - 1], Vk[o + EF_SX], wvu_x,
+                                               wvu_y);
+                    const double wu = interp4w(Wk[o - 1], wc, Wf[o - 1], Wf[o], wwu_x,
                                                W[W_WU_Z * wm + k]);
                     r = eno_u5(Uk[o - 2], Uk[o - 1], uc, Uk[o + 1], Uk[o + 2], rx, hx, uc);
                     r += eno_u5(Uk[o - 2 * EF_SX], Uk[o - EF_SX], uc, Uk[o + EF_SX], Uk[o + 2 * EF_SX], ry, hy, vu);
@@ -610,10 +660,10 @@ __global__ void __launch_bounds__(EF_THREADS) k_eno_fused2(FusedArgs A, EfTiles 
                 double r = 0.0;
                 const int s = A.sv[id];
                 if (s != TB_BOUNDARY) {
-                    const double uv = interp4w(Uk[o - EF_SX], Uk[o - EF_SX + 1], uc, Uk[o + 1], W[W_UV_X * wm + i],
-                                               W[W_UV_Y * wm + j]);
+                    const double uv = interp4w(Uk[o - EF_SX], Uk[o - EF_SX + 1], uc, Uk[o + 1], wuv_x,
+                                               wuv_y);
                     const double wv = interp4w(Wk[o - EF_SX], Wf[o - EF_SX], wc, Wf[o], W[W_WV_Z * wm + k],
-                                               W[W_WV_Y * wm + j]);
+                                               wwv_y);
                     r = eno_u5(Vk[o - 2], Vk[o - 1], vc, Vk[o + 1], Vk[o + 2], rx, hx, uv);
                     r += eno_u5(Vk[o - 2 * EF_SX], Vk[o - EF_SX], vc, Vk[o + EF_SX], Vk[o + 2 * EF_SX], ry, hy, vc);
                     r += eno_u5(zm2[1], Vbk[o], vc, Vf[o], zp2[1], rz, hz, wv);
@@ -622,7 +672,7 @@ __global__ void __launch_bounds__(EF_THREADS) k_eno_fused2(FusedArgs A, EfTiles 
                 if constexpr (RHS) {
                     double rh = 0.0;
                     if (vel_row(s)) {
-                        const double c_term = interp2d(Ck[o - EF_SX], A.yc[j - 1], cc, A.yc[j], A.yv[j]);
+                        const double c_term = interp2d(Ck[o - EF_SX], ycm, cc, yc0, yv0);
                         rh = vc * A.a_dt - r - A.dpy[id] - A.buoy * c_term;
                     }
                     st_stream(&A.rv[id], rh);
@@ -633,10 +683,10 @@ __global__ void __launch_bounds__(EF_THREADS) k_eno_fused2(FusedArgs A, EfTiles 
                 double r = 0.0;
                 const int s = A.sw[id];
                 if (s != TB_BOUNDARY) {
-                    const double uw = interp4w(Ubk[o], Ubk[o + 1], uc, Uk[o + 1], W[W_UW_X * wm + i],
+                    const double uw = interp4w(Ubk[o], Ubk[o + 1], uc, Uk[o + 1], wuw_x,
                                                W[W_UW_Z * wm + k]);
                     const double vw = interp4w(Vbk[o], vc, Vbk[o + EF_SX], Vk[o + EF_SX], W[W_VW_Z * wm + k],
-                                               W[W_VW_Y * wm + j]);
+                                               wvw_y);
                     r = eno_u5(Wk[o - 2], Wk[o - 1], wc, Wk[o + 1], Wk[o + 2], rx, hx, uw);
                     r += eno_u5(Wk[o - 2 * EF_SX], Wk[o - EF_SX], wc, Wk[o + EF_SX], Wk[o + 2 * EF_SX], ry, hy, vw);
                     r += eno_u5(zm2[2], Wbk[o], wc, Wf[o], zp2[2], rz, hz, wc);
@@ -650,8 +700,8 @@ __global__ void __launch_bounds__(EF_THREADS) k_eno_fused2(FusedArgs A, EfTiles 
                 const int s = A.sc[id];
                 if (s != TB_BOUNDARY) {
                     // Velocity_cell_center (Velocity.c:1476-1602) at this cell
-                    const double ubc = interp2d(uc, A.xu[i], Uk[o + 1], A.xu[i + 1], A.xc[i]);
-                    double vbc = interp2d(vc, A.yv[j], Vk[o + EF_SX], A.yv[j + 1], A.yc[j]);
+                    const double ubc = interp2d(uc, xu0, Uk[o + 1], xu1, xc0);
+                    double vbc = interp2d(vc, yv0, Vk[o + EF_SX], yv1, yc0);
                     const double wbc = interp2d(wc, A.zw[k], Wf[o], A.zw[k + 1], A.zc[k]);
                     if (A.particle) vbc = particle_v(vbc, s, cc, A.hindered, A.phi_max, A.v_settle);
                     r = eno_u5(Ck[o - 2], Ck[o - 1], cc, Ck[o + 1], Ck[o + 2], rx, hx, ubc);
@@ -665,8 +715,10 @@ __global__ void __launch_bounds__(EF_THREADS) k_eno_fused2(FusedArgs A, EfTiles 
             zm2[0] = Ubk[o]; zm2[1] = Vbk[o]; zm2[2] = Wbk[o]; zm2[3] = Cbk[o];
         }
         __syncthreads();
-        // the slot of plane k - 1 is free: plane k + 2 goes there
-        if (k + 2 < d.NZ) stage(k + 2);
+        // the slot of plane k - 1 is free: plane k + 2 (fetched in the
+        // previous iteration) goes there, and plane k + 3 is fetched
+        if (k + 2 < d.NZ) put(k + 2);
+        if (k + 3 < d.NZ && k + 1 < k1) fetch(k + 3);
         if (act && k + 1 < k1) {
             const long p3 = (long)(k + 3) * d.plane + col;
             zp2[0] = A.u[p3]; zp2[1] = A.v[p3]; zp2[2] = A.w[p3]; zp2[3] = A.c[p3];

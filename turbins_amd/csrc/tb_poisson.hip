@@ -572,10 +572,10 @@ __global__ void __launch_bounds__(256) k_prolong(LV<T> F, LV<T> C, int cx, int c
 // rings of three planes of xo (halo 2) and of first-colour values (halo 1),
 // the next plane's xo and b fetched into registers one plane ahead, two
 // barriers per plane (plane m-2 is stored after the first).
-constexpr int PAIR_TX = 64, PAIR_TY = 8;
+constexpr int PAIR_TX = 64, PAIR_TY = 16;
 constexpr int PAIR_OX = PAIR_TX + 4, PAIR_OY = PAIR_TY + 4;
 constexpr int PAIR_HX = PAIR_TX + 2, PAIR_HY = PAIR_TY + 2;
-constexpr int PAIR_THREADS = 256;
+constexpr int PAIR_THREADS = 512;
 constexpr int PAIR_ORING = 3;   // the second-colour pass reads no xo, so staging may overwrite plane m-1
 constexpr int PAIR_OLD_PER = (PAIR_OY * PAIR_OX + PAIR_THREADS - 1) / PAIR_THREADS;
 constexpr int PAIR_RED = PAIR_HY * (PAIR_HX / 2);
@@ -608,7 +608,7 @@ static PairTiles make_pair_tiles(int nx, int ny, int nz, int tk) {
 // values of both colours) goes to part[blockIdx.x] -- r . z of a V-cycle
 // whose last sweep pair this is.
 template <typename T, int F, bool ZERO, bool DOT>
-__global__ void __launch_bounds__(PAIR_THREADS, sizeof(T) == 4 ? 8 : 4)
+__global__ void __launch_bounds__(PAIR_THREADS, sizeof(T) == 4 ? 4 : 2)
     k_rbgs_pair(LV<T> L, const T *__restrict__ xo, T *__restrict__ xn, const T *__restrict__ b, PairTiles P,
                 double *__restrict__ part) {
     __shared__ T so[PAIR_ORING][PAIR_OY * PAIR_OX];
