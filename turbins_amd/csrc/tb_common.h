@@ -253,13 +253,23 @@ struct Level {
 // ---------------------------------------------------------------------------
 // One concentration field (a Concentration object, Conc.c:22-217): its data,
 // convective term, right-hand side and stage-start copy, its parameters, and
-// the Gauss-Seidel pair count and contraction its last Helmholtz solve took.
+// the Gauss-Seidel history of its Helmholtz solves (HelmHistory).
+struct HelmHistory {
+    // pair count and residual contraction per pair of the last solves, by the
+    // parity of the solve count: the solves alternate between the two RK
+    // stages, whose extrapolated guesses differ in quality
+    int sweeps[2] = {0, 0};
+    double rate[2] = {0.0, 0.0};
+    unsigned nsolve = 0;
+    double *inc = nullptr;       // the last solve's increment x_new - x_start (allocated on first use)
+    bool inc_ok = false;         // inc belongs to this field's current trajectory
+};
+
 struct ConcSlot {
     double *c = nullptr, *conv = nullptr, *rhs = nullptr, *old = nullptr;
     double *dep = nullptr;       // deposited height per (i, k) column, x fastest (allocated on first use)
     tb_conc_field p{};
-    int sweeps = 0;
-    double rate = 0.0;
+    HelmHistory h;
 };
 
 struct tb_ctx {
@@ -322,8 +332,7 @@ struct tb_ctx {
     bool use_graphs = true;
     hipGraphExec_t vc_exec[2] = {nullptr, nullptr};
     int helm_method = TB_HELM_GS;    // Helmholtz solver
-    int helm_sweeps[4] = {0, 0, 0, 0};      // sweep pairs the last GS solve of u, v, w, c took
-    double helm_rate[4] = {0.0, 0.0, 0.0, 0.0};   // and its residual contraction per pair
+    HelmHistory helm[4];         // Gauss-Seidel history of u, v, w and the selected concentration field
     double ir_floor = 1e-5;      // smallest residual reduction asked of one inner FP32 solve
     int ir_maxit = 40;           // inner iterations per refinement pass
     int ir_first_its = 0;        // iterations of the last mixed solve's first pass
@@ -425,6 +434,7 @@ int tb_net_predict(tb_ctx *c, double *out, int correction_only);
 int tb_net_record_one(tb_ctx *c);
 double tb_elapsed_ms(tb_ctx *c);
 int tb_conc_use(tb_ctx *c, int n);
+int tb_helm_inc_alloc(tb_ctx *c, int q);   // the increment buffer of quantity q (once)
 void tb_free_deposits(tb_ctx *c, int first);
 int tb_ib_rows(tb_ctx *c, int q);
 
@@ -437,6 +447,7 @@ inline void tb_conc_sync(tb_ctx *c) {
     s.conv = c->f[TB_F_CONV_C];
     s.rhs = c->f[TB_F_RHS_C];
     s.old = c->old[TB_Q_C];
+    s.h = c->helm[TB_Q_C];
 }
 
 // the source of the buoyancy term of the v right-hand side
@@ -465,4 +476,5 @@ __host__ __device__ inline double particle_v(double vbc, int status, double conc
 // every entry point that writes a caller-chosen field reports it here
 inline void tb_field_written(tb_ctx *c, int f) {
     if (f == TB_F_PHI) c->phi_pinned = c->phi_is_hist0 = false;
+    if (f >= TB_F_U && f <= TB_F_C) c->helm[f - TB_F_U].inc_ok = false;   // a new trajectory
 }

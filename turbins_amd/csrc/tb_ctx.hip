@@ -66,6 +66,12 @@ static int dev_alloc(tb_ctx *c, void **p, size_t bytes) {
     return 0;
 }
 
+int tb_helm_inc_alloc(tb_ctx *c, int q) {
+    if (c->helm[q].inc) return 0;
+    c->helm[q].inc_ok = false;
+    return dev_alloc(c, (void **)&c->helm[q].inc, sizeof(double) * (size_t)c->d.ntot);
+}
+
 double tb_elapsed_ms(tb_ctx *c) {
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, c->ev0, c->ev1) != hipSuccess) {
@@ -165,6 +171,8 @@ extern "C" void tb_destroy(tb_ctx *c) {
     for (int n = 1; n < TB_NCONC_MAX; n++) {
         hipFree(c->cs[n].c); hipFree(c->cs[n].conv); hipFree(c->cs[n].rhs); hipFree(c->cs[n].old);
     }
+    for (int q = 0; q < 3; q++) hipFree(c->helm[q].inc);
+    for (int n = 0; n < TB_NCONC_MAX; n++) hipFree(c->cs[n].h.inc);   // slot conc_sel's is helm[3].inc (synced)
     hipFree(c->ctot);
     tb_free_deposits(c, 0);
     for (int f = 0; f < TB_F_COUNT; f++) hipFree(c->f[f]);
@@ -468,18 +476,14 @@ extern "C" int tb_rk_average(tb_ctx *c, double old_frac, double new_frac) {
 // at field n; the IMMERSED rows of the concentration grid are scaled by
 // ap_solid, which depends on Pe, so they are rebuilt when Pe changes.
 int tb_conc_use(tb_ctx *c, int n) {
-    tb_conc_sync(c);
-    ConcSlot &was = c->cs[c->conc_sel];
-    was.sweeps = c->helm_sweeps[TB_Q_C];
-    was.rate = c->helm_rate[TB_Q_C];
+    tb_conc_sync(c);   // the selected field's views and Gauss-Seidel history back into its slot
     const ConcSlot &s = c->cs[n];
     const bool pe_changed = c->cfg.pe != s.p.pe;
     c->f[TB_F_C] = s.c; c->f[TB_F_CONV_C] = s.conv; c->f[TB_F_RHS_C] = s.rhs; c->old[TB_Q_C] = s.old;
     c->cfg.pe = s.p.pe;
     c->cfg.conc_type = s.p.type;
     c->cfg.v_settle = s.p.v_settle;
-    c->helm_sweeps[TB_Q_C] = s.sweeps;
-    c->helm_rate[TB_Q_C] = s.rate;
+    c->helm[TB_Q_C] = s.h;
     c->conc_sel = n;
     if (pe_changed && c->nib[TB_Q_C] > 0) return tb_ib_rows(c, TB_Q_C);
     return 0;
@@ -518,6 +522,7 @@ extern "C" int tb_set_conc_fields(tb_ctx *c, int nconc, const tb_conc_field *fie
             }
         } else {
             dev_release(c, s.c); dev_release(c, s.conv); dev_release(c, s.rhs); dev_release(c, s.old);
+            dev_release(c, s.h.inc);
         }
     }
     tb_free_deposits(c, nconc);
@@ -528,11 +533,11 @@ extern "C" int tb_set_conc_fields(tb_ctx *c, int nconc, const tb_conc_field *fie
     }
     for (int n = 0; n < nconc; n++) {
         c->cs[n].p = fields[n];
-        c->cs[n].sweeps = 0;
-        c->cs[n].rate = 0.0;
+        double *inc = c->cs[n].h.inc;   // kept, its content no longer a guess
+        c->cs[n].h = HelmHistory();
+        c->cs[n].h.inc = inc;
     }
-    c->helm_sweeps[TB_Q_C] = 0;
-    c->helm_rate[TB_Q_C] = 0.0;
+    c->helm[TB_Q_C] = c->cs[0].h;
     c->nconc = nconc;
     c->hindered = hindered != 0;
     if (tb_conc_use(c, 0)) return -1;   // field 0's (possibly new) parameters

@@ -634,5 +634,8 @@ int tb_geometry_run(tb_ctx *c, const double *h_xz) {
 
 extern "C" int tb_setup_geometry(tb_ctx *c, const double *h_xz) {
     if (!c) return -1;
+    // the increments of earlier Helmholtz solves belong to another operator
+    for (int q = 0; q < 4; q++) c->helm[q].inc_ok = false;
+    for (int n = 0; n < TB_NCONC_MAX; n++) c->cs[n].h.inc_ok = false;
     return tb_geometry_run(c, h_xz);
 }

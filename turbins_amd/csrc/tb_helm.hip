@@ -41,11 +41,14 @@ __device__ inline double wallmod(double ap, double a, int rule) {
 // MODE 0: y = A x ; MODE 1: also partial w.y ; MODE 2: also partials x.y, y.y
 // MODE 3: residual only: partials |w - A x|^2 and |w|^2 with w = b, nothing
 // stored except that IMMERSED rows must hold (A x) from k_ib_apply in y.
+// MODE 4: MODE 3 and the increment dn = x - x0 over the box (the Gauss-Seidel
+// solve's change to the field it started from).
 // IMMERSED rows are produced by k_ib_apply, which runs first; here their y is
 // only read for the dot products.
 template <int MODE>
 __global__ void __launch_bounds__(256) k_helm_km(HelmOp H, const double *__restrict__ x, double *__restrict__ y,
-                                                 const double *__restrict__ w, double *part, KTiles t) {
+                                                 const double *__restrict__ w, double *part, KTiles t,
+                                                 const double *__restrict__ x0, double *__restrict__ dn) {
     __shared__ double sh[32];
     double acc[2] = {0.0, 0.0};
     int i, j, k0, k1;
@@ -116,7 +119,7 @@ __global__ void __launch_bounds__(256) k_helm_km(HelmOp H, const double *__restr
         for (int k = k0; k < k1; k++) {
             double xp = x[id + ((k + 1 < NZ) ? d.plane : 0)];
             const double xs = x[id - os], xw = x[id - ow], xe = x[id + oe], xn = x[id + on];
-            const double wv = (MODE == 1 || MODE == 3) ? __builtin_nontemporal_load(&w[id]) : 0.0;
+            const double wv = (MODE == 1 || MODE >= 3) ? __builtin_nontemporal_load(&w[id]) : 0.0;
             const int s = H.st[id];
             // the fluid row is formed for every node and selected afterwards:
             // no status-dependent branch sits between the loads and the math
@@ -149,11 +152,12 @@ __global__ void __launch_bounds__(256) k_helm_km(HelmOp H, const double *__restr
             double yv = fluid ? r : diag_other * xc;
             if (s == TB_IMMERSED) {
                 yv = (MODE > 0) ? y[id] : 0.0;   // row written by k_ib_apply
-            } else if (MODE != 3) {
+            } else if (MODE < 3) {
                 st_stream(&y[id], yv);
             }
+            if (MODE == 4) st_stream(&dn[id], xc - x0[id]);
             if (MODE == 1) acc[0] += wv * yv;
-            if (MODE == 3) { acc[0] += (wv - yv) * (wv - yv); acc[1] += wv * wv; }
+            if (MODE >= 3) { acc[0] += (wv - yv) * (wv - yv); acc[1] += wv * wv; }
             if (MODE == 2) { acc[0] += xc * yv; acc[1] += yv * yv; }
             xm = xc;
             xc = xp;
@@ -211,7 +215,7 @@ static HelmOp make_op(tb_ctx *c, int q, double dt) {
 // y = A x (scaled: IB rows rescaled for the Krylov solve); mode 1/2/3 leave
 // partial sums in c->red.  Returns the number of partials, or -1.
 static int helm_apply(tb_ctx *c, const HelmOp &H, const double *x, double *y, bool scaled, int mode,
-                      const double *w = nullptr) {
+                      const double *w = nullptr, const double *x0 = nullptr, double *dn = nullptr) {
     int q = H.q;
     if (c->nib[q] > 0)
         hipLaunchKernelGGL(k_ib_apply, dim3((c->nib[q] + 255) / 256), dim3(256), 0, c->s, c->ib_row[q], c->ib_cnt[q],
@@ -219,13 +223,15 @@ static int helm_apply(tb_ctx *c, const HelmOp &H, const double *x, double *y, bo
     KTiles t = make_ktiles(c->d.NX, c->d.NY, c->d.NZ);
     if (c->prof_on) tb_prof_begin(c, TB_K_HELM_OP);
     if (mode == 0)
-        hipLaunchKernelGGL(k_helm_km<0>, ktiles_grid(t), tiles_block(), 0, c->s, H, x, y, w, c->red, t);
+        hipLaunchKernelGGL(k_helm_km<0>, ktiles_grid(t), tiles_block(), 0, c->s, H, x, y, w, c->red, t, x0, dn);
     else if (mode == 1)
-        hipLaunchKernelGGL(k_helm_km<1>, ktiles_grid(t), tiles_block(), 0, c->s, H, x, y, w, c->red, t);
+        hipLaunchKernelGGL(k_helm_km<1>, ktiles_grid(t), tiles_block(), 0, c->s, H, x, y, w, c->red, t, x0, dn);
     else if (mode == 2)
-        hipLaunchKernelGGL(k_helm_km<2>, ktiles_grid(t), tiles_block(), 0, c->s, H, x, y, w, c->red, t);
+        hipLaunchKernelGGL(k_helm_km<2>, ktiles_grid(t), tiles_block(), 0, c->s, H, x, y, w, c->red, t, x0, dn);
+    else if (mode == 3)
+        hipLaunchKernelGGL(k_helm_km<3>, ktiles_grid(t), tiles_block(), 0, c->s, H, x, y, w, c->red, t, x0, dn);
     else
-        hipLaunchKernelGGL(k_helm_km<3>, ktiles_grid(t), tiles_block(), 0, c->s, H, x, y, w, c->red, t);
+        hipLaunchKernelGGL(k_helm_km<4>, ktiles_grid(t), tiles_block(), 0, c->s, H, x, y, w, c->red, t, x0, dn);
     if (c->prof_on) tb_prof_end(c, TB_K_HELM_OP);
     TB_LAUNCH_CHECK(c);
     return (int)(t.per_xcd * NXCD);
@@ -386,10 +392,13 @@ static GsTiles make_gs_tiles(const Dims &d) {
 // neighbour index leaves the box either has a zero coefficient there or is
 // not a FLUID row (the last plane of each axis is BOUNDARY), and the LDS
 // positions outside the box hold zeros or finite values of other planes.
-// Element offsets are 32-bit (the host checks the field size).
+// Element offsets are 32-bit (the host checks the field size).  ADD: the
+// pair starts from xo + dd (the field plus the last solve's increment, an
+// extrapolated initial guess) without forming it in memory.
+template <bool ADD>
 __global__ void __launch_bounds__(GS_THREADS) k_helm_gs_pair(HelmOp H, const double *__restrict__ xo,
                                                              double *__restrict__ xn, const double *__restrict__ b,
-                                                             GsTiles T) {
+                                                             GsTiles T, const double *__restrict__ dd) {
     __shared__ double so[GS_ORING][GS_OY * GS_OX];
     __shared__ double sr[3][GS_HY * GS_HX];
     __shared__ AxisGS cx[GS_OX], cy[GS_OY], cz[GS_OZ];
@@ -432,17 +441,22 @@ __global__ void __launch_bounds__(GS_THREADS) k_helm_gs_pair(HelmOp H, const dou
         const int i = i0 - 2 + cc, j = j0 - 2 + r;
         ooff[u] = (e < GS_OY * GS_OX && i >= 0 && i < NX && j >= 0 && j < NY) ? j * px + i : -1;
     }
-    double ro[GS_OLD_PER];
+    // the loads of a plane stay in flight until put_old (with ADD the sum is
+    // formed there too, not between the loads)
+    double ro[GS_OLD_PER], rd[GS_OLD_PER];
     auto fetch_old = [&](int m) {
 #pragma unroll
-        for (int u = 0; u < GS_OLD_PER; u++) ro[u] = ooff[u] >= 0 ? xo[m * plane + ooff[u]] : 0.0;
+        for (int u = 0; u < GS_OLD_PER; u++) {
+            ro[u] = ooff[u] >= 0 ? xo[m * plane + ooff[u]] : 0.0;
+            if (ADD) rd[u] = ooff[u] >= 0 ? dd[m * plane + ooff[u]] : 0.0;
+        }
     };
     auto put_old = [&](int m) {
         double *S = so[m % GS_ORING];
 #pragma unroll
         for (int u = 0; u < GS_OLD_PER; u++) {
             const int e = t + u * GS_THREADS;
-            if (e < GS_OY * GS_OX) S[e] = ro[u];
+            if (e < GS_OY * GS_OX) S[e] = ADD ? ro[u] + rd[u] : ro[u];
         }
     };
     // this thread's red points of the halo-1 plane: row, column pair, y
@@ -613,9 +627,10 @@ __global__ void k_ib_scatter(const long *__restrict__ row, int n, const double *
 }
 
 // ||b - A x||^2 and ||b||^2 into S_TMP1, S_TMP2 (host mirror)
+// |b - A x|^2 and |b|^2; with x0 given also the increment x - x0 into dn
 static int helm_residual(tb_ctx *c, const HelmOp &H, const double *x, const double *b, double *ibuf, double &rr,
-                         double &bb) {
-    int nb = helm_apply(c, H, x, ibuf, false, 3, b);
+                         double &bb, const double *x0 = nullptr, double *dn = nullptr) {
+    int nb = helm_apply(c, H, x, ibuf, false, x0 ? 4 : 3, b, x0, dn);
     if (nb < 0 || tb_reduce_finish(c, nb, 2, S_TMP1)) return -1;
     if (tb_read_scalars(c, S_TMP1, 2)) return -1;
     rr = c->hscal[S_TMP1];
@@ -625,8 +640,12 @@ static int helm_residual(tb_ctx *c, const HelmOp &H, const double *x, const doub
 
 // n sweep pairs, each followed by the Jacobi update of the IMMERSED rows,
 // alternating between the buffers cur and alt (the same size and pitch);
-// cur is swapped to the buffer that holds the result
-static int gs_pairs(tb_ctx *c, const HelmOp &H, double *&cur, double *&alt, const double *b, int n) {
+// cur is swapped to the buffer that holds the result.  With src given the
+// first pair starts from src (plus inc, when given) instead of cur; inc may
+// be cur itself: the last planes written into cur first are read there only
+// with zero coefficients (no row couples to them).
+static int gs_pairs(tb_ctx *c, const HelmOp &H, double *&cur, double *&alt, const double *b, int n,
+                    const double *src = nullptr, const double *inc = nullptr) {
     const int q = H.q;
     double *tmp = c->kv[4];
     const GsTiles T = make_gs_tiles(c->d);
@@ -641,9 +660,16 @@ static int gs_pairs(tb_ctx *c, const HelmOp &H, double *&cur, double *&alt, cons
         TB_LAUNCH_CHECK(c);
     }
     for (int sw = 0; sw < n; sw++) {
-        TB_PROF(c, TB_K_HELM_GS,
-                hipLaunchKernelGGL(k_helm_gs_pair, dim3(T.per_xcd * NXCD), dim3(GS_THREADS), 0, c->s, H,
-                                   (const double *)cur, alt, b, T));
+        const double *from = (sw == 0 && src) ? src : cur;
+        if (sw == 0 && src && inc) {
+            TB_PROF(c, TB_K_HELM_GS,
+                    hipLaunchKernelGGL(k_helm_gs_pair<true>, dim3(T.per_xcd * NXCD), dim3(GS_THREADS), 0, c->s, H,
+                                       from, alt, b, T, inc));
+        } else {
+            TB_PROF(c, TB_K_HELM_GS,
+                    hipLaunchKernelGGL(k_helm_gs_pair<false>, dim3(T.per_xcd * NXCD), dim3(GS_THREADS), 0, c->s, H,
+                                       from, alt, b, T, (const double *)nullptr));
+        }
         double *w = cur;
         cur = alt;
         alt = w;
@@ -668,11 +694,20 @@ static int copy_box(tb_ctx *c, double *dst, const double *src) {
 }
 
 // returns 0 and sets ok when converged (the field of quantity q then holds
-// the solution, possibly in the buffer that was the scratch vector); ok =
-// false when the sweeps stall (contraction per pair worse than 0.5) or run
-// out of sweeps, x then holds the last iterate.  The true residual costs about two thirds of a
-// sweep pair, so it is evaluated only where the solve may have converged:
-// first after as many pairs as the last solve of the same quantity needed
+// the solution); ok = false when the sweeps stall (contraction per pair worse
+// than 0.5) or run out of sweeps, x then holds the last iterate.  The initial
+// guess is the field plus the increment of the quantity's last solve, when
+// that solve converged on the same trajectory (the reference starts from the
+// field alone; both stop at the same residual test).  Three buffers take
+// part: the field x, read by the first pair and kept so that each residual
+// check also writes the increment, and the two iterate buffers, kv[6] and the
+// increment buffer, which the first pair may read from as it writes kv[6].
+// On convergence the field takes the result's buffer, the increment buffer
+// the new increment's and kv[6] the old field's (no kernel or graph keeps
+// any of these pointers between calls).
+// The true residual costs about two thirds of a sweep pair, so it is
+// evaluated only where the solve may have converged: first after as many
+// pairs as the last solve of the same quantity and RK stage parity needed
 // (the count is steady from step to step), then after the pairs predicted
 // from the observed contraction (at most 3 at a time: the contraction
 // improves as the sweeps go, a longer prediction overshoots).  The initial
@@ -681,19 +716,28 @@ static int helm_gs_solve(tb_ctx *c, const HelmOp &H, double *x, const double *b,
                          double &res0, double &res, double &bn, bool &ok) {
     const int q = H.q;
     double *ibuf = c->kv[5];
-    double *cur = x, *alt = c->kv[6];
     double rr, bb;
     res0 = NAN;
     res = NAN;
     its = 0;
     ok = false;
-    int chunk = c->helm_sweeps[q] > 0 ? c->helm_sweeps[q] : 2;
-    double prev = NAN, rate = c->helm_rate[q];
+    if (maxit < 1) return 0;
+    if (tb_helm_inc_alloc(c, q)) return -1;
+    HelmHistory &h = c->helm[q];
+    const bool field = x == c->f[TB_F_U + q];
+    const double *inc = (field && h.inc_ok) ? h.inc : nullptr;
+    h.inc_ok = false;   // the increment buffer is overwritten from here on
+    double *cur = h.inc, *alt = c->kv[6];
+    const int par = (int)(h.nsolve++ & 1u);
+    int chunk = h.sweeps[par] > 0 ? h.sweeps[par] : 2;
+    double prev = NAN, rate = h.rate[par];
+    const double *src = x;
     while (!ok && its < maxit) {
         const int m = chunk < maxit - its ? chunk : maxit - its;
-        if (gs_pairs(c, H, cur, alt, b, m)) return -1;
+        if (gs_pairs(c, H, cur, alt, b, m, src, inc)) return -1;
+        src = inc = nullptr;
         its += m;
-        if (helm_residual(c, H, cur, b, ibuf, rr, bb)) return -1;
+        if (helm_residual(c, H, cur, b, ibuf, rr, bb, field ? x : nullptr, alt)) return -1;
         bn = sqrt(bb);
         if (bn == 0.0) bn = 1.0;
         res = sqrt(rr) / bn;
@@ -711,18 +755,18 @@ static int helm_gs_solve(tb_ctx *c, const HelmOp &H, double *x, const double *b,
         // one pair fewer next time when this residual shows it would have
         // sufficed at the last contraction
         const double r1 = (rate > 0.0 && rate < 0.5) ? rate : 0.05;
-        c->helm_sweeps[q] = (its > 1 && res / r1 <= rtol) ? its - 1 : its;
-        if (prev == prev) c->helm_rate[q] = rate;
+        h.sweeps[par] = (its > 1 && res / r1 <= rtol) ? its - 1 : its;
+        if (prev == prev) h.rate[par] = rate;
     }
-    if (cur != x) {
-        if (ok && x == c->f[TB_F_U + q] && alt == x) {
-            // the field and the scratch vector trade buffers (no kernel or
-            // graph keeps either pointer between calls)
-            c->f[TB_F_U + q] = cur;
-            c->kv[6] = x;
-        } else if (copy_box(c, x, cur)) {
-            return -1;   // BiCGStab continues from x
-        }
+    if (ok && field) {
+        // cur: the solution, alt: its increment over x (the iterates never
+        // write x, which is now free)
+        c->f[TB_F_U + q] = cur;
+        h.inc = alt;
+        c->kv[6] = x;
+        h.inc_ok = true;
+    } else if (copy_box(c, x, cur)) {
+        return -1;   // BiCGStab continues from x
     }
     return 0;
 }
@@ -893,6 +937,7 @@ extern "C" int tb_helmholtz_solve(tb_ctx *c, int q, double dt, tb_solve_stats *s
         // sweeps stalled: BiCGStab continues from the current x
         handover = 1;
     }
+    c->helm[q].inc_ok = false;   // the increment is recorded by the Gauss-Seidel solve only
     if (helm_apply(c, H, x, t, true, 0) < 0) return -1;
     hipLaunchKernelGGL(k_bicg_init, dim3(VB), dim3(VT), 0, c->s, n, b, t, r, rh, p, v, c->red);
     if (tb_reduce_finish(c, VB, 2, S_RR)) return -1;   // S_RR = r.r, S_RR+1 = b.b
