@@ -464,6 +464,57 @@ int tb_deposit_set(tb_ctx *ctx, const double *h);
  * plus weight times the column integral of the suspended c dy */
 int tb_deposit_dumped(tb_ctx *ctx, double weight, double *h);
 
+/* ---- open x boundaries (Inflow.c, Outflow.c) ----
+ * A closed box by default.  With inflow the x = 0 face is open: u there
+ * takes the inflow plane (u->G_inflow) and every concentration field its own
+ * (c->G_inflow).  With outflow the x = Lx face is open: u, v, w and every
+ * concentration field take there the planes of the convective boundary
+ * condition (G_outflow).  The planes enter the x-line ENO ghosts
+ * (Velocity.c:2193-2232, 2442-2444, 2669-2671; Conc.c:1099-1122), the
+ * transposes (Velocity.c:3491-3505, 3589-3603), the cell-centre velocities
+ * (Velocity.c:1544-1554), the right-hand sides (Velocity.c:1691-1721,
+ * 1885-1897, 2011-2023; Conc.c:938-963), the divergence (Pressure.c:242-249,
+ * 541-548) and the viscous dissipation (Extract.c:577-645); the open faces
+ * drop their wall rule from the Helmholtz rows (Velocity.c:989, 1336;
+ * Conc.c:582-607).  A plane holds NY*NZ doubles indexed [k][j], j fastest,
+ * over the padded point counts NY = ny+1, NZ = nz+1. */
+typedef struct tb_open_bc {
+    int32_t inflow;             /* params->inflow (Input.c:371) */
+    int32_t outflow;            /* params->outflow (Input.c:372) */
+    int32_t transient_inflow;   /* params->transient_inflow (Input.c:446) */
+    int32_t reserved;
+    double u_conv;              /* params->U, the convective outflow speed (Input.c:424) */
+    double time_c;              /* params->time_c (Input.c:447) */
+} tb_open_bc;
+enum { TB_PLANE_INFLOW = 0, TB_PLANE_OUTFLOW = 1 };
+/* Opens or closes the x faces; the planes start at zero (calloc'd as
+ * Memory_allocate_2D_double does), the influx at zero. */
+int tb_set_open_boundaries(tb_ctx *ctx, const tb_open_bc *bc);
+/* Inflow_u_velocity_profile (Inflow.c:13-158) and Inflow_conc_profile
+ * (Inflow.c:161-260) at time t: 1 times the transient factor
+ * 1 - exp(-(t + 0.01)/time_c) on the FLUID u nodes of i = 1 and on the
+ * non-SOLID cells of i = 0 of every concentration field (other entries keep
+ * their values); *influx = sum of u_in dy dz over those u nodes
+ * (u->W_influx), or null */
+int tb_inflow_profile(tb_ctx *ctx, double time, double *influx);
+/* A plane of quantity q (TB_Q_U .. TB_Q_C, the selected concentration
+ * field for TB_Q_C; inflow planes exist for u and c only).  Setting the u
+ * inflow plane recomputes the influx as tb_inflow_profile does. */
+int tb_set_boundary_plane(tb_ctx *ctx, int which, int q, const double *plane);
+int tb_get_boundary_plane(tb_ctx *ctx, int which, int q, double *plane);
+int tb_get_influx(tb_ctx *ctx, double *influx);
+/* Outflow_impose_convective_boundary (Outflow.c:12-101, 104-159): every
+ * outflow plane of u, v, w and each concentration field advances as
+ * out = (1 - c) out + c q[NI-1] with c = dt U / dx at the rows whose node
+ * at i = NI-1 is FLUID or IMMERSED; then the u plane takes the correction
+ * (influx - outflux) / area over those rows so that its outflux equals the
+ * influx (Outflow_update_u_velocity_to_conserve_mass, Outflow.c:163-304).
+ * tb_step calls it at the start of each stage; tb_store_old and
+ * tb_rk_average carry the outflow planes with the fields
+ * (Velocity_store_old_outflow / Velocity_rk_average_outflow,
+ * Velocity.c:2058-2120; Conc.c:1223-1289). */
+int tb_outflow_convective(tb_ctx *ctx, double dt);
+
 /* composite SSP-RK2 step: the per-step sequence of the unpublished driver
  * (README.md:26), reconstructed from the TVD_RK2 scheme (definitions.h:148-149);
  * every concentration field is stepped */

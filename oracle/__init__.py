@@ -86,6 +86,13 @@ def lib():
         L.ora_deposit_dumped.argtypes = [P, C.c_double, P]
         L.ora_wall_shear.argtypes = [P, C.c_int, P, P, P, P]
         L.ora_hindered_speed.argtypes = [C.c_double, C.c_double, C.c_double]
+        L.ora_set_open_boundaries.argtypes = [P, C.POINTER(abi.OpenBC)]
+        L.ora_inflow_profile.argtypes = [P, C.c_double, C.POINTER(C.c_double)]
+        L.ora_set_boundary_plane.argtypes = [P, C.c_int, C.c_int, P]
+        L.ora_get_boundary_plane.argtypes = [P, C.c_int, C.c_int, P]
+        L.ora_get_influx.restype = C.c_double
+        L.ora_get_influx.argtypes = [P]
+        L.ora_outflow_convective.argtypes = [P, C.c_double]
         L.ora_gmg_levels.argtypes = [P, P, C.c_int]
         L.ora_helmholtz_matrix.argtypes = [P, C.c_int, C.c_double, P, P, P]
         L.ora_poisson_matrix.argtypes = [P, P, P, P]
@@ -251,6 +258,31 @@ class Oracle:
 
     def select_conc(self, n):
         assert self.L.ora_select_conc(self.h, n) == 0
+
+    def set_open_boundaries(self, **kw):
+        bc = abi.open_bc(**kw)
+        assert self.L.ora_set_open_boundaries(self.h, C.byref(bc)) == 0
+
+    def inflow_profile(self, time):
+        fl = C.c_double()
+        assert self.L.ora_inflow_profile(self.h, time, C.byref(fl)) == 0
+        return fl.value
+
+    def set_plane(self, which, q, plane):
+        a = np.ascontiguousarray(plane, dtype=np.float64).reshape(-1)
+        assert a.size == (self.cfg.ny + 1) * (self.cfg.nz + 1)
+        assert self.L.ora_set_boundary_plane(self.h, which, q, _ptr(a)) == 0
+
+    def get_plane(self, which, q):
+        out = np.zeros((self.cfg.nz + 1) * (self.cfg.ny + 1))
+        assert self.L.ora_get_boundary_plane(self.h, which, q, _ptr(out)) == 0
+        return out
+
+    def influx(self):
+        return self.L.ora_get_influx(self.h)
+
+    def outflow_convective(self, dt):
+        assert self.L.ora_outflow_convective(self.h, dt) == 0
 
     def total_concentration(self):
         return self.L.ora_total_concentration(self.h)

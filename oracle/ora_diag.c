@@ -26,9 +26,10 @@ int ora_interface_y_index(const ora_ctx *o, int i, int k) {
  * choice into the ghost factor: no-slip -1, free-slip +1 */
 static double ghost_factor(const ora_ctx *o, int wall) { return o->cfg.wall_noslip[wall] ? -1.0 : 1.0; }
 
-/* Extract_viscous_dissipation_rate (Extract.c:453-725), without inflow or
- * outflow: (1/Re) sum over FLUID cells of u lap(u) dV_u + v lap(v) dV_v +
- * w lap(w) dV_w, the Laplacians with the wall ghosts u_ghost = k u */
+/* Extract_viscous_dissipation_rate (Extract.c:453-725): (1/Re) sum over
+ * FLUID cells of u lap(u) dV_u + v lap(v) dV_v + w lap(w) dV_w, the
+ * Laplacians with the wall ghosts u_ghost = k u; an open x face gives the
+ * neighbours there from its planes (Extract.c:577-590, 630-645) */
 double ora_viscous_dissipation(ora_ctx *o) {
     const double *u = o->f[TB_F_U], *v = o->f[TB_F_V], *w = o->f[TB_F_W];
     const double k_left = ghost_factor(o, 0), k_right = ghost_factor(o, 1);
@@ -38,6 +39,9 @@ double ora_viscous_dissipation(ora_ctx *o) {
     const int NI = o->NX - 1, NJ = o->NY - 1, NK = o->NZ - 1;
     const size_t pl = (size_t)o->NX * o->NY, rx = (size_t)o->NX;
     const double *mxu = o->mxu, *mxc = o->mxc, *myv = o->myv, *myc = o->myc, *mzw = o->mzw, *mzc = o->mzc;
+    const double *uin = ora_in_plane(o, TB_Q_U);
+    const double *uout = ora_out_plane(o, TB_Q_U), *vout = ora_out_plane(o, TB_Q_V),
+                 *wout = ora_out_plane(o, TB_Q_W);
     double rate = 0.0;
     for (int k = 0; k < o->NZ; k++)
         for (int j = 0; j < o->NY; j++)
@@ -56,7 +60,8 @@ double ora_viscous_dissipation(ora_ctx *o) {
                 double U_w, V_w, W_w, U_e, V_e, W_e, U_s, V_s, W_s, U_n, V_n, W_n, U_b, V_b, W_b, U_f, V_f, W_f;
                 if (i == 0) {
                     U_aw *= mxc[i];
-                    U_w = 0.0; V_w = k_left * v[id]; W_w = k_left * w[id];
+                    U_w = uin ? uin[ora_pidx(o, j, k)] : 0.0;
+                    V_w = k_left * v[id]; W_w = k_left * w[id];
                 } else {
                     U_aw *= mxc[i - 1];
                     U_w = u[id - 1]; V_w = v[id - 1]; W_w = w[id - 1];
@@ -76,7 +81,12 @@ double ora_viscous_dissipation(ora_ctx *o) {
                     U_b = u[id - pl]; V_b = v[id - pl]; W_b = w[id - pl];
                 }
                 if (i == NI - 1) {
-                    U_e = 0.0; V_e = k_right * v[id]; W_e = k_right * w[id];
+                    if (uout) {
+                        size_t p = ora_pidx(o, j, k);
+                        U_e = uout[p]; V_e = vout[p]; W_e = wout[p];
+                    } else {
+                        U_e = 0.0; V_e = k_right * v[id]; W_e = k_right * w[id];
+                    }
                 } else {
                     U_e = u[id + 1]; V_e = v[id + 1]; W_e = w[id + 1];
                 }

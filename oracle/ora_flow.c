@@ -51,6 +51,7 @@ void ora_destroy(ora_ctx *o) {
     free(o->tr1); free(o->tr2); free(o->ubc); free(o->vbc); free(o->wbc);
     ora_mat_free(&o->pois);
     ora_gmg_free(o);
+    ora_open_free(o);
     free(o);
 }
 
@@ -136,25 +137,32 @@ static void ghost_fill(int order, int N, double *D0, double *P, int kind, double
     }
 }
 
-/* Velocity_cell_center (Velocity.c:1476-1602) */
+/* Velocity_cell_center (Velocity.c:1476-1602); an open x face gives u
+ * there from its plane (Velocity.c:1544-1558) */
 static void cell_center(ora_ctx *o) {
     const double *u = o->f[TB_F_U], *v = o->f[TB_F_V], *w = o->f[TB_F_W];
+    const double *uin = ora_in_plane(o, TB_Q_U), *uout = ora_out_plane(o, TB_Q_U);
     int NI = o->NX - 1, NJ = o->NY - 1, NK = o->NZ - 1;
 #pragma omp parallel for schedule(static)
     for (int k = 0; k < NK; k++)
         for (int j = 0; j < NJ; j++)
             for (int i = 0; i < NI; i++) {
                 size_t id = ora_idx(o, i, j, k);
-                o->ubc[id] = ora_interp2(u[id], o->xu[i], u[id + 1], o->xu[i + 1], o->xc[i]);
+                double u0 = (uin && i == 0) ? uin[ora_pidx(o, j, k)] : u[id];
+                double u1 = (uout && i == NI - 1) ? uout[ora_pidx(o, j, k)] : u[id + 1];
+                o->ubc[id] = ora_interp2(u0, o->xu[i], u1, o->xu[i + 1], o->xc[i]);
                 o->vbc[id] = ora_interp2(v[id], o->yv[j], v[id + o->NX], o->yv[j + 1], o->yc[j]);
                 size_t pl = (size_t)o->NX * o->NY;
                 o->wbc[id] = ora_interp2(w[id], o->zw[k], w[id + pl], o->zw[k + 1], o->zc[k]);
             }
 }
 
-/* Velocity_transpose_velocities (Velocity.c:3341-3663) */
+/* Velocity_transpose_velocities (Velocity.c:3341-3663); u at the v and w
+ * nodes next to an open x face comes from its plane (Velocity.c:3491-3505,
+ * 3589-3603) */
 static void transpose(ora_ctx *o, int q) {
     const double *u = o->f[TB_F_U], *v = o->f[TB_F_V], *w = o->f[TB_F_W];
+    const double *uin = ora_in_plane(o, TB_Q_U), *uout = ora_out_plane(o, TB_Q_U);
     double *t1 = o->tr1, *t2 = o->tr2;
     int NX = o->NX, NY = o->NY, NZ = o->NZ;
     size_t pl = (size_t)NX * NY;
@@ -177,6 +185,8 @@ static void transpose(ora_ctx *o, int q) {
                 } else if (q == TB_Q_V) {
                     qq[0] = u[id - NX]; qq[2] = u[id];
                     qq[1] = u[id - NX + 1]; qq[3] = u[id + 1];
+                    if (uin && i == 0) { qq[0] = uin[ora_pidx(o, j - 1, k)]; qq[2] = uin[ora_pidx(o, j, k)]; }
+                    if (uout && i == NX - 2) { qq[1] = uout[ora_pidx(o, j - 1, k)]; qq[3] = uout[ora_pidx(o, j, k)]; }
                     X1[0] = o->xu[i]; X1[1] = o->xu[i + 1]; X1[2] = o->xu[i]; X1[3] = o->xu[i + 1];
                     X2[0] = o->yu[j - 1]; X2[1] = o->yu[j - 1]; X2[2] = o->yu[j]; X2[3] = o->yu[j];
                     t1[id] = ora_interp4(qq, X1, X2, o->xv[i], o->yv[j]);
@@ -187,6 +197,8 @@ static void transpose(ora_ctx *o, int q) {
                 } else {
                     qq[0] = u[id - pl]; qq[2] = u[id];
                     qq[1] = u[id - pl + 1]; qq[3] = u[id + 1];
+                    if (uin && i == 0) { qq[0] = uin[ora_pidx(o, j, k - 1)]; qq[2] = uin[ora_pidx(o, j, k)]; }
+                    if (uout && i == NX - 2) { qq[1] = uout[ora_pidx(o, j, k - 1)]; qq[3] = uout[ora_pidx(o, j, k)]; }
                     X1[0] = o->xu[i]; X1[1] = o->xu[i + 1]; X1[2] = o->xu[i]; X1[3] = o->xu[i + 1];
                     X2[0] = o->zu[k - 1]; X2[1] = o->zu[k - 1]; X2[2] = o->zu[k]; X2[3] = o->zu[k];
                     t1[id] = ora_interp4(qq, X1, X2, o->xw[i], o->zw[k]);
@@ -229,6 +241,14 @@ static void eno_direction(ora_ctx *o, int q, int d, const double *adv, double *c
     int N = n + 2 * order;
     size_t stride = (d == 0) ? 1 : (d == 1) ? (size_t)o->NX : (size_t)o->NX * o->NY;
     int a0 = (d == 0) ? 1 : 0, a1 = (d == 2) ? 1 : 2;   /* the two other axes */
+    /* open x faces: every ghost of that side takes the plane value
+     * (u: Velocity.c:2193-2200, 2227-2233; v, w: outflow only, 2442-2444,
+     * 2669-2671; c: Conc.c:1099-1101, 1120-1122) */
+    const double *gin = NULL, *gout = NULL;
+    if (d == 0) {
+        if (q == TB_Q_U || q == TB_Q_C) gin = ora_in_plane(o, q);
+        gout = ora_out_plane(o, q);
+    }
 #pragma omp parallel
     {
         double *D0 = dalloc((size_t)N), *P = dalloc((size_t)N), *V = dalloc((size_t)N), *out = dalloc((size_t)N);
@@ -246,6 +266,10 @@ static void eno_direction(ora_ctx *o, int q, int d, const double *adv, double *c
                 }
                 ghost_fill(order, N, D0, P, kind, wall_lo, wall_hi, s_lo, s_hi, u_x_quirk, hi_mode,
                            hi_scale);
+                if (gin)
+                    for (int ex = 0; ex < order; ex++) D0[ex] = gin[ora_pidx(o, a, b)];
+                if (gout)
+                    for (int ex = 0; ex < order; ex++) D0[N - ex - 1] = gout[ora_pidx(o, a, b)];
                 ora_eno_line(order, N, D0, P, V, out);
                 for (int m = 0; m < n; m++) {
                     size_t id = base + (size_t)m * stride;
@@ -304,6 +328,12 @@ int ora_eno_conv(ora_ctx *o, int q) {
 
 int ora_set_rhs(ora_ctx *o, int q, double dt) {
     if (q < 0 || q > 3) return -1;
+    /* open x faces: the plane value times the dropped coefficient
+     * (u: Velocity.c:1691-1721; v: 1885-1897; w: 2011-2023; c: Conc.c:938-963) */
+    const double *pin = (q == TB_Q_U || q == TB_Q_C) ? ora_in_plane(o, q) : NULL;
+    const double *pout = ora_out_plane(o, q);
+    double kappa = (q == TB_Q_C) ? o->cfg.pe : o->cfg.re;
+    double Cx = -1.0 / (kappa * o->dEta * o->dEta);
     const double *data = o->f[TB_F_U + q], *conv = o->f[TB_F_CONV_U + q];
     const double *dp = (q < 3) ? o->f[TB_F_DPDX + q] : NULL;
     /* Velocity.c:1798-1815: c_total with several fields, else c[0] */
@@ -319,12 +349,40 @@ int ora_set_rhs(ora_ctx *o, int q, double dt) {
                 int s = o->st[q][id];
                 double r = 0.0;
                 if (q == TB_Q_C) {
-                    if (s == TB_FLUID) r = data[id] * a_dt - conv[id];
+                    if (s == TB_FLUID) {
+                        r = data[id] * a_dt - conv[id];
+                        if (pin && i == 0) {
+                            double aw = -Cx * o->mxc[i] * o->mxu[i];
+                            double value = aw * pin[ora_pidx(o, j, k)];
+                            r += value;
+                        }
+                        if (pout && i == o->NX - 2) {
+                            double ae = -Cx * o->mxc[i] * o->mxu[i];   /* metric_xu[i], Conc.c:957 */
+                            double value = ae * pout[ora_pidx(o, j, k)];
+                            r += value;
+                        }
+                    }
                 } else if (s == TB_FLUID || s == TB_BUFFER_SOLID) {
                     r = data[id] * a_dt - conv[id] - dp[id];
                     if (q == TB_Q_V) {
                         double c_term = ora_interp2(c[id - o->NX], o->yc[j - 1], c[id], o->yc[j], o->yv[j]);
                         r = data[id] * a_dt - conv[id] - dp[id] - buoy * c_term;
+                    }
+                    if (q == TB_Q_U) {
+                        if (pin && i == 1) {
+                            double aw = -Cx * o->mxu[i] * o->mxc[i - 1];
+                            double value = aw * pin[ora_pidx(o, j, k)];
+                            r += value;
+                        }
+                        if (pout && i == o->NX - 2) {
+                            double ae = -Cx * o->mxu[i] * o->mxc[i];
+                            double value = ae * pout[ora_pidx(o, j, k)];
+                            r += value;
+                        }
+                    } else if (pout && i == o->NX - 2) {
+                        double ae = -Cx * o->mxc[i] * o->mxu[i + 1];
+                        double value = ae * pout[ora_pidx(o, j, k)];
+                        r += value;
                     }
                 }
                 rhs[id] = r;
@@ -394,6 +452,10 @@ static void build_helm(ora_ctx *o, int q, double dt) {
     size_t pl = (size_t)NX * NY;
     int vNI = NX - 1, uNJ = NY - 1, uNK = o->NZ - 1;
     double mod = 1.0 / dt;
+    /* open x faces drop their wall rule: v, w at x = Lx (Velocity.c:989,
+     * 1336), c at x = 0 and x = Lx (Conc.c:582-607) */
+    int open_w = (q == TB_Q_C) && o->obc.inflow;
+    int open_e = (q != TB_Q_U) && o->obc.outflow;
 #pragma omp parallel for schedule(static)
     for (int k = 0; k < o->NZ; k++)
         for (int j = 0; j < NY; j++)
@@ -439,22 +501,22 @@ static void build_helm(ora_ctx *o, int q, double dt) {
                         if (k == 0) WALL(ab, 4); else ADD(ab, id - pl);
                         if (ora_status(o, q, i, j - 1, k) != TB_BOUNDARY) ADD(as, id - NX);
                         if (i == 0) WALL(aw, 0); else ADD(aw, id - 1);
-                        if (i == vNI - 1) WALL(ae, 1); else if (i < vNI - 1) ADD(ae, id + 1);
+                        if (i == vNI - 1) { if (!open_e) WALL(ae, 1); } else if (i < vNI - 1) ADD(ae, id + 1);
                         if (ora_status(o, q, i, j + 1, k) != TB_BOUNDARY) ADD(an, id + NX);
                         if (k == uNK - 1) WALL(af, 5); else ADD(af, id + pl);
                     } else if (q == TB_Q_W) {
                         if (ora_status(o, q, i, j, k - 1) != TB_BOUNDARY) ADD(ab, id - pl);
                         if (j == 0) WALL(as, 2); else ADD(as, id - NX);
                         if (i == 0) WALL(aw, 0); else ADD(aw, id - 1);
-                        if (i == vNI - 1) WALL(ae, 1); else if (i < vNI - 1) ADD(ae, id + 1);
+                        if (i == vNI - 1) { if (!open_e) WALL(ae, 1); } else if (i < vNI - 1) ADD(ae, id + 1);
                         if (j == uNJ - 1) WALL(an, 3); else ADD(an, id + NX);
                         if (ora_status(o, q, i, j, k + 1) != TB_BOUNDARY) ADD(af, id + pl);
                     } else {
                         int cNI = NX - 1, cNJ = NY - 1, cNK = o->NZ - 1;
                         if (k > 0) ADD(ab, id - pl); else ap += ab;
                         if (j > 0) ADD(as, id - NX); else ap += as;
-                        if (i > 0) ADD(aw, id - 1); else ap += aw;
-                        if (i < cNI - 1) ADD(ae, id + 1); else ap += ae;
+                        if (i > 0) ADD(aw, id - 1); else if (!open_w) ap += aw;
+                        if (i < cNI - 1) ADD(ae, id + 1); else if (!(open_e && i == cNI - 1)) ap += ae;
                         if (j < cNJ - 1) ADD(an, id + NX);
                         else {
                             double coef = an;
@@ -759,6 +821,9 @@ int ora_poisson_rhs(ora_ctx *o, double dt) {
     double a_dEta = 1.0 / o->dEta, a_dXi = 1.0 / o->dXi, a_dPhi = 1.0 / o->dPhi;
     double a_dt = 1.0 / dt;
     size_t pl = (size_t)o->NX * o->NY;
+    /* open x faces: u there from its plane (Pressure.c:242-249) */
+    const double *uin = ora_in_plane(o, TB_Q_U), *uout = ora_out_plane(o, TB_Q_U);
+    int NI = o->NX - 1;
 #pragma omp parallel for schedule(static)
     for (int k = 0; k < o->NZ; k++) {
         double C_z = o->mzc[k] * a_dPhi;
@@ -770,6 +835,8 @@ int ora_poisson_rhs(ora_ctx *o, double dt) {
                 if (o->st[TB_Q_C][id] != TB_BOUNDARY) {
                     double C_x = o->mxc[i] * a_dEta;
                     double dudx = (u[id + 1] - u[id]) * C_x;
+                    if (uin && i == 0) dudx = (u[id + 1] - uin[ora_pidx(o, j, k)]) * C_x;
+                    if (uout && i == NI - 1) dudx = (uout[ora_pidx(o, j, k)] - u[id]) * C_x;
                     double dvdy = (v[id + o->NX] - v[id]) * C_y;
                     double dwdz = (w[id + pl] - w[id]) * C_z;
                     r = -(dudx + dvdy + dwdz) * a_dt;
@@ -974,6 +1041,9 @@ double ora_div_max(ora_ctx *o, int32_t *ijk) {
     size_t pl = (size_t)o->NX * o->NY;
     double div_max = 0.0;
     int im = -1, jm = -1, km = -1;
+    /* open x faces: u there from its plane (Pressure.c:541-548) */
+    const double *uin = ora_in_plane(o, TB_Q_U), *uout = ora_out_plane(o, TB_Q_U);
+    int NI = o->NX - 1;
     for (int k = 0; k < o->NZ; k++)
         for (int j = 0; j < o->NY; j++)
             for (int i = 0; i < o->NX; i++) {
@@ -981,6 +1051,8 @@ double ora_div_max(ora_ctx *o, int32_t *ijk) {
                 double dc = 0.0;
                 if (o->st[TB_Q_C][id] != TB_BOUNDARY) {
                     double dudx = (u[id + 1] - u[id]) * a_dEta * o->mxc[i];
+                    if (uin && i == 0) dudx = (u[id + 1] - uin[ora_pidx(o, j, k)]) * a_dEta * o->mxc[i];
+                    if (uout && i == NI - 1) dudx = (uout[ora_pidx(o, j, k)] - u[id]) * a_dEta * o->mxc[i];
                     double dvdy = (v[id + o->NX] - v[id]) * a_dXi * o->myc[j];
                     double dwdz = (w[id + pl] - w[id]) * a_dPhi * o->mzc[k];
                     dc = fabs(dudx + dvdy + dwdz);
@@ -995,6 +1067,7 @@ double ora_div_max(ora_ctx *o, int32_t *ijk) {
 int ora_store_old(ora_ctx *o) {
     for (int q = 0; q < 3; q++) memcpy(o->old[q], o->f[TB_F_U + q], sizeof(double) * o->NT);
     for (int n = 0; n < o->nconc; n++) memcpy(o->cs[n].old, o->cs[n].c, sizeof(double) * o->NT);
+    ora_open_store_old(o);
     return 0;
 }
 
@@ -1028,6 +1101,7 @@ int ora_rk_average(ora_ctx *o, double old_frac, double new_frac) {
 #pragma omp parallel for schedule(static)
         for (size_t i = 0; i < o->NT; i++) y[i] = old_frac * x[i] + new_frac * y[i];
     }
+    ora_open_rk_average(o, old_frac, new_frac);
     return 0;
 }
 
@@ -1040,6 +1114,8 @@ int ora_step(ora_ctx *o, double dt, tb_step_stats *st) {
     ora_store_old(o);
     int sel = o->conc_sel;
     for (int stage = 0; stage < 2; stage++) {
+        /* the convective outflow condition from the stage-start fields */
+        if (o->obc.outflow) ora_outflow_convective(o, dt);
         ora_total_concentration(o);
         for (int q = 0; q < 3; q++) ora_eno_conv(o, q);
         for (int n = 0; n < o->nconc; n++) {

@@ -37,6 +37,7 @@ typedef struct {
     double *c, *conv, *rhs, *old;
     double *dep;              /* deposited height per (i, k) column, x fastest */
     tb_conc_field p;
+    double *cin, *cout, *cout_old;   /* open-boundary planes (G_inflow, G_outflow, G_outflow_old) */
 } ora_conc;
 
 struct ora_ctx {
@@ -63,6 +64,11 @@ struct ora_ctx {
     int nconc, conc_sel, hindered;
     double *ctot;
     double *tr1, *tr2;         /* transposed velocities (grid->G_dummy1/2) */
+    /* open x faces (Inflow.c, Outflow.c): planes NY*NZ, [k][j] j fastest */
+    tb_open_bc obc;
+    double *u_in;              /* u->G_inflow */
+    double *out[3], *out_old[3];   /* u, v, w G_outflow and G_outflow_old */
+    double influx;             /* u->W_influx */
     double *ubc, *vbc, *wbc;   /* cell-centre velocities (G_data_bc) */
 
     ora_mat helm[4];
@@ -79,6 +85,17 @@ struct ora_ctx {
 static inline size_t ora_idx(const ora_ctx *o, int i, int j, int k) {
     return ((size_t)k * (size_t)o->NY + (size_t)j) * (size_t)o->NX + (size_t)i;
 }
+
+/* entry (j, k) of an open-boundary plane */
+static inline size_t ora_pidx(const ora_ctx *o, int j, int k) { return (size_t)k * (size_t)o->NY + (size_t)j; }
+
+/* the inflow / outflow plane of quantity q (the selected field for c) when
+ * that face is open, else NULL (ora_open.c) */
+const double *ora_in_plane(const ora_ctx *o, int q);
+const double *ora_out_plane(const ora_ctx *o, int q);
+void ora_open_free(ora_ctx *o);
+void ora_open_store_old(ora_ctx *o);
+void ora_open_rk_average(ora_ctx *o, double old_frac, double new_frac);
 
 /* grid / geometry (ora_grid.c) */
 int ora_grid_init(ora_ctx *o);

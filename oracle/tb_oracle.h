@@ -74,6 +74,15 @@ int ora_total_concentration(ora_ctx *o);
 /* Conc_settling_speed_function (Conc.c:815-836) as restated: V_s0 (1 - c phi)^4 */
 double ora_hindered_speed(double conc, double phi_max, double v_s0);
 double ora_cfl_dt(ora_ctx *o, double alpha, double dt_max, double *rate);
+/* open x faces: tb_set_open_boundaries, tb_inflow_profile,
+ * tb_set/get_boundary_plane, tb_get_influx, tb_outflow_convective
+ * (Inflow.c:13-260, Outflow.c:12-304) */
+int ora_set_open_boundaries(ora_ctx *o, const tb_open_bc *bc);
+int ora_inflow_profile(ora_ctx *o, double time, double *influx);
+int ora_set_boundary_plane(ora_ctx *o, int which, int q, const double *plane);
+int ora_get_boundary_plane(ora_ctx *o, int which, int q, double *plane);
+double ora_get_influx(ora_ctx *o);
+int ora_outflow_convective(ora_ctx *o, double dt);
 
 /* output-stage diagnostics (ora_diag.c); per-column arrays are
  * (NZ+1) x (NX+1), x fastest */

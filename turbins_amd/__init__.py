@@ -69,6 +69,12 @@ def lib():
     L.tb_step.argtypes = [P, C.c_double, C.POINTER(abi.StepStats)]
     L.tb_cfl_dt.argtypes = [P, C.c_double, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.tb_set_conc_fields.argtypes = [P, C.c_int, P, C.c_int]
+    L.tb_set_open_boundaries.argtypes = [P, C.POINTER(abi.OpenBC)]
+    L.tb_inflow_profile.argtypes = [P, C.c_double, C.POINTER(C.c_double)]
+    L.tb_set_boundary_plane.argtypes = [P, C.c_int, C.c_int, P]
+    L.tb_get_boundary_plane.argtypes = [P, C.c_int, C.c_int, P]
+    L.tb_get_influx.argtypes = [P, C.POINTER(C.c_double)]
+    L.tb_outflow_convective.argtypes = [P, C.c_double]
     L.tb_viscous_dissipation.argtypes = [P, C.POINTER(C.c_double)]
     L.tb_kinetic_potential_conversion.argtypes = [P, C.POINTER(C.c_double)]
     L.tb_potential_energies.argtypes = [P, C.POINTER(abi.Energies)]
@@ -392,6 +398,37 @@ class Solver:
         """Field n becomes the one the concentration operators and the C,
         CONV_C, RHS_C transfers act on (the reference's c[iconc])."""
         return self._ok(self.L.tb_select_conc(self.h, n), "tb_select_conc")
+
+    def set_open_boundaries(self, **kw):
+        """Open or close the x faces: inflow, outflow, transient_inflow,
+        u_conv (params->U), time_c (abi.open_bc); planes start at zero."""
+        bc = abi.open_bc(**kw)
+        return self._ok(self.L.tb_set_open_boundaries(self.h, C.byref(bc)), "tb_set_open_boundaries")
+
+    def inflow_profile(self, time):
+        """Inflow_u_velocity_profile + Inflow_conc_profile at time; returns
+        the influx."""
+        fl = C.c_double()
+        self._ok(self.L.tb_inflow_profile(self.h, time, C.byref(fl)), "tb_inflow_profile")
+        return fl.value
+
+    def set_plane(self, which, q, plane):
+        a = np.ascontiguousarray(plane, dtype=np.float64).reshape(-1)
+        assert a.size == (self.cfg.ny + 1) * (self.cfg.nz + 1)
+        return self._ok(self.L.tb_set_boundary_plane(self.h, which, q, _ptr(a)), "tb_set_boundary_plane")
+
+    def get_plane(self, which, q):
+        out = np.zeros((self.cfg.ny + 1) * (self.cfg.nz + 1))
+        self._ok(self.L.tb_get_boundary_plane(self.h, which, q, _ptr(out)), "tb_get_boundary_plane")
+        return out
+
+    def influx(self):
+        fl = C.c_double()
+        self._ok(self.L.tb_get_influx(self.h, C.byref(fl)), "tb_get_influx")
+        return fl.value
+
+    def outflow_convective(self, dt):
+        return self._ok(self.L.tb_outflow_convective(self.h, dt), "tb_outflow_convective")
 
     def total_concentration(self):
         """Conc_compute_total_concentration: the v buoyancy source of

@@ -80,6 +80,23 @@ class ConcField(C.Structure):
     ]
 
 
+class OpenBC(C.Structure):
+    """tb_open_bc: the open x faces (params->inflow, outflow,
+    transient_inflow, U, time_c; Input.c:371-372, 424, 446-447)."""
+    _fields_ = [
+        ("inflow", C.c_int32), ("outflow", C.c_int32), ("transient_inflow", C.c_int32), ("reserved", C.c_int32),
+        ("u_conv", C.c_double), ("time_c", C.c_double),
+    ]
+
+
+PLANE_INFLOW, PLANE_OUTFLOW = 0, 1
+
+
+def open_bc(inflow=False, outflow=False, transient_inflow=False, u_conv=1.0, time_c=0.035):
+    """tb_open_bc with the reference's defaults (U = 1, time_c = 0.035)."""
+    return OpenBC(1 if inflow else 0, 1 if outflow else 0, 1 if transient_inflow else 0, 0, u_conv, time_c)
+
+
 def conc_fields(fields):
     """ctypes array of tb_conc_field from dicts with keys type, pe,
     v_settle, alpha (default 1) and phi_max (default 0)."""

@@ -270,6 +270,8 @@ struct ConcSlot {
     double *dep = nullptr;       // deposited height per (i, k) column, x fastest (allocated on first use)
     tb_conc_field p{};
     HelmHistory h;
+    // open x faces: inflow, outflow and stage-start outflow planes (NY*NZ, [k][j])
+    double *cin = nullptr, *cout = nullptr, *cout_old = nullptr;
 };
 
 struct tb_ctx {
@@ -300,6 +302,12 @@ struct tb_ctx {
     int conc_sel = 0;
     int hindered = 0;            // params->hindered_settling
     double *ctot = nullptr;      // total concentration (nconc > 1)
+
+    // open x faces (tb_open.hip): planes of NY*NZ doubles indexed [k][j]
+    tb_open_bc obc{};
+    double *u_in = nullptr;                     // u->G_inflow
+    double *outp[3] = {}, *outp_old[3] = {};    // u, v, w G_outflow, G_outflow_old
+    double *open_scal = nullptr;                // device: [0] the influx u->W_influx
 
     // immersed boundary: node tables and the scaled matrix rows
     int nib[4] = {0, 0, 0, 0};
@@ -434,6 +442,16 @@ int tb_net_predict(tb_ctx *c, double *out, int correction_only);
 int tb_net_record_one(tb_ctx *c);
 double tb_elapsed_ms(tb_ctx *c);
 int tb_conc_use(tb_ctx *c, int n);
+// open x faces (tb_open.hip): the plane of quantity q (the selected field for
+// c) while that face is open, else null; the right-hand-side terms of every
+// quantity and field; the outflow planes' stage-start copy and RK average
+const double *tb_in_plane(const tb_ctx *c, int q);
+const double *tb_out_plane(const tb_ctx *c, int q);
+int tb_open_rhs(tb_ctx *c, int q);   // q = -1: every quantity and field
+int tb_open_store_old(tb_ctx *c);
+int tb_open_rk_average(tb_ctx *c, double old_frac, double new_frac);
+int tb_outflow_async(tb_ctx *c, double dt);   // tb_outflow_convective without the final sync
+void tb_open_free(tb_ctx *c);
 int tb_helm_inc_alloc(tb_ctx *c, int q);   // the increment buffer of quantity q (once)
 void tb_free_deposits(tb_ctx *c, int first);
 int tb_ib_rows(tb_ctx *c, int q);

@@ -171,6 +171,7 @@ extern "C" void tb_destroy(tb_ctx *c) {
     for (int n = 1; n < TB_NCONC_MAX; n++) {
         hipFree(c->cs[n].c); hipFree(c->cs[n].conv); hipFree(c->cs[n].rhs); hipFree(c->cs[n].old);
     }
+    tb_open_free(c);
     for (int q = 0; q < 3; q++) hipFree(c->helm[q].inc);
     for (int n = 0; n < TB_NCONC_MAX; n++) hipFree(c->cs[n].h.inc);   // slot conc_sel's is helm[3].inc (synced)
     hipFree(c->ctot);
@@ -451,7 +452,7 @@ extern "C" int tb_store_old(tb_ctx *c) {
         TB_CHECK(c, hipMemcpyAsync(c->old[q], c->f[TB_F_U + q], bytes, hipMemcpyDeviceToDevice, c->s));
     for (int n = 0; n < c->nconc; n++)
         TB_CHECK(c, hipMemcpyAsync(c->cs[n].old, c->cs[n].c, bytes, hipMemcpyDeviceToDevice, c->s));
-    return 0;
+    return tb_open_store_old(c);
 }
 
 extern "C" int tb_rk_average(tb_ctx *c, double old_frac, double new_frac) {
@@ -465,7 +466,7 @@ extern "C" int tb_rk_average(tb_ctx *c, double old_frac, double new_frac) {
         hipLaunchKernelGGL(k_axpby1, grid, dim3(256), 0, c->s, c->cs[m].c, (const double *)c->cs[m].old, old_frac,
                            new_frac, n);
     TB_LAUNCH_CHECK(c);
-    return 0;
+    return tb_open_rk_average(c, old_frac, new_frac);
 }
 
 // ---------------------------------------------------------------------------

@@ -36,6 +36,7 @@ struct ViscArgs {
     const uint8_t *sc;
     double k_left, k_right, k_bottom, k_top, k_back, k_front;   // ghost factors
     double Cx, Cy, Cz;
+    const double *uin, *uout, *vout, *wout;   // planes of open x faces (null: closed)
 };
 
 __device__ double visc_cell(const ViscArgs &A, int i, int j, int k) {
@@ -61,7 +62,8 @@ __device__ double visc_cell(const ViscArgs &A, int i, int j, int k) {
     double U_w, V_w, W_w, U_e, V_e, W_e, U_s, V_s, W_s, U_n, V_n, W_n, U_b, V_b, W_b, U_f, V_f, W_f;
     if (i == 0) {
         U_aw *= g.mxc[i];
-        U_w = 0.0; V_w = A.k_left * v_; W_w = A.k_left * w_;
+        U_w = A.uin ? A.uin[(long)k * d.NY + j] : 0.0;   // Extract.c:579-585
+        V_w = A.k_left * v_; W_w = A.k_left * w_;
     } else {
         U_aw *= g.mxc[i - 1];
         U_w = u[id - 1]; V_w = v[id - 1]; W_w = w[id - 1];
@@ -81,7 +83,12 @@ __device__ double visc_cell(const ViscArgs &A, int i, int j, int k) {
         U_b = u[id - pl]; V_b = v[id - pl]; W_b = w[id - pl];
     }
     if (i == NI - 1) {
-        U_e = 0.0; V_e = A.k_right * v_; W_e = A.k_right * w_;
+        if (A.uout) {   // Extract.c:632-639
+            const long p = (long)k * d.NY + j;
+            U_e = A.uout[p]; V_e = A.vout[p]; W_e = A.wout[p];
+        } else {
+            U_e = 0.0; V_e = A.k_right * v_; W_e = A.k_right * w_;
+        }
     } else {
         U_e = u[id + 1]; V_e = v[id + 1]; W_e = w[id + 1];
     }
@@ -429,6 +436,10 @@ int cell_sums(tb_ctx *c, bool visc, int nslots, double *out, const char *who) {
         A.Cx = 1.0 / (c->dEta * c->dEta);
         A.Cy = 1.0 / (c->dXi * c->dXi);
         A.Cz = 1.0 / (c->dPhi * c->dPhi);
+        A.uin = tb_in_plane(c, TB_Q_U);
+        A.uout = tb_out_plane(c, TB_Q_U);
+        A.vout = tb_out_plane(c, TB_Q_V);
+        A.wout = tb_out_plane(c, TB_Q_W);
         hipLaunchKernelGGL(k_visc, grid, tiles_block(), 0, c->s, A, c->red, t);
     } else {
         if (tb_total_concentration(c)) return -1;

@@ -62,13 +62,16 @@ struct LineSpec {
     int uni;               // 1: every gap pos[m+1]-pos[m] of [lo, hi] is h, every pos[m+1]-pos[m-1] is 2h,
                            //    h a power of two (pow2_uniform)
     double h, rh;          // that h and its exact reciprocal
+    // x lines next to an open face: every ghost of that side takes the
+    // plane's value at the line's (j, k) (null: the wall rule)
+    const double *open_lo, *open_hi;
 };
 
 // Window of 2R+1 values and positions around axis position m0 (buffer
 // index R + m0 - lo of the reference's line).
 template <int R>
 __device__ inline void load_window(const LineSpec &L, const double *__restrict__ f, long base, int m0,
-                                   double (&w)[7], double (&p)[7]) {
+                                   double (&w)[7], double (&p)[7], double olo, double ohi) {
     // every load first, at axis positions clamped into the interior span, so
     // the loads are issued together; a position left of lo therefore holds
     // the first interior value D0[order], one right of hi the last one
@@ -95,6 +98,7 @@ __device__ inline void load_window(const LineSpec &L, const double *__restrict__
                 pv = 2.0 * L.wall_lo - L.pos[L.lo + R - ex - 1];
                 wv = (L.sgn_lo < 0.0) ? -first : +first;
             }
+            if (L.open_lo) wv = olo;
             w[3 + o] = wv;
             p[3 + o] = pv;
         } else if (m > L.hi) {
@@ -111,6 +115,7 @@ __device__ inline void load_window(const LineSpec &L, const double *__restrict__
                 if (L.hi_mode) wv = L.hi_scale * last;
                 else wv = (L.sgn_hi < 0.0) ? -last : +last;
             }
+            if (L.open_hi) wv = ohi;
             w[3 + o] = wv;
             p[3 + o] = pv;
         }
@@ -192,14 +197,18 @@ __device__ inline double eno_uniform(const LineSpec &L, const double *__restrict
     return vel * dQ;
 }
 
+// pidx: the node's (j, k) plane entry k NY + j (x lines; read only where the
+// window reaches an open face)
 template <int R>
 __device__ inline double eno_line_point(const LineSpec &L, const double *__restrict__ f, long base, int m0,
-                                        double vel) {
+                                        double vel, long pidx = 0) {
     if constexpr (R <= 2) {
         if (L.uni && m0 - R >= L.lo && m0 + R <= L.hi) return eno_uniform<R>(L, f, base, vel);
     }
     double w[7], p[7];
-    load_window<R>(L, f, base, m0, w, p);
+    const double olo = (L.open_lo && m0 - R < L.lo) ? L.open_lo[pidx] : 0.0;
+    const double ohi = (L.open_hi && m0 + R > L.hi) ? L.open_hi[pidx] : 0.0;
+    load_window<R>(L, f, base, m0, w, p, olo, ohi);
     return eno_eval<R>(w, p, vel);
 }
 
@@ -222,6 +231,7 @@ struct MomArgs {
     double *ru, *rv, *rw;
     double a_dt, buoy;
     int store_conv;   // 0 with RHS: the convective terms are consumed in registers only
+    const double *uin, *uout;   // u's planes of open x faces (null: closed)
 };
 
 __device__ inline bool vel_row(int s) { return s == TB_FLUID || s == TB_BUFFER_SOLID; }
@@ -234,6 +244,7 @@ __device__ inline void mom_point(const MomArgs &a, int i, int j, int k) {
     const double *u = a.u, *v = a.v, *w = a.w;
     const double *W = a.W;
     const int wm = a.Wmax;
+    const long pidx = (long)k * d.NY + j;
     // u node
     {
         double r = 0.0;
@@ -243,7 +254,7 @@ __device__ inline void mom_point(const MomArgs &a, int i, int j, int k) {
                                  W[W_VU_Y * wm + j]);
             double wu = interp4w(w[id - 1], w[id], w[id + d.plane - 1], w[id + d.plane], W[W_WU_X * wm + i],
                                  W[W_WU_Z * wm + k]);
-            r = eno_line_point<R>(a.L[0].ax[0], u, id, i, u[id]);
+            r = eno_line_point<R>(a.L[0].ax[0], u, id, i, u[id], pidx);
             r += eno_line_point<R>(a.L[0].ax[1], u, id, j, vu);
             r += eno_line_point<R>(a.L[0].ax[2], u, id, k, wu);
         }
@@ -255,11 +266,14 @@ __device__ inline void mom_point(const MomArgs &a, int i, int j, int k) {
         double r = 0.0;
         const int s = a.sv[id];
         if (s != TB_BOUNDARY) {
-            double uv = interp4w(u[id - d.px], u[id - d.px + 1], u[id], u[id + 1], W[W_UV_X * wm + i],
-                                 W[W_UV_Y * wm + j]);
+            // u beside an open face from its plane (Velocity.c:3491-3506)
+            double u0 = u[id - d.px], u1 = u[id - d.px + 1], u2 = u[id], u3 = u[id + 1];
+            if (a.uin && i == 0) { u0 = a.uin[pidx - 1]; u2 = a.uin[pidx]; }
+            if (a.uout && i == d.NX - 2) { u1 = a.uout[pidx - 1]; u3 = a.uout[pidx]; }
+            double uv = interp4w(u0, u1, u2, u3, W[W_UV_X * wm + i], W[W_UV_Y * wm + j]);
             double wv = interp4w(w[id - d.px], w[id + d.plane - d.px], w[id], w[id + d.plane], W[W_WV_Z * wm + k],
                                  W[W_WV_Y * wm + j]);
-            r = eno_line_point<R>(a.L[1].ax[0], v, id, i, uv);
+            r = eno_line_point<R>(a.L[1].ax[0], v, id, i, uv, pidx);
             r += eno_line_point<R>(a.L[1].ax[1], v, id, j, v[id]);
             r += eno_line_point<R>(a.L[1].ax[2], v, id, k, wv);
         }
@@ -278,11 +292,14 @@ __device__ inline void mom_point(const MomArgs &a, int i, int j, int k) {
         double r = 0.0;
         const int s = a.sw[id];
         if (s != TB_BOUNDARY) {
-            double uw = interp4w(u[id - d.plane], u[id - d.plane + 1], u[id], u[id + 1], W[W_UW_X * wm + i],
-                                 W[W_UW_Z * wm + k]);
+            // u beside an open face from its plane (Velocity.c:3589-3603)
+            double u0 = u[id - d.plane], u1 = u[id - d.plane + 1], u2 = u[id], u3 = u[id + 1];
+            if (a.uin && i == 0) { u0 = a.uin[pidx - d.NY]; u2 = a.uin[pidx]; }
+            if (a.uout && i == d.NX - 2) { u1 = a.uout[pidx - d.NY]; u3 = a.uout[pidx]; }
+            double uw = interp4w(u0, u1, u2, u3, W[W_UW_X * wm + i], W[W_UW_Z * wm + k]);
             double vw = interp4w(v[id - d.plane], v[id], v[id - d.plane + d.px], v[id + d.px], W[W_VW_Z * wm + k],
                                  W[W_VW_Y * wm + j]);
-            r = eno_line_point<R>(a.L[2].ax[0], w, id, i, uw);
+            r = eno_line_point<R>(a.L[2].ax[0], w, id, i, uw, pidx);
             r += eno_line_point<R>(a.L[2].ax[1], w, id, j, vw);
             r += eno_line_point<R>(a.L[2].ax[2], w, id, k, w[id]);
         }
@@ -312,6 +329,7 @@ struct ConcArgs {
     double *rc;     // right-hand side in the same pass (RHS = true): the k_rhs_conc formula
     double a_dt;
     int store_conv;
+    const double *uin, *uout;   // u's planes of open x faces (null: closed)
 };
 
 template <int R, bool RHS>
@@ -322,12 +340,16 @@ __device__ inline void conc_point(const ConcArgs &a, int i, int j, int k) {
     double r = 0.0;
     const int s = a.sc[id];
     if (s != TB_BOUNDARY) {
-        // Velocity_cell_center (Velocity.c:1476-1602) at this cell
-        double ubc = interp2d(a.u[id], g.xu[i], a.u[id + 1], g.xu[i + 1], g.xc[i]);
+        // Velocity_cell_center (Velocity.c:1476-1602) at this cell; u on an
+        // open face from its plane (Velocity.c:1544-1558)
+        const long pidx = (long)k * d.NY + j;
+        const double uw_ = (a.uin && i == 0) ? a.uin[pidx] : a.u[id];
+        const double ue_ = (a.uout && i == d.NX - 2) ? a.uout[pidx] : a.u[id + 1];
+        double ubc = interp2d(uw_, g.xu[i], ue_, g.xu[i + 1], g.xc[i]);
         double vbc = interp2d(a.v[id], g.yv[j], a.v[id + d.px], g.yv[j + 1], g.yc[j]);
         double wbc = interp2d(a.w[id], g.zw[k], a.w[id + d.plane], g.zw[k + 1], g.zc[k]);
         if (a.particle) vbc = particle_v(vbc, s, a.c[id], a.hindered, a.phi_max, a.v_settle);
-        r = eno_line_point<R>(a.L.ax[0], a.c, id, i, ubc);
+        r = eno_line_point<R>(a.L.ax[0], a.c, id, i, ubc, pidx);
         r += eno_line_point<R>(a.L.ax[1], a.c, id, j, vbc);
         r += eno_line_point<R>(a.L.ax[2], a.c, id, k, wbc);
     }
@@ -382,6 +404,9 @@ static QLines make_lines(tb_ctx *c, int q) {
         L.wall_hi = whi[ax];
         L.pos = posq[q][ax];
         L.stride = stride[ax];
+        // open x faces (Velocity.c:2193-2233, 2442-2444, 2669-2671; Conc.c:1099-1122)
+        L.open_lo = (ax == 0 && (q == TB_Q_U || q == TB_Q_C)) ? tb_in_plane(c, q) : nullptr;
+        L.open_hi = ax == 0 ? tb_out_plane(c, q) : nullptr;
         const int hq = (q == TB_Q_C) ? ax : 3 + 3 * q + ax;   // host copy of posq[q][ax]
         L.uni = pow2_uniform(c->hgrid[hq], L.lo, L.hi, L.h) ? 1 : 0;
         L.rh = L.uni ? 1.0 / L.h : 0.0;
@@ -780,6 +805,8 @@ static MomArgs mom_args(tb_ctx *c, double rhs_dt, bool store_conv) {
     a.a_dt = rhs_dt > 0.0 ? 1.0 / rhs_dt : 0.0;
     a.buoy = c->cfg.buoyancy;
     a.store_conv = store_conv ? 1 : 0;
+    a.uin = tb_in_plane(c, TB_Q_U);
+    a.uout = tb_out_plane(c, TB_Q_U);
     return a;
 }
 
@@ -825,6 +852,8 @@ static ConcArgs conc_args(tb_ctx *c, double rhs_dt, bool store_conv) {
     a.rc = c->f[TB_F_RHS_C];
     a.a_dt = rhs_dt > 0.0 ? 1.0 / rhs_dt : 0.0;
     a.store_conv = store_conv ? 1 : 0;
+    a.uin = tb_in_plane(c, TB_Q_U);
+    a.uout = tb_out_plane(c, TB_Q_U);
     return a;
 }
 
@@ -929,7 +958,7 @@ static int eno_fused(tb_ctx *c, double rhs_dt, bool store_conv) {
 
 // every concentration field is computed (each Conc_* call of the reference
 // takes its c[iconc])
-int tb_eno_rhs_async(tb_ctx *c, double dt, bool store_conv) {
+static int eno_rhs_kernels(tb_ctx *c, double dt, bool store_conv) {
     if (c->cfg.eno_order == 2) {
         const int rc = eno_fused(c, dt, store_conv);
         if (rc <= 0) return rc;
@@ -940,6 +969,12 @@ int tb_eno_rhs_async(tb_ctx *c, double dt, bool store_conv) {
     for (int n = 0; n < c->nconc; n++)
         if (tb_conc_use(c, n) || eno_conc(c, dt, store_conv)) return -1;
     return tb_conc_use(c, sel);
+}
+
+// and the open faces' right-hand-side terms after them
+int tb_eno_rhs_async(tb_ctx *c, double dt, bool store_conv) {
+    if (eno_rhs_kernels(c, dt, store_conv)) return -1;
+    return tb_open_rhs(c, -1);
 }
 
 extern "C" int tb_eno_conv_rhs(tb_ctx *c, double dt) {
@@ -1008,7 +1043,7 @@ extern "C" int tb_set_rhs(tb_ctx *c, int q, double dt) {
                            c->f[TB_F_RHS_U + q],
                            a_dt, c->cfg.buoyancy, t);
     TB_LAUNCH_CHECK(c);
-    return 0;
+    return tb_open_rhs(c, q);
 }
 
 // ---------------------------------------------------------------------------
@@ -1021,14 +1056,18 @@ extern "C" int tb_set_rhs(tb_ctx *c, int q, double dt) {
 __global__ void __launch_bounds__(256) k_diag(Dims d, GridDev g, const double *__restrict__ u,
                                               const double *__restrict__ v, const double *__restrict__ w,
                                               const double *__restrict__ c, const uint8_t *__restrict__ sc,
-                                              double *part, Tiles t) {
+                                              double *part, Tiles t, const double *__restrict__ uin,
+                                              const double *__restrict__ uout) {
     __shared__ double sh[48];
     double acc[3] = {0.0, 0.0, 0.0};
     int i, j, k;
     if (tile_point(t, i, j, k)) {
         const long id = gidx(d, i, j, k);
         if (sc[id] == TB_FLUID) {
-            const double ubc = interp2d(u[id], g.xu[i], u[id + 1], g.xu[i + 1], g.xc[i]);
+            const long pidx = (long)k * d.NY + j;
+            const double u0 = (uin && i == 0) ? uin[pidx] : u[id];
+            const double u1 = (uout && i == d.NX - 2) ? uout[pidx] : u[id + 1];
+            const double ubc = interp2d(u0, g.xu[i], u1, g.xu[i + 1], g.xc[i]);
             const double vbc = interp2d(v[id], g.yv[j], v[id + d.px], g.yv[j + 1], g.yc[j]);
             const double wbc = interp2d(w[id], g.zw[k], w[id + d.plane], g.zw[k + 1], g.zc[k]);
             const double dx = g.dxc[i], dy = g.dyc[j], dz = g.dzc[k];
@@ -1056,7 +1095,7 @@ static int diag_sums(tb_ctx *c, double out[3]) {
     }
     hipLaunchKernelGGL(k_diag, grid, tiles_block(), 0, c->s, c->d, c->g, (const double *)c->f[TB_F_U],
                        (const double *)c->f[TB_F_V], (const double *)c->f[TB_F_W], (const double *)c->f[TB_F_C],
-                       (const uint8_t *)c->st[3], c->red, t);
+                       (const uint8_t *)c->st[3], c->red, t, tb_in_plane(c, TB_Q_U), tb_out_plane(c, TB_Q_U));
     TB_LAUNCH_CHECK(c);
     if (tb_reduce_finish(c, (int)grid.x, 3, S_TMP0)) return -1;
     if (tb_read_scalars(c, S_TMP0, 3)) return -1;
@@ -1149,7 +1188,8 @@ __device__ inline double max_nan(double a, double b) { return (a != a || a > b) 
 
 __global__ void __launch_bounds__(256) k_cfl(Dims d, GridDev g, const double *__restrict__ u,
                                              const double *__restrict__ v, const double *__restrict__ w,
-                                             const uint8_t *__restrict__ sc, double *part, Tiles t) {
+                                             const uint8_t *__restrict__ sc, double *part, Tiles t,
+                                             const double *__restrict__ uin, const double *__restrict__ uout) {
     __shared__ double sh[256];
     const int tid = threadIdx.y * blockDim.x + threadIdx.x;
     double r = 0.0;
@@ -1157,7 +1197,10 @@ __global__ void __launch_bounds__(256) k_cfl(Dims d, GridDev g, const double *__
     if (tile_point(t, i, j, k)) {
         const long id = gidx(d, i, j, k);
         if (sc[id] == TB_FLUID) {
-            const double ubc = interp2d(u[id], g.xu[i], u[id + 1], g.xu[i + 1], g.xc[i]);
+            const long pidx = (long)k * d.NY + j;
+            const double u0 = (uin && i == 0) ? uin[pidx] : u[id];
+            const double u1 = (uout && i == d.NX - 2) ? uout[pidx] : u[id + 1];
+            const double ubc = interp2d(u0, g.xu[i], u1, g.xu[i + 1], g.xc[i]);
             const double vbc = interp2d(v[id], g.yv[j], v[id + d.px], g.yv[j + 1], g.yc[j]);
             const double wbc = interp2d(w[id], g.zw[k], w[id + d.plane], g.zw[k + 1], g.zc[k]);
             r = fabs(ubc) / g.dxc[i] + fabs(vbc) / g.dyc[j] + fabs(wbc) / g.dzc[k];
@@ -1198,7 +1241,7 @@ extern "C" int tb_cfl_dt(tb_ctx *c, double alpha, double dt_max, double *dt, dou
     }
     hipLaunchKernelGGL(k_cfl, grid, tiles_block(), 0, c->s, c->d, c->g, (const double *)c->f[TB_F_U],
                        (const double *)c->f[TB_F_V], (const double *)c->f[TB_F_W], (const uint8_t *)c->st[3], c->red,
-                       t);
+                       t, tb_in_plane(c, TB_Q_U), tb_out_plane(c, TB_Q_U));
     hipLaunchKernelGGL(k_max_final, dim3(1), dim3(1024), 0, c->s, (const double *)c->red, (long)grid.x,
                        c->scal + S_TMP0);
     TB_LAUNCH_CHECK(c);

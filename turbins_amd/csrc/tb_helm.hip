@@ -27,6 +27,9 @@ struct HelmOp {
     int particle;
     double robin_b_scale;   // 2/(Pe*dXi)
     double vs_abs;
+    // open x faces drop their wall rule: c at x = 0 with inflow, v, w and c
+    // at x = Lx with outflow (Velocity.c:989, 1336; Conc.c:582-607)
+    int open_w, open_e;
 };
 
 // wall rule applied to ap when a neighbour is absent:
@@ -86,17 +89,17 @@ __global__ void __launch_bounds__(256) k_helm_km(HelmOp H, const double *__restr
         } else if (q == TB_Q_V) {
             hs = j - 1 >= 1;   rs = 0;
             hw = i != 0;       rw = W[0] ? 1 : 2;
-            he = i < NX - 2;   re = (i == NX - 2) ? (W[1] ? 1 : 2) : 0;
+            he = i < NX - 2;   re = (i == NX - 2) ? (H.open_e ? 0 : (W[1] ? 1 : 2)) : 0;
             hn = j + 1 <= NY - 2; rn = 0;
         } else if (q == TB_Q_W) {
             hs = j != 0;       rs = W[2] ? 1 : 2;
             hw = i != 0;       rw = W[0] ? 1 : 2;
-            he = i < NX - 2;   re = (i == NX - 2) ? (W[1] ? 1 : 2) : 0;
+            he = i < NX - 2;   re = (i == NX - 2) ? (H.open_e ? 0 : (W[1] ? 1 : 2)) : 0;
             hn = j != NY - 2;  rn = W[3] ? 1 : 2;
         } else {
             hs = j > 0;        rs = 2;
-            hw = i > 0;        rw = 2;
-            he = i < NX - 2;   re = 2;
+            hw = i > 0;        rw = H.open_w ? 0 : 2;
+            he = i < NX - 2;   re = H.open_e ? 0 : 2;
             hn = j < NY - 2;   rn = 2;
         }
         double cn = an;   // north wall term (Robin for settling particles, Conc.c:625-635)
@@ -209,6 +212,8 @@ static HelmOp make_op(tb_ctx *c, int q, double dt) {
     H.particle = (q == TB_Q_C && c->cfg.conc_type == TB_PARTICLE);
     H.robin_b_scale = 2.0 / (c->cfg.pe * c->dXi);
     H.vs_abs = fabs(c->cfg.v_settle);
+    H.open_w = (q == TB_Q_C) && c->obc.inflow;
+    H.open_e = (q != TB_Q_U) && c->obc.outflow;
     return H;
 }
 
@@ -279,8 +284,9 @@ __device__ inline AxisGS axis_x(const HelmOp &H, int i) {
     const double Cx = H.Cx;
     const double t = -Cx * mpx * (mex + mwx), aw = Cx * mpx * mwx, ae = Cx * mpx * mex;
     if (q == TB_Q_U) return axis_gs(t, aw, i - 1 >= 1, 0, ae, ae, i + 1 <= NX - 2, 0);
-    if (q == TB_Q_C) return axis_gs(t, aw, i > 0, 2, ae, ae, i < NX - 2, 2);
-    return axis_gs(t, aw, i != 0, W[0] ? 1 : 2, ae, ae, i < NX - 2, (i == NX - 2) ? (W[1] ? 1 : 2) : 0);
+    if (q == TB_Q_C) return axis_gs(t, aw, i > 0, H.open_w ? 0 : 2, ae, ae, i < NX - 2, H.open_e ? 0 : 2);
+    return axis_gs(t, aw, i != 0, W[0] ? 1 : 2, ae, ae, i < NX - 2,
+                   (i == NX - 2) ? (H.open_e ? 0 : (W[1] ? 1 : 2)) : 0);
 }
 
 __device__ inline AxisGS axis_y(const HelmOp &H, int j) {

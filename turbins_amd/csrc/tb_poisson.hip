@@ -22,9 +22,11 @@
 
 // ---------------------------------------------------------------------------
 // Pressure_set_RHS (Pressure.c:171-276)
+// u on an open x face from its plane (Pressure.c:242-249; null: closed)
 __global__ void k_prhs(Dims d, GridDev g, const double *__restrict__ u, const double *__restrict__ v,
                        const double *__restrict__ w, const uint8_t *__restrict__ sc, double *__restrict__ rhs,
-                       double a_dEta, double a_dXi, double a_dPhi, double a_dt, Tiles t) {
+                       double a_dEta, double a_dXi, double a_dPhi, double a_dt, Tiles t,
+                       const double *__restrict__ uin, const double *__restrict__ uout) {
     int i, j, k;
     if (!tile_point(t, i, j, k)) return;
     long id = gidx(d, i, j, k);
@@ -34,6 +36,8 @@ __global__ void k_prhs(Dims d, GridDev g, const double *__restrict__ u, const do
         double C_y = g.myc[j] * a_dXi;
         double C_x = g.mxc[i] * a_dEta;
         double dudx = (u[id + 1] - u[id]) * C_x;
+        if (uin && i == 0) dudx = (u[id + 1] - uin[(long)k * d.NY + j]) * C_x;
+        if (uout && i == d.NX - 2) dudx = (uout[(long)k * d.NY + j] - u[id]) * C_x;
         double dvdy = (v[id + d.px] - v[id]) * C_y;
         double dwdz = (w[id + d.plane] - w[id]) * C_z;
         r = -(dudx + dvdy + dwdz) * a_dt;
@@ -45,7 +49,8 @@ extern "C" int tb_poisson_rhs(tb_ctx *c, double dt) {
     if (!c) return -1;
     Tiles t = make_tiles(c->d.NX, c->d.NY, c->d.NZ);
     hipLaunchKernelGGL(k_prhs, tiles_grid(t), tiles_block(), 0, c->s, c->d, c->g, c->f[TB_F_U], c->f[TB_F_V],
-                       c->f[TB_F_W], c->st[3], c->f[TB_F_PRHS], 1.0 / c->dEta, 1.0 / c->dXi, 1.0 / c->dPhi, 1.0 / dt, t);
+                       c->f[TB_F_W], c->st[3], c->f[TB_F_PRHS], 1.0 / c->dEta, 1.0 / c->dXi, 1.0 / c->dPhi, 1.0 / dt, t,
+                       tb_in_plane(c, TB_Q_U), tb_out_plane(c, TB_Q_U));
     TB_LAUNCH_CHECK(c);
     return 0;
 }
@@ -60,7 +65,7 @@ extern "C" int tb_real_pressure_rhs(tb_ctx *c) {
     Tiles t = make_tiles(c->d.NX, c->d.NY, c->d.NZ);
     hipLaunchKernelGGL(k_prhs, tiles_grid(t), tiles_block(), 0, c->s, c->d, c->g, c->f[TB_F_DPDX], c->f[TB_F_DPDY],
                        c->f[TB_F_DPDZ], c->st[3], c->f[TB_F_PRHS], 1.0 / c->dEta, 1.0 / c->dXi, 1.0 / c->dPhi, 1.0,
-                       t);
+                       t, (const double *)nullptr, (const double *)nullptr);
     TB_LAUNCH_CHECK(c);
     return 0;
 }
@@ -2280,7 +2285,8 @@ __global__ void __launch_bounds__(256) k_div(Dims d, GridDev g, const double *__
                                              const double *__restrict__ v, const double *__restrict__ w,
                                              const double *__restrict__ cc, const uint8_t *__restrict__ sc,
                                              double *__restrict__ div, double *pmax, long *pidx, double a_dEta,
-                                             double a_dXi, double a_dPhi, Tiles t) {
+                                             double a_dXi, double a_dPhi, Tiles t, const double *__restrict__ uin,
+                                             const double *__restrict__ uout) {
     __shared__ double sv[256];
     __shared__ long si[256];
     int tid = threadIdx.y * blockDim.x + threadIdx.x;
@@ -2295,6 +2301,9 @@ __global__ void __launch_bounds__(256) k_div(Dims d, GridDev g, const double *__
         bool finite = true;
         if (sc[id] != TB_BOUNDARY) {
             double dudx = (u[id + 1] - u[id]) * a_dEta * g.mxc[i];
+            // u on an open x face from its plane (Pressure.c:541-548)
+            if (uin && i == 0) dudx = (u[id + 1] - uin[(long)k * d.NY + j]) * a_dEta * g.mxc[i];
+            if (uout && i == d.NX - 2) dudx = (uout[(long)k * d.NY + j] - u[id]) * a_dEta * g.mxc[i];
             double dvdy = (v[id + d.px] - v[id]) * a_dXi * g.myc[j];
             double dwdz = (w[id + d.plane] - w[id]) * a_dPhi * g.mzc[k];
             dc = fabs(dudx + dvdy + dwdz);
@@ -2355,7 +2364,7 @@ extern "C" int tb_div_max(tb_ctx *c, double *div_max, int32_t *ijk) {
     long *pi = reinterpret_cast<long *>(c->red + RED_MAX_BLOCKS);
     hipLaunchKernelGGL(k_div, tiles_grid(t), tiles_block(), 0, c->s, c->d, c->g, c->f[TB_F_U], c->f[TB_F_V],
                        c->f[TB_F_W], c->f[TB_F_C], c->st[3], c->f[TB_F_DIV], pm, pi, 1.0 / c->dEta, 1.0 / c->dXi,
-                       1.0 / c->dPhi, t);
+                       1.0 / c->dPhi, t, tb_in_plane(c, TB_Q_U), tb_out_plane(c, TB_Q_U));
     hipLaunchKernelGGL(k_div_final, dim3(1), dim3(1024), 0, c->s, pm, pi, nb, c->scal + S_TMP0,
                        reinterpret_cast<long *>(c->scal + S_TMP1));
     TB_LAUNCH_CHECK(c);
@@ -2442,7 +2451,8 @@ extern "C" int tb_div_max_noalloc(const tb_box *box, const double *mxc, const do
     long *pi = reinterpret_cast<long *>(scr + NOALLOC_BLOCKS);
     hipStream_t st = static_cast<hipStream_t>(stream);
     hipLaunchKernelGGL(k_div, dim3(nb), tiles_block(), 0, st, d, g, u, v, w, c, flags_c, div, pm, pi,
-                       1.0 / box->deta, 1.0 / box->dxi, 1.0 / box->dphi, t);
+                       1.0 / box->deta, 1.0 / box->dxi, 1.0 / box->dphi, t, (const double *)nullptr,
+                       (const double *)nullptr);
     hipLaunchKernelGGL(k_div_final, dim3(1), dim3(1024), 0, st, (const double *)pm, (const long *)pi, (long)nb,
                        out_max, reinterpret_cast<long *>(out_index));
     if (hipGetLastError() != hipSuccess) return TB_NOALLOC_LAUNCH;

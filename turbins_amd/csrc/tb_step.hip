@@ -20,6 +20,11 @@ extern "C" int tb_step(tb_ctx *c, double dt, tb_step_stats *st) {
     if (tb_store_old(c)) return -1;
     const int sel = c->conc_sel;
     for (int stage = 0; stage < 2; stage++) {
+        // the convective outflow condition from the stage-start fields
+        // (Outflow_impose_convective_boundary; its place in the unpublished
+        // driver is reconstructed: each RK stage advances the boundary ODE
+        // with an Euler step, and the RK average combines the two)
+        if (c->obc.outflow && tb_outflow_async(c, dt)) return -1;
         // c_total of the stage-start fields for the v buoyancy (several fields)
         if (tb_total_concentration(c)) return -1;
         // ENO and the right-hand sides (tb_eno_conv + tb_set_rhs) in one
