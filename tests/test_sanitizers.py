@@ -7,7 +7,8 @@ sanitizer runtimes preloaded loads that build (TURBINS_ORACLE_LIB) and runs
 every oracle entry point the parity tests rely on -- geometry with immersed
 boundaries, ENO of every order, right-hand sides, Helmholtz and Poisson
 operators and solves, multigrid, steps with several concentration fields and
-hindered settling, the diagnostics and the network's forward and gradient --
+hindered settling, open x faces, the diagnostics and the network's forward
+and gradient --
 on small boxes.  `make -C turbins_amd/csrc asan` builds the library with
 the sanitizers on its host code (device code as usual); a child with
 clang's ASan runtime preloaded runs the host paths that need no GPU: the
@@ -71,6 +72,17 @@ for name, dims in (("bump_small", (24, 16, 8)), ("turbidity_small", (16, 12, 8))
         o.front_location(0.01)
         o.sedimentation_rate(1, 0.02)
         o.stokes_dissipation(1, 0.02, 0.6)
+        o.set_open_boundaries(inflow=True, outflow=True, transient_inflow=True, time_c=0.1)
+        o.inflow_profile(0.05)
+        o.set_plane(1, 0, o.get_plane(0, 0))
+        o.outflow_convective(1e-3)
+        for q in range(4):
+            o.eno_conv(q)
+            o.set_rhs(q, 1e-3)
+            o.helmholtz_apply(q, 1e-3, rng.uniform(size=o.NT))
+        o.step(c["dt"])
+        o.viscous_dissipation()
+        o.div_max()
         o.set_conc_fields(fields[:1])
         o.close()
 

@@ -13,7 +13,7 @@
  *             [--in DIR] [--out DIR] [--topography FILE] [--no-offset]
  *             [--eno-order K] [--walls W0 W1 W2 W3 W4 W5] [--buoyancy B]
  *             [--particle VS] [--pois-mode M] [--pois-rtol R] [--helm-rtol R]
- *             [--output-every N]
+ *             [--output-every N] [--inflow] [--transient-inflow TC] [--outflow U]
  *
  * --in reads a restart set (Resume_*.bin + Resume_primary.dat, the
  * reference's Resume_read_data, Resume.c:92-319) into the fields; without it
@@ -22,7 +22,11 @@
  * says those heights already carry the reference's 2.001*dy offset
  * (grid->interface_position, Grid.c:913,1118).  --cfl sets each step's dt
  * from the CFL rule of tb_cfl_dt (alpha, capped at --dt-max, the reference's
- * default_dt, Input.c:421-423).  Every --output-every steps: the real
+ * default_dt, Input.c:421-423).  --inflow opens the x = 0 face: before each
+ * step the inflow profiles at the step's start time (Inflow.c:13-260; with
+ * --transient-inflow the factor 1 - exp(-(t + 0.01)/TC)); --outflow opens
+ * x = Lx with the convective condition at speed U (Outflow.c:12-304, applied
+ * by tb_step at the start of each stage).  Every --output-every steps: the real
  * pressure, the kinetic energy and suspended mass, and the field files
  * U, V, W, C, PHI as PETSc binary vectors in OUT/output_NNNNN/.  At the end
  * the restart set is written to --out.  One line per step on stdout:
@@ -45,7 +49,8 @@ static void usage(void) {
             "                 (--dt DT | --cfl ALPHA --dt-max DTMAX) [--in DIR] [--out DIR]\n"
             "                 [--topography FILE] [--no-offset] [--eno-order K] [--walls W0..W5]\n"
             "                 [--buoyancy B] [--particle VS] [--pois-mode M] [--pois-rtol R]\n"
-            "                 [--helm-rtol R] [--output-every N]\n");
+            "                 [--helm-rtol R] [--output-every N] [--inflow] [--transient-inflow TC]\n"
+            "                 [--outflow U]\n");
 }
 
 static int make_dir(const char *d) {
@@ -104,6 +109,8 @@ int main(int argc, char **argv) {
     double sc = 1.0, dt_fixed = 0.0, alpha = 0.0, dt_max = 0.0;
     int steps = -1, output_every = 0, have_grid = 0, have_box = 0, have_re = 0;
     const char *in = NULL, *out = NULL, *topo = NULL;
+    tb_open_bc obc;
+    memset(&obc, 0, sizeof(obc));
     for (int i = 1; i < argc; i++) {
         const char *a = argv[i];
         if (!strcmp(a, "--grid")) {
@@ -120,6 +127,8 @@ int main(int argc, char **argv) {
             if (need(i, argc, 6, a)) return 1;
             for (int w = 0; w < 6; w++) cfg.wall_noslip[w] = atoi(argv[i + 1 + w]);
             i += 6;
+        } else if (!strcmp(a, "--inflow")) {
+            obc.inflow = 1;
         } else if (!strcmp(a, "--no-offset")) {
             cfg.bottom_offset = 0;
         } else {
@@ -146,6 +155,8 @@ int main(int argc, char **argv) {
             else if (!strcmp(a, "--pois-rtol")) cfg.pois_rtol = atof(v);
             else if (!strcmp(a, "--helm-rtol")) cfg.helm_rtol = atof(v);
             else if (!strcmp(a, "--output-every")) output_every = atoi(v);
+            else if (!strcmp(a, "--outflow")) { obc.outflow = 1; obc.u_conv = atof(v); }
+            else if (!strcmp(a, "--transient-inflow")) { obc.transient_inflow = 1; obc.time_c = atof(v); }
             else {
                 fprintf(stderr, "tb_driver: unknown option %s\n", a);
                 usage();
@@ -190,11 +201,19 @@ int main(int argc, char **argv) {
         if (rc > 0) fprintf(stderr, "tb_driver: %s: dt or dt_old read as 0 (six-decimal record)\n", in);
     }
     if (out && make_dir(out)) goto done;
+    if ((obc.inflow || obc.outflow) && tb_set_open_boundaries(ctx, &obc) < 0) {
+        fprintf(stderr, "tb_driver: tb_set_open_boundaries: %s\n", tb_last_error(ctx));
+        goto done;
+    }
     int missed = 0;
     for (int s = 0; s < steps; s++) {
         double dt = dt_fixed;
         if (alpha > 0.0 && tb_cfl_dt(ctx, alpha, dt_max, &dt, NULL) < 0) {
             fprintf(stderr, "tb_driver: tb_cfl_dt: %s\n", tb_last_error(ctx));
+            goto done;
+        }
+        if (obc.inflow && tb_inflow_profile(ctx, info.time, NULL) < 0) {
+            fprintf(stderr, "tb_driver: tb_inflow_profile: %s\n", tb_last_error(ctx));
             goto done;
         }
         tb_step_stats st;
