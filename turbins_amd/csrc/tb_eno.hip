@@ -663,78 +663,64 @@ __global__ void __launch_bounds__(EF_THREADS) k_eno_fused2(FusedArgs A, EfTiles 
                          *Cf = &S[(k + 1) % 3][3][0];
             const double *Wbk = &S[(k + 2) % 3][2][0], *Cbk = &S[(k + 2) % 3][3][0];
             const long id = gidx(d, i, j, k);
+            // BOUNDARY statuses mark the box planes only, so none lies in the
+            // fused region: every node here takes its convective term, and
+            // the statuses and pressure gradients (read for the right-hand
+            // sides and the particle velocity) are loaded first, their
+            // latency overlapping the ENO arithmetic instead of gating it
+            const int s_u = A.su[id], s_v = A.sv[id], s_w = A.sw[id], s_c = A.sc[id];
+            double gx = 0.0, gy = 0.0, gz = 0.0;
+            if constexpr (RHS) { gx = A.dpx[id]; gy = A.dpy[id]; gz = A.dpz[id]; }
             const double uc = Uk[o], vc = Vk[o], wc = Wk[o], cc = Ck[o];
             // u node
             {
-                double r = 0.0;
-                const int s = A.su[id];
-                if (s != TB_BOUNDARY) {
-                    const double vu = interp4w(Vk[o - 1], vc, Vk[o + EF_SX - 1], Vk[o + EF_SX], wvu_x,
-                                               wvu_y);
-                    const double wu = interp4w(Wk[o - 1], wc, Wf[o - 1], Wf[o], wwu_x,
-                                               W[W_WU_Z * wm + k]);
-                    r = eno_u5(Uk[o - 2], Uk[o - 1], uc, Uk[o + 1], Uk[o + 2], rx, hx, uc);
-                    r += eno_u5(Uk[o - 2 * EF_SX], Uk[o - EF_SX], uc, Uk[o + EF_SX], Uk[o + 2 * EF_SX], ry, hy, vu);
-                    r += eno_u5(zm2[0], Ubk[o], uc, Uf[o], zp2[0], rz, hz, wu);
-                }
+                const double vu = interp4w(Vk[o - 1], vc, Vk[o + EF_SX - 1], Vk[o + EF_SX], wvu_x, wvu_y);
+                const double wu = interp4w(Wk[o - 1], wc, Wf[o - 1], Wf[o], wwu_x, W[W_WU_Z * wm + k]);
+                double r = eno_u5(Uk[o - 2], Uk[o - 1], uc, Uk[o + 1], Uk[o + 2], rx, hx, uc);
+                r += eno_u5(Uk[o - 2 * EF_SX], Uk[o - EF_SX], uc, Uk[o + EF_SX], Uk[o + 2 * EF_SX], ry, hy, vu);
+                r += eno_u5(zm2[0], Ubk[o], uc, Uf[o], zp2[0], rz, hz, wu);
                 if (!RHS || A.store_conv) st_stream(&A.cu[id], r);
-                if constexpr (RHS) st_stream(&A.ru[id], vel_row(s) ? uc * A.a_dt - r - A.dpx[id] : 0.0);
+                if constexpr (RHS) st_stream(&A.ru[id], vel_row(s_u) ? uc * A.a_dt - r - gx : 0.0);
             }
             // v node
             {
-                double r = 0.0;
-                const int s = A.sv[id];
-                if (s != TB_BOUNDARY) {
-                    const double uv = interp4w(Uk[o - EF_SX], Uk[o - EF_SX + 1], uc, Uk[o + 1], wuv_x,
-                                               wuv_y);
-                    const double wv = interp4w(Wk[o - EF_SX], Wf[o - EF_SX], wc, Wf[o], W[W_WV_Z * wm + k],
-                                               wwv_y);
-                    r = eno_u5(Vk[o - 2], Vk[o - 1], vc, Vk[o + 1], Vk[o + 2], rx, hx, uv);
-                    r += eno_u5(Vk[o - 2 * EF_SX], Vk[o - EF_SX], vc, Vk[o + EF_SX], Vk[o + 2 * EF_SX], ry, hy, vc);
-                    r += eno_u5(zm2[1], Vbk[o], vc, Vf[o], zp2[1], rz, hz, wv);
-                }
+                const double uv = interp4w(Uk[o - EF_SX], Uk[o - EF_SX + 1], uc, Uk[o + 1], wuv_x, wuv_y);
+                const double wv = interp4w(Wk[o - EF_SX], Wf[o - EF_SX], wc, Wf[o], W[W_WV_Z * wm + k], wwv_y);
+                double r = eno_u5(Vk[o - 2], Vk[o - 1], vc, Vk[o + 1], Vk[o + 2], rx, hx, uv);
+                r += eno_u5(Vk[o - 2 * EF_SX], Vk[o - EF_SX], vc, Vk[o + EF_SX], Vk[o + 2 * EF_SX], ry, hy, vc);
+                r += eno_u5(zm2[1], Vbk[o], vc, Vf[o], zp2[1], rz, hz, wv);
                 if (!RHS || A.store_conv) st_stream(&A.cv[id], r);
                 if constexpr (RHS) {
                     double rh = 0.0;
-                    if (vel_row(s)) {
+                    if (vel_row(s_v)) {
                         const double c_term = interp2d(Ck[o - EF_SX], ycm, cc, yc0, yv0);
-                        rh = vc * A.a_dt - r - A.dpy[id] - A.buoy * c_term;
+                        rh = vc * A.a_dt - r - gy - A.buoy * c_term;
                     }
                     st_stream(&A.rv[id], rh);
                 }
             }
             // w node
             {
-                double r = 0.0;
-                const int s = A.sw[id];
-                if (s != TB_BOUNDARY) {
-                    const double uw = interp4w(Ubk[o], Ubk[o + 1], uc, Uk[o + 1], wuw_x,
-                                               W[W_UW_Z * wm + k]);
-                    const double vw = interp4w(Vbk[o], vc, Vbk[o + EF_SX], Vk[o + EF_SX], W[W_VW_Z * wm + k],
-                                               wvw_y);
-                    r = eno_u5(Wk[o - 2], Wk[o - 1], wc, Wk[o + 1], Wk[o + 2], rx, hx, uw);
-                    r += eno_u5(Wk[o - 2 * EF_SX], Wk[o - EF_SX], wc, Wk[o + EF_SX], Wk[o + 2 * EF_SX], ry, hy, vw);
-                    r += eno_u5(zm2[2], Wbk[o], wc, Wf[o], zp2[2], rz, hz, wc);
-                }
+                const double uw = interp4w(Ubk[o], Ubk[o + 1], uc, Uk[o + 1], wuw_x, W[W_UW_Z * wm + k]);
+                const double vw = interp4w(Vbk[o], vc, Vbk[o + EF_SX], Vk[o + EF_SX], W[W_VW_Z * wm + k], wvw_y);
+                double r = eno_u5(Wk[o - 2], Wk[o - 1], wc, Wk[o + 1], Wk[o + 2], rx, hx, uw);
+                r += eno_u5(Wk[o - 2 * EF_SX], Wk[o - EF_SX], wc, Wk[o + EF_SX], Wk[o + 2 * EF_SX], ry, hy, vw);
+                r += eno_u5(zm2[2], Wbk[o], wc, Wf[o], zp2[2], rz, hz, wc);
                 if (!RHS || A.store_conv) st_stream(&A.cw[id], r);
-                if constexpr (RHS) st_stream(&A.rw[id], vel_row(s) ? wc * A.a_dt - r - A.dpz[id] : 0.0);
+                if constexpr (RHS) st_stream(&A.rw[id], vel_row(s_w) ? wc * A.a_dt - r - gz : 0.0);
             }
             // concentration cell
             {
-                double r = 0.0;
-                const int s = A.sc[id];
-                if (s != TB_BOUNDARY) {
-                    // Velocity_cell_center (Velocity.c:1476-1602) at this cell
-                    const double ubc = interp2d(uc, xu0, Uk[o + 1], xu1, xc0);
-                    double vbc = interp2d(vc, yv0, Vk[o + EF_SX], yv1, yc0);
-                    const double wbc = interp2d(wc, A.zw[k], Wf[o], A.zw[k + 1], A.zc[k]);
-                    if (A.particle) vbc = particle_v(vbc, s, cc, A.hindered, A.phi_max, A.v_settle);
-                    r = eno_u5(Ck[o - 2], Ck[o - 1], cc, Ck[o + 1], Ck[o + 2], rx, hx, ubc);
-                    r += eno_u5(Ck[o - 2 * EF_SX], Ck[o - EF_SX], cc, Ck[o + EF_SX], Ck[o + 2 * EF_SX], ry, hy, vbc);
-                    r += eno_u5(zm2[3], Cbk[o], cc, Cf[o], zp2[3], rz, hz, wbc);
-                }
+                // Velocity_cell_center (Velocity.c:1476-1602) at this cell
+                const double ubc = interp2d(uc, xu0, Uk[o + 1], xu1, xc0);
+                double vbc = interp2d(vc, yv0, Vk[o + EF_SX], yv1, yc0);
+                const double wbc = interp2d(wc, A.zw[k], Wf[o], A.zw[k + 1], A.zc[k]);
+                if (A.particle) vbc = particle_v(vbc, s_c, cc, A.hindered, A.phi_max, A.v_settle);
+                double r = eno_u5(Ck[o - 2], Ck[o - 1], cc, Ck[o + 1], Ck[o + 2], rx, hx, ubc);
+                r += eno_u5(Ck[o - 2 * EF_SX], Ck[o - EF_SX], cc, Ck[o + EF_SX], Ck[o + 2 * EF_SX], ry, hy, vbc);
+                r += eno_u5(zm2[3], Cbk[o], cc, Cf[o], zp2[3], rz, hz, wbc);
                 if (!RHS || A.store_conv) st_stream(&A.cc[id], r);
-                if constexpr (RHS) st_stream(&A.rc[id], s == TB_FLUID ? cc * A.a_dt - r : 0.0);
+                if constexpr (RHS) st_stream(&A.rc[id], s_c == TB_FLUID ? cc * A.a_dt - r : 0.0);
             }
             // the column's z window moves up: k-1 becomes k-2, k+3 arrives
             zm2[0] = Ubk[o]; zm2[1] = Vbk[o]; zm2[2] = Wbk[o]; zm2[3] = Cbk[o];

@@ -573,10 +573,12 @@ __global__ void __launch_bounds__(256) k_prolong(LV<T> F, LV<T> C, int cx, int c
 // values of this pair, so the result equals two half sweeps bit for bit
 // (same expression, rb_value) while xo and b cross HBM once.  ZERO: xo is
 // the zero initial guess and is not read.  The block layout is the one of
-// k_helm_gs_pair: a 64 x 8 column tile walking PAIR_TK planes through LDS
+// k_helm_gs_pair: a 64 x 16 column tile walking PAIR_TK planes through LDS
 // rings of three planes of xo (halo 2) and of first-colour values (halo 1),
-// the next plane's xo and b fetched into registers one plane ahead, two
-// barriers per plane (plane m-2 is stored after the first).
+// xo and b fetched into registers two planes ahead (every block of the
+// finest level is resident at once, so the plane-to-plane chain is the
+// latency to hide), two barriers per plane (plane m-2 is stored after the
+// first).
 constexpr int PAIR_TX = 64, PAIR_TY = 16;
 constexpr int PAIR_OX = PAIR_TX + 4, PAIR_OY = PAIR_TY + 4;
 constexpr int PAIR_HX = PAIR_TX + 2, PAIR_HY = PAIR_TY + 2;
@@ -645,17 +647,19 @@ __global__ void __launch_bounds__(PAIR_THREADS, sizeof(T) == 4 ? 4 : 2)
         const int i = i0 - 2 + cc, j = j0 - 2 + r;
         ooff[u] = (e < PAIR_OY * PAIR_OX && i >= 0 && i < nx && j >= 0 && j < ny) ? j * px + i : -1;
     }
-    T ro[PAIR_OLD_PER];
-    auto fetch_old = [&](int m) {
+    T ro[PAIR_OLD_PER], ro2[PAIR_OLD_PER];   // xo of the next two planes to stage
 #pragma unroll
-        for (int u = 0; u < PAIR_OLD_PER; u++) ro[u] = ooff[u] >= 0 ? xo[m * plane + ooff[u]] : (T)0;
+    for (int u = 0; u < PAIR_OLD_PER; u++) ro2[u] = (T)0;
+    auto fetch_old = [&](int m, T *r) {
+#pragma unroll
+        for (int u = 0; u < PAIR_OLD_PER; u++) r[u] = ooff[u] >= 0 ? xo[m * plane + ooff[u]] : (T)0;
     };
-    auto put_old = [&](int m) {
+    auto put_old = [&](int m, const T *r) {
         T *S = so[m % PAIR_ORING];
 #pragma unroll
         for (int u = 0; u < PAIR_OLD_PER; u++) {
             const int e = t + u * PAIR_THREADS;
-            if (e < PAIR_OY * PAIR_OX) S[e] = ro[u];
+            if (e < PAIR_OY * PAIR_OX) S[e] = r[u];
         }
     };
     int rrow[PAIR_RED_PER], rc2[PAIR_RED_PER];
@@ -698,23 +702,34 @@ __global__ void __launch_bounds__(PAIR_THREADS, sizeof(T) == 4 ? 4 : 2)
         }
     };
 
+    // b of the first-colour cells of planes m + 1 and m + 2 (rb: plane m),
+    // and the second-colour b of planes k + 1 and k + 2 (bb: k = m - 1)
+    T rb_n[PAIR_RED_PER];
+    bool rv_n[PAIR_RED_PER];
+#pragma unroll
+    for (int u = 0; u < PAIR_RED_PER; u++) { rb_n[u] = (T)0; rv_n[u] = false; }
+    T bb_n = (T)0;
     if (!ZERO) {
-        if (k0 >= 2) { fetch_old(k0 - 2); put_old(k0 - 2); }
-        if (k0 >= 1) { fetch_old(k0 - 1); put_old(k0 - 1); }
-        fetch_old(k0);
+        if (k0 >= 2) { fetch_old(k0 - 2, ro); put_old(k0 - 2, ro); }
+        if (k0 >= 1) { fetch_old(k0 - 1, ro); put_old(k0 - 1, ro); }
+        fetch_old(k0, ro);
+        if (k0 + 1 < nz) fetch_old(k0 + 1, ro2);
     }
     if (k0 >= 1) fetch_red(k0 - 1, rb, rv);
+    if (k0 < nz) fetch_red(k0, rb_n, rv_n);
     for (int m = k0 - 1; m <= k1; m++) {
         if (!ZERO) {
-            if (m + 1 < nz) put_old(m + 1);
-            if (m + 2 < nz) fetch_old(m + 2);
-        }
-        T rb_n[PAIR_RED_PER];
-        bool rv_n[PAIR_RED_PER];
+            if (m + 1 < nz) put_old(m + 1, ro);
 #pragma unroll
-        for (int u = 0; u < PAIR_RED_PER; u++) { rb_n[u] = (T)0; rv_n[u] = false; }
-        if (m + 1 < nz) fetch_red(m + 1, rb_n, rv_n);
-        const T bb_n = (m >= k0 && m < k1) ? fetch_black(m) : (T)0;
+            for (int u = 0; u < PAIR_OLD_PER; u++) ro[u] = ro2[u];
+            if (m + 3 < nz) fetch_old(m + 3, ro2);
+        }
+        T rb_nn[PAIR_RED_PER];
+        bool rv_nn[PAIR_RED_PER];
+#pragma unroll
+        for (int u = 0; u < PAIR_RED_PER; u++) { rb_nn[u] = (T)0; rv_nn[u] = false; }
+        if (m + 2 < nz) fetch_red(m + 2, rb_nn, rv_nn);
+        const T bb_nn = (m + 1 >= k0 && m + 1 < k1) ? fetch_black(m + 1) : (T)0;
         __syncthreads();
         write_plane(m - 2);   // final; its sr slot is next written by the first-colour pass of m + 1
         if (m >= 0 && m < nz) {
@@ -751,8 +766,12 @@ __global__ void __launch_bounds__(PAIR_THREADS, sizeof(T) == 4 ? 4 : 2)
         // no barrier: the next staging writes the xo slot of plane m - 1,
         // which the second-colour pass does not read
 #pragma unroll
-        for (int u = 0; u < PAIR_RED_PER; u++) { rb[u] = rb_n[u]; rv[u] = rv_n[u]; }
+        for (int u = 0; u < PAIR_RED_PER; u++) {
+            rb[u] = rb_n[u]; rv[u] = rv_n[u];
+            rb_n[u] = rb_nn[u]; rv_n[u] = rv_nn[u];
+        }
         bb = bb_n;
+        bb_n = bb_nn;
     }
     __syncthreads();
     write_plane(k1 - 1);
