@@ -550,9 +550,13 @@ __device__ inline double wave_uniform(double v) {
 }
 
 template <bool RHS>
-__global__ void __launch_bounds__(EF_THREADS) k_eno_fused2(FusedArgs A, EfTiles T) {
+__global__ void __launch_bounds__(EF_THREADS, 3) k_eno_fused2(FusedArgs A, EfTiles T) {
     // [ring slot][field u, v, w, c][plane with halo]
     __shared__ double S[3][4][EF_PLANE];
+    // per-plane values, the same for every thread, staged once: the z
+    // weights of the four transposes, zw and zc (vector loads of them inside
+    // the plane loop would each wait on every load issued before them)
+    __shared__ double ZK[EF_KC + 1][6];
     const Dims &d = A.d;
     const long b = blockIdx.x;
     const long lt = (b % NXCD) * T.per_xcd + b / NXCD;
@@ -564,6 +568,11 @@ __global__ void __launch_bounds__(EF_THREADS) k_eno_fused2(FusedArgs A, EfTiles 
     const int i0 = txi * EF_TX, j0 = tyi * EF_TY;
     const int k0 = EF_SHELL + kci * EF_KC;
     const int k1 = (k0 + EF_KC < d.NZ - EF_SHELL) ? k0 + EF_KC : d.NZ - EF_SHELL;
+    for (int e = threadIdx.y * EF_TX + threadIdx.x; e < 6 * (k1 - k0 + 1); e += EF_THREADS) {
+        const int m = e / 6, f = e - 6 * m, kk = k0 + m;   // kk <= k1 < NZ
+        const int tab[4] = {W_WU_Z, W_WV_Z, W_UW_Z, W_VW_Z};
+        ZK[m][f] = f < 4 ? A.W[tab[f] * A.wm + kk] : (f == 4 ? A.zw[kk] : A.zc[kk]);
+    }
     const int tx = threadIdx.x, ty = threadIdx.y, t = ty * EF_TX + tx;
     const int i = i0 + tx, j = j0 + ty;
     const bool act = i >= EF_SHELL && i < d.NX - EF_SHELL && j >= EF_SHELL && j < d.NY - EF_SHELL;
@@ -650,8 +659,6 @@ __global__ void __launch_bounds__(EF_THREADS) k_eno_fused2(FusedArgs A, EfTiles 
     stage(k0 + 1);
     if (k0 + 2 < d.NZ) fetch(k0 + 2);
     const int o = (ty + EF_H) * EF_SX + tx + EF_H;   // this thread's cell in a staged plane
-    const double *W = A.W;
-    const int wm = A.wm;
     const double hx = A.h[0], hy = A.h[1], hz = A.h[2], rx = A.rh[0], ry = A.rh[1], rz = A.rh[2];
     for (int k = k0; k < k1; k++) {
         __syncthreads();
@@ -663,6 +670,7 @@ __global__ void __launch_bounds__(EF_THREADS) k_eno_fused2(FusedArgs A, EfTiles 
                          *Cf = &S[(k + 1) % 3][3][0];
             const double *Wbk = &S[(k + 2) % 3][2][0], *Cbk = &S[(k + 2) % 3][3][0];
             const long id = gidx(d, i, j, k);
+            const double *zk = ZK[k - k0];
             // BOUNDARY statuses mark the box planes only, so none lies in the
             // fused region: every node here takes its convective term, and
             // the statuses and pressure gradients (read for the right-hand
@@ -675,7 +683,7 @@ __global__ void __launch_bounds__(EF_THREADS) k_eno_fused2(FusedArgs A, EfTiles 
             // u node
             {
                 const double vu = interp4w(Vk[o - 1], vc, Vk[o + EF_SX - 1], Vk[o + EF_SX], wvu_x, wvu_y);
-                const double wu = interp4w(Wk[o - 1], wc, Wf[o - 1], Wf[o], wwu_x, W[W_WU_Z * wm + k]);
+                const double wu = interp4w(Wk[o - 1], wc, Wf[o - 1], Wf[o], wwu_x, zk[0]);
                 double r = eno_u5(Uk[o - 2], Uk[o - 1], uc, Uk[o + 1], Uk[o + 2], rx, hx, uc);
                 r += eno_u5(Uk[o - 2 * EF_SX], Uk[o - EF_SX], uc, Uk[o + EF_SX], Uk[o + 2 * EF_SX], ry, hy, vu);
                 r += eno_u5(zm2[0], Ubk[o], uc, Uf[o], zp2[0], rz, hz, wu);
@@ -685,7 +693,7 @@ __global__ void __launch_bounds__(EF_THREADS) k_eno_fused2(FusedArgs A, EfTiles 
             // v node
             {
                 const double uv = interp4w(Uk[o - EF_SX], Uk[o - EF_SX + 1], uc, Uk[o + 1], wuv_x, wuv_y);
-                const double wv = interp4w(Wk[o - EF_SX], Wf[o - EF_SX], wc, Wf[o], W[W_WV_Z * wm + k], wwv_y);
+                const double wv = interp4w(Wk[o - EF_SX], Wf[o - EF_SX], wc, Wf[o], zk[1], wwv_y);
                 double r = eno_u5(Vk[o - 2], Vk[o - 1], vc, Vk[o + 1], Vk[o + 2], rx, hx, uv);
                 r += eno_u5(Vk[o - 2 * EF_SX], Vk[o - EF_SX], vc, Vk[o + EF_SX], Vk[o + 2 * EF_SX], ry, hy, vc);
                 r += eno_u5(zm2[1], Vbk[o], vc, Vf[o], zp2[1], rz, hz, wv);
@@ -701,8 +709,8 @@ __global__ void __launch_bounds__(EF_THREADS) k_eno_fused2(FusedArgs A, EfTiles 
             }
             // w node
             {
-                const double uw = interp4w(Ubk[o], Ubk[o + 1], uc, Uk[o + 1], wuw_x, W[W_UW_Z * wm + k]);
-                const double vw = interp4w(Vbk[o], vc, Vbk[o + EF_SX], Vk[o + EF_SX], W[W_VW_Z * wm + k], wvw_y);
+                const double uw = interp4w(Ubk[o], Ubk[o + 1], uc, Uk[o + 1], wuw_x, zk[2]);
+                const double vw = interp4w(Vbk[o], vc, Vbk[o + EF_SX], Vk[o + EF_SX], zk[3], wvw_y);
                 double r = eno_u5(Wk[o - 2], Wk[o - 1], wc, Wk[o + 1], Wk[o + 2], rx, hx, uw);
                 r += eno_u5(Wk[o - 2 * EF_SX], Wk[o - EF_SX], wc, Wk[o + EF_SX], Wk[o + 2 * EF_SX], ry, hy, vw);
                 r += eno_u5(zm2[2], Wbk[o], wc, Wf[o], zp2[2], rz, hz, wc);
@@ -714,7 +722,7 @@ __global__ void __launch_bounds__(EF_THREADS) k_eno_fused2(FusedArgs A, EfTiles 
                 // Velocity_cell_center (Velocity.c:1476-1602) at this cell
                 const double ubc = interp2d(uc, xu0, Uk[o + 1], xu1, xc0);
                 double vbc = interp2d(vc, yv0, Vk[o + EF_SX], yv1, yc0);
-                const double wbc = interp2d(wc, A.zw[k], Wf[o], A.zw[k + 1], A.zc[k]);
+                const double wbc = interp2d(wc, zk[4], Wf[o], zk[6 + 4], zk[5]);   // zw[k], zw[k+1], zc[k]
                 if (A.particle) vbc = particle_v(vbc, s_c, cc, A.hindered, A.phi_max, A.v_settle);
                 double r = eno_u5(Ck[o - 2], Ck[o - 1], cc, Ck[o + 1], Ck[o + 2], rx, hx, ubc);
                 r += eno_u5(Ck[o - 2 * EF_SX], Ck[o - EF_SX], cc, Ck[o + EF_SX], Ck[o + 2 * EF_SX], ry, hy, vbc);
