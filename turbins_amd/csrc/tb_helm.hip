@@ -123,6 +123,7 @@ __global__ void __launch_bounds__(256) k_helm_km(HelmOp H, const double *__restr
             double xp = x[id + ((k + 1 < NZ) ? d.plane : 0)];
             const double xs = x[id - os], xw = x[id - ow], xe = x[id + oe], xn = x[id + on];
             const double wv = (MODE == 1 || MODE >= 3) ? __builtin_nontemporal_load(&w[id]) : 0.0;
+            const double x0v = (MODE == 4) ? __builtin_nontemporal_load(&x0[id]) : 0.0;
             const int s = H.st[id];
             // the fluid row is formed for every node and selected afterwards:
             // no status-dependent branch sits between the loads and the math
@@ -158,7 +159,7 @@ __global__ void __launch_bounds__(256) k_helm_km(HelmOp H, const double *__restr
             } else if (MODE < 3) {
                 st_stream(&y[id], yv);
             }
-            if (MODE == 4) st_stream(&dn[id], xc - x0[id]);
+            if (MODE == 4) st_stream(&dn[id], xc - x0v);
             if (MODE == 1) acc[0] += wv * yv;
             if (MODE >= 3) { acc[0] += (wv - yv) * (wv - yv); acc[1] += wv * wv; }
             if (MODE == 2) { acc[0] += xc * yv; acc[1] += yv * yv; }

@@ -236,7 +236,11 @@ __device__ inline T rb_value(const LV<T> &L, int i, int j, int k, T bv, T xw, T 
     s = hn ? s + L.cy * xn : s; dg = hn ? dg + L.cy : dg;
     s = hb ? s + L.cz * xb : s; dg = hb ? dg + L.cz : dg;
     s = hf ? s + L.cz * xf : s; dg = hf ? dg + L.cz : dg;
-    return s / dg;
+    // the FP32 smoother (inside the FP64 refinement) takes the hardware
+    // reciprocal, within 1 ulp, one instruction instead of a correctly
+    // rounded division's ten; the FP64 one divides exactly
+    if constexpr (sizeof(T) == 4) return s * __builtin_amdgcn_rcpf(dg);
+    else return s / dg;
 }
 
 template <typename T>
