@@ -21,9 +21,9 @@ run W untimed then K timed steps from it, so they time the same stretch of
 the trajectory.  The seeded start is not solenoidal; its divergence halves
 from step to step, and over the first ~25 steps the Poisson solves start
 from large residuals that a production run (hundreds of steps) does not
-see, so the snapshot is taken after them (at 512 x 256 x 128: headline
-39.4 / plain 30.2 timesteps/s from step 4, 45.2 / 33.5 from step 30, 45.8 /
-34.1 from step 60).
+see, so the snapshot is taken after them (at 512 x 256 x 128, measured
+with earlier kernels: headline 39.4 / plain 30.2 timesteps/s from step 4,
+45.2 / 33.5 from step 30, 45.8 / 34.1 from step 60).
 
 `roofline` reports the instrumented kernel with the largest total time in
 a few further headline-path steps run with HIP events on (outside the timed
@@ -58,8 +58,10 @@ BYTES_PER_POINT = {
 WEIGHTS = os.path.join(ROOT, "turbins_amd", "weights", "poisson_guess.f32")
 # HBM bytes per instrumented launch measured with rocprofv3 FETCH_SIZE /
 # WRITE_SIZE on this workload, scaled by a plain copy of known size
-# (profiles/hbm_traffic_bump512x256.md); only valid for the kernels as built
-MEASURED_TRAFFIC = {"HELM_GS": 501.1e6}
+# (profiles/hbm_traffic_bump512x256.md); only valid for the kernels as built.
+# HELM_GS: the sweep pair, and the first pair of a solve, which also reads
+# the last solve's increment
+MEASURED_TRAFFIC = {"HELM_GS": 500.6e6, "HELM_GS_FIRST": 698.9e6}
 
 
 def log(*a):
@@ -136,6 +138,9 @@ def roofline(prof, stats, c):
     NT = (c["nx"] + 1) * (c["ny"] + 1) * (c["nz"] + 1)
     ncell = c["nx"] * c["ny"] * c["nz"]
     helm_its = sum(h.iterations for st in stats for stage in st.helm for h in stage)
+    # solves that swept (in the steady stepping measured here each starts
+    # from the field plus its last increment)
+    n_first = sum(1 for st in stats for stage in st.helm for h in stage if h.iterations > 0 and not h.fallback)
     best = None
     for name, (ms, n) in prof.items():
         if name in BYTES_PER_POINT:
@@ -146,8 +151,9 @@ def roofline(prof, stats, c):
             total = 17 * NT * n + 8 * NT * helm_its
         elif name == "HELM_GS":
             # one fused red+black sweep pair: x, b and the status byte read
-            # once, the new x written once
-            total = 25 * NT * n
+            # once, the new x written once; the first pair of each solve also
+            # reads the last solve's increment (8 B/point)
+            total = 25 * NT * n + 8 * NT * n_first
         elif name == "RBGS0":
             total = 24 * ncell * n      # FP64 fused pair: x, b read, x written
         elif name == "RBGS0_F32":
@@ -163,6 +169,8 @@ def roofline(prof, stats, c):
     avg_s = ms / n * 1e-3
     achieved = per_launch / avg_s / 1e9
     traffic = MEASURED_TRAFFIC.get(name) if c["nx"] == 512 and c["ny"] == 256 and c["nz"] == 128 else None
+    if name == "HELM_GS" and traffic is not None:
+        traffic = ((n - n_first) * traffic + n_first * MEASURED_TRAFFIC["HELM_GS_FIRST"]) / n
     return {"kernel": name, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
             "bytes_per_launch": int(per_launch), "avg_launch_us": round(avg_s * 1e6, 2), "launches": n}
