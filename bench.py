@@ -322,6 +322,8 @@ def main():
     log("device ms per step: plain", phase_ms(st_plain), "headline", phase_ms(st_head))
     for name, (ms, n) in sorted(prof.items(), key=lambda kv: -kv[1][0]):
         log(f"  {name:10s} {n:6d} launches  {ms:9.2f} ms  {1e3 * ms / n:8.1f} us/launch")
+    covered = sum(ms for ms, _ in prof.values()) / len(st_prof)
+    log(f"  instrumented kernels: {covered:.2f} ms per step of {1e3 / head_sps:.2f} ms wall per timed step")
 
     rl = roofline(prof, st_prof, c)
     result = {

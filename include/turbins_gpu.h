@@ -599,6 +599,11 @@ enum {
     TB_K_ENO_MOM = 0, TB_K_ENO_CONC, TB_K_HELM_OP, TB_K_BICG_UPD, TB_K_POIS_OP,
     TB_K_RBGS0, TB_K_RBGS0_F32, TB_K_PCG_UPD, TB_K_PROJECT, TB_K_RHS, TB_K_NET,
     TB_K_CG32_OP, TB_K_CG32_UPD, TB_K_HELM_GS,
+    TB_K_MG_LEVELS,      /* sweeps of the multigrid levels below the finest */
+    TB_K_MG_TRANSFER,    /* residual + restriction and prolongation, every level */
+    TB_K_MG_SMALL,       /* the one-workgroup cycle of the small levels, or the coarse sweeps */
+    TB_K_GUESS,          /* the Galerkin guess's dot products and combination */
+    TB_K_IB,             /* the immersed rows' Jacobi updates and scaled applications */
     TB_K_COUNT
 };
 int tb_profile_enable(tb_ctx *ctx, int on);

@@ -9,7 +9,8 @@ HELM_GS, HELM_BICGSTAB = 0, 1
 REINIT_SCALED, REINIT_REFERENCE = 0, 1
 
 KERNELS = ["ENO_MOM", "ENO_CONC", "HELM_OP", "BICG_UPD", "POIS_OP", "RBGS0", "RBGS0_F32",
-           "PCG_UPD", "PROJECT", "RHS", "NET", "CG32_OP", "CG32_UPD", "HELM_GS"]
+           "PCG_UPD", "PROJECT", "RHS", "NET", "CG32_OP", "CG32_UPD", "HELM_GS",
+           "MG_LEVELS", "MG_TRANSFER", "MG_SMALL", "GUESS", "IB"]
 K = {name: i for i, name in enumerate(KERNELS)}
 
 FIELDS = [

@@ -682,10 +682,12 @@ static int gs_pairs(tb_ctx *c, const HelmOp &H, double *&cur, double *&alt, cons
         alt = w;
         if (c->nib[q] > 0) {
             const dim3 gi((c->nib[q] + 255) / 256);
+            if (c->prof_on) tb_prof_begin(c, TB_K_IB);
             hipLaunchKernelGGL(k_ib_gs, gi, dim3(256), 0, c->s, c->ib_row[q], c->ib_cnt[q], c->ib_col[q],
                                c->ib_val[q], c->nib[q], (const double *)cur, b, tmp);
             hipLaunchKernelGGL(k_ib_scatter, gi, dim3(256), 0, c->s, c->ib_row[q], c->nib[q], (const double *)tmp,
                                cur);
+            if (c->prof_on) tb_prof_end(c, TB_K_IB);
         }
         TB_LAUNCH_CHECK(c);
     }

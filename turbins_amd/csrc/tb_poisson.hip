@@ -1218,8 +1218,9 @@ static int vcycle_level(tb_ctx *c, int l) {
     if (l == c->nlev - 1 && !c->coarse_dense) {
         const Tiles half = make_tiles((L.nx + 1) / 2, L.ny, L.nz);
         auto sweep = [&](int color, int zero_init) {
-            hipLaunchKernelGGL(k_rbgs<T>, tiles_grid(half), tiles_block(), 0, c->s, lv, B.x, (const T *)B.b, color,
-                               zero_init, half);
+            TB_PROF(c, TB_K_MG_SMALL,
+                    hipLaunchKernelGGL(k_rbgs<T>, tiles_grid(half), tiles_block(), 0, c->s, lv, B.x, (const T *)B.b,
+                                       color, zero_init, half));
         };
         sweep(0, 1);
         sweep(1, 0);
@@ -1257,13 +1258,14 @@ static int vcycle_level(tb_ctx *c, int l) {
             A.cx[m] = Lm.cx; A.cy[m] = Lm.cy; A.cz[m] = Lm.cz;
         }
         A.ro = off;
-        hipLaunchKernelGGL(k_small_cycle<T>, dim3(1), dim3(1024), small_lds_bytes<T>(c, l), c->s, A);
+        TB_PROF(c, TB_K_MG_SMALL,
+                hipLaunchKernelGGL(k_small_cycle<T>, dim3(1), dim3(1024), small_lds_bytes<T>(c, l), c->s, A));
         TB_LAUNCH_CHECK(c);
         return 0;
     }
     Tiles all = make_tiles(L.nx, L.ny, L.nz);
     Tiles half = make_tiles((L.nx + 1) / 2, L.ny, L.nz);
-    const int kid = (l == 0) ? (sizeof(T) == 8 ? TB_K_RBGS0 : TB_K_RBGS0_F32) : -1;
+    const int kid = (l == 0) ? (sizeof(T) == 8 ? TB_K_RBGS0 : TB_K_RBGS0_F32) : TB_K_MG_LEVELS;
     const int prec = sizeof(T) == 8 ? 1 : 0;
     const int pre = (l == 0 && c->mg_pre0[prec] > 0) ? c->mg_pre0[prec] : c->cfg.mg_pre;
     const int post = (l == 0 && c->mg_post0[prec] > 0) ? c->mg_post0[prec] : c->cfg.mg_post;
@@ -1291,7 +1293,7 @@ static int vcycle_level(tb_ctx *c, int l) {
         } else {
             hipLaunchKernelGGL(k_zero<T>, tiles_grid(all), tiles_block(), 0, c->s, lv, B.x, all);
         }
-        resid_restrict<T>(c, l, (const T *)cur);
+        TB_PROF(c, TB_K_MG_TRANSFER, resid_restrict<T>(c, l, (const T *)cur));
         const Level &Cl = c->lev[l + 1];
         MGBuf<T> CB = bufs<T>(c, l + 1);
         LV<T> cv = lv_of<T>(Cl);
@@ -1300,7 +1302,7 @@ static int vcycle_level(tb_ctx *c, int l) {
         // the post pairs alternate buffers, so the prolonged iterate goes to
         // the buffer from which they end in x
         T *dst = (post % 2 == 0) ? B.x : B.a;
-        prolong<T>(c, l, (const T *)cur, dst);
+        TB_PROF(c, TB_K_MG_TRANSFER, prolong<T>(c, l, (const T *)cur, dst));
         cur = dst;
         // on the finest level the last pair also forms the partial sums of
         // r . z (b . x) for the Krylov method around the cycle
@@ -1330,13 +1332,13 @@ static int vcycle_level(tb_ctx *c, int l) {
     } else {
         hipLaunchKernelGGL(k_zero<T>, tiles_grid(all), tiles_block(), 0, c->s, lv, B.x, all);
     }
-    resid_restrict<T>(c, l, (const T *)B.x);
+    TB_PROF(c, TB_K_MG_TRANSFER, resid_restrict<T>(c, l, (const T *)B.x));
     const Level &Cl = c->lev[l + 1];
     MGBuf<T> CB = bufs<T>(c, l + 1);
     LV<T> cv = lv_of<T>(Cl);
     TB_LAUNCH_CHECK(c);
     if (vcycle_level<T>(c, l + 1)) return -1;
-    prolong<T>(c, l, (const T *)B.x, B.x);
+    TB_PROF(c, TB_K_MG_TRANSFER, prolong<T>(c, l, (const T *)B.x, B.x));
     for (int s = 0; s < post; s++) {
         smooth(1, 0);
         smooth(0, 0);
@@ -1955,6 +1957,7 @@ static int learned_guess(tb_ctx *c, const PoisCtx &P, double *x, const double *b
         TB_LAUNCH_CHECK(c);
     }
     const double *dp = use_d ? dd : nullptr, *qp = use_d ? q : nullptr;
+    if (c->prof_on) tb_prof_begin(c, TB_K_GUESS);
     switch (H) {
     case 1: launch_dots<1>(c, n, D, b, dp, qp); break;
     case 2: launch_dots<2>(c, n, D, b, dp, qp); break;
@@ -1972,6 +1975,7 @@ static int learned_guess(tb_ctx *c, const PoisCtx &P, double *x, const double *b
     const int ns = use_d ? 2 * H + 2 : H;
     hipLaunchKernelGGL(k_sum_cols, dim3(ns), dim3(SUM_COLS_THREADS), 0, c->s, (const double *)c->red, PB, ns,
                        c->scal + S_SUB);
+    if (c->prof_on) tb_prof_end(c, TB_K_GUESS);
     TB_LAUNCH_CHECK(c);
     if (read_s(c, S_SUB, ns)) return -1;
     const double *h = c->hscal + S_SUB;
@@ -2009,7 +2013,7 @@ static int learned_guess(tb_ctx *c, const PoisCtx &P, double *x, const double *b
     }
     Coefs C;
     for (int k = 0; k < GUESS_DIRS; k++) C.a[k] = k < ndir ? a[k] : 0.0;
-    launch_combine(c, n, x, D, with_d ? H : ndir, with_d ? (const double *)dd : nullptr, C);
+    TB_PROF(c, TB_K_GUESS, launch_combine(c, n, x, D, with_d ? H : ndir, with_d ? (const double *)dd : nullptr, C));
     TB_LAUNCH_CHECK(c);
     // Every stored solution is pinned (each solve leaves its x with corner
     // exactly 0 and half-cell planes 0), so a combination of them is too and
