@@ -94,6 +94,7 @@ def lib():
     L.tb_set_helmholtz_method.argtypes = [P, C.c_int]
     L.tb_set_graphs.argtypes = [P, C.c_int]
     L.tb_set_mg_finest_sweeps.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_int]
+    L.tb_set_mg_smoother.argtypes = [P, C.c_int]
     L.tb_petsc_write_array.argtypes = [C.c_char_p, P, C.c_int64]
     L.tb_petsc_read_array.restype = C.c_int64
     L.tb_petsc_read_array.argtypes = [C.c_char_p, P, C.c_int64]
@@ -530,6 +531,12 @@ class Solver:
         config's mg_pre / mg_post, which the coarser levels keep)."""
         return self._ok(self.L.tb_set_mg_finest_sweeps(self.h, pre32, post32, pre64, post64),
                         "tb_set_mg_finest_sweeps")
+
+    def set_mg_smoother(self, blocked=True):
+        """Smoother of the levels between the finest and the LDS cycle: one
+        blocked launch per V-cycle side (default) or one launch per half
+        sweep; both give the same bits."""
+        return self._ok(self.L.tb_set_mg_smoother(self.h, 1 if blocked else 0), "tb_set_mg_smoother")
 
     def profile(self, on=True):
         return self._ok(self.L.tb_profile_enable(self.h, 1 if on else 0), "tb_profile_enable")

@@ -349,6 +349,7 @@ struct tb_ctx {
     int vc_dot_blocks[2] = {0, 0};
     int mg_pre0[2] = {0, 0};     // sweeps on the finest level per V-cycle side (FP32, FP64);
     int mg_post0[2] = {0, 0};    // 0: cfg.mg_pre / cfg.mg_post as on the coarser levels
+    bool mg_blocked = true;      // mid levels smoothed by one blocked launch per V-cycle side
     int coarse_n = 0;
     bool coarse_dense = true;    // coarsest level solved by its dense inverse (<= 1024 cells), else by sweeps
     int small_l0[2] = {0, 0};   // first level of the one-workgroup cycle: FP32, FP64

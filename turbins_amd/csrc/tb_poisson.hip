@@ -243,10 +243,11 @@ __device__ inline T rb_value(const LV<T> &L, int i, int j, int k, T bv, T xw, T 
     else return s / dg;
 }
 
+// the Gauss-Seidel value of cell (i, j, k) from x and b (id: its index)
 template <typename T>
-__device__ inline void rbgs_cell(const LV<T> &L, T *__restrict__ x, const T *__restrict__ b, int i, int j, int k,
-                                 bool zero_init) {
-    const long id = lidx(L, i, j, k);
+__device__ inline T rbgs_value(const LV<T> &L, const T *x, const T *b, int i, int j, int k, bool zero_init,
+                               long &id) {
+    id = lidx(L, i, j, k);
     const long pl = (long)L.px * L.py;
     const bool hw = i > 0, he = i < L.nx - 1, hs = j > 0, hn = j < L.ny - 1, hb = k > 0, hf = k < L.nz - 1;
     T xw = (T)0, xe = (T)0, xs = (T)0, xn = (T)0, xb = (T)0, xf = (T)0;
@@ -256,13 +257,21 @@ __device__ inline void rbgs_cell(const LV<T> &L, T *__restrict__ x, const T *__r
         xs = x[id - (hs ? L.px : 0)]; xn = x[id + (hn ? L.px : 0)];
         xb = x[id - (hb ? pl : 0)]; xf = x[id + (hf ? pl : 0)];
     }
-    x[id] = rb_value(L, i, j, k, b[id], xw, xe, xs, xn, xb, xf);
+    return rb_value(L, i, j, k, b[id], xw, xe, xs, xn, xb, xf);
 }
 
 template <typename T>
-__device__ inline void resid_cell(const LV<T> &L, const T *__restrict__ x, const T *__restrict__ b,
-                                  T *__restrict__ r, int i, int j, int k) {
-    const long id = lidx(L, i, j, k);
+__device__ inline void rbgs_cell(const LV<T> &L, T *__restrict__ x, const T *__restrict__ b, int i, int j, int k,
+                                 bool zero_init) {
+    long id;
+    const T v = rbgs_value(L, (const T *)x, b, i, j, k, zero_init, id);
+    x[id] = v;
+}
+
+// b - A x at cell (i, j, k) (id: its index)
+template <typename T>
+__device__ inline T resid_value(const LV<T> &L, const T *x, const T *b, int i, int j, int k, long &id) {
+    id = lidx(L, i, j, k);
     const long pl = (long)L.px * L.py;
     const bool hw = i > 0, he = i < L.nx - 1, hs = j > 0, hn = j < L.ny - 1, hb = k > 0, hf = k < L.nz - 1;
     // absent neighbours are clamped to the cell itself, contributing xc - xc = 0
@@ -273,7 +282,15 @@ __device__ inline void resid_cell(const LV<T> &L, const T *__restrict__ x, const
     const T bb = b[id];
     T s = L.cx * (xc - xw) + L.cx * (xc - xe) + L.cy * (xc - xs) + L.cy * (xc - xn) + L.cz * (xc - xb) +
           L.cz * (xc - xf);
-    r[id] = bb - s;
+    return bb - s;
+}
+
+template <typename T>
+__device__ inline void resid_cell(const LV<T> &L, const T *__restrict__ x, const T *__restrict__ b,
+                                  T *__restrict__ r, int i, int j, int k) {
+    long id;
+    const T v = resid_value(L, x, b, i, j, k, id);
+    r[id] = v;
 }
 
 __device__ inline int clampi(int a, int lo, int hi) { return a < lo ? lo : (a > hi ? hi : a); }
@@ -281,8 +298,7 @@ __device__ inline int clampi(int a, int lo, int hi) { return a < lo ? lo : (a > 
 // restriction: adjoint of the prolongation / 2 per halved axis,
 // 1-D weights (1/8, 3/8, 3/8, 1/8) with reflection at the walls
 template <typename T>
-__device__ inline void restrict_cell(const LV<T> &F, const LV<T> &C, int cx, int cy, int cz, const T *__restrict__ rf,
-                                     T *__restrict__ bc, int I, int J, int K) {
+__device__ inline T restrict_value(const LV<T> &F, int cx, int cy, int cz, const T *rf, int I, int J, int K) {
     const T W[4] = {(T)0.125, (T)0.375, (T)0.375, (T)0.125};
     T s = (T)0;
     int tk0 = cz ? -1 : 0, tk1 = cz ? 2 : 0;
@@ -303,24 +319,65 @@ __device__ inline void restrict_cell(const LV<T> &F, const LV<T> &C, int cx, int
             s += wk * wj * sj;
         }
     }
+    return s;
+}
+
+template <typename T>
+__device__ inline void restrict_cell(const LV<T> &F, const LV<T> &C, int cx, int cy, int cz, const T *__restrict__ rf,
+                                     T *__restrict__ bc, int I, int J, int K) {
+    const T s = restrict_value(F, cx, cy, cz, rf, I, J, K);
     bc[lidx(C, I, J, K)] = s;
 }
 
-// prolongation: (3/4, 1/4) per halved axis, reflecting; x_f = x_in + P x_c
-// (x_in may be x_f itself)
-template <typename T>
-__device__ inline void prolong_cell(const LV<T> &F, const LV<T> &C, int cx, int cy, int cz, const T *__restrict__ xc,
-                                    const T *xin, T *xf, int i, int j, int k) {
+// restriction with the halved axes fixed at compile time: the 4 x 4 x 4
+// taps unroll, so their loads are issued together; same sums in the same
+// order as restrict_cell, so the same bits
+template <typename T, int CX, int CY, int CZ>
+__device__ inline T restrict_sum(const LV<T> &F, const T *__restrict__ rf, int I, int J, int K) {
+    const T W[4] = {(T)0.125, (T)0.375, (T)0.375, (T)0.125};
+    T s = (T)0;
+#pragma unroll
+    for (int tk = (CZ ? -1 : 0); tk <= (CZ ? 2 : 0); tk++) {
+        const int kf = CZ ? clampi(2 * K + tk, 0, F.nz - 1) : K;
+        const T wk = CZ ? W[tk + 1] : (T)1;
+#pragma unroll
+        for (int tj = (CY ? -1 : 0); tj <= (CY ? 2 : 0); tj++) {
+            const int jf = CY ? clampi(2 * J + tj, 0, F.ny - 1) : J;
+            const T wj = CY ? W[tj + 1] : (T)1;
+            T sj = (T)0;
+#pragma unroll
+            for (int ti = (CX ? -1 : 0); ti <= (CX ? 2 : 0); ti++) {
+                const int ifn = CX ? clampi(2 * I + ti, 0, F.nx - 1) : I;
+                const T wi = CX ? W[ti + 1] : (T)1;
+                sj += wi * rf[lidx(F, ifn, jf, kf)];
+            }
+            s += wk * wj * sj;
+        }
+    }
+    return s;
+}
+
+// prolongation: (3/4, 1/4) per halved axis, reflecting: the correction
+// P x_c at fine cell (i, j, k), coarse values from xcv(I, J, K)
+template <typename T, typename G>
+__device__ inline T prolong_sum(const LV<T> &C, int cx, int cy, int cz, int i, int j, int k, G xcv) {
     int K0, K1, J0, J1, I0, I1;
     T wk1, wj1, wi1;
     if (cz) { K0 = k >> 1; K1 = clampi((k & 1) ? K0 + 1 : K0 - 1, 0, C.nz - 1); wk1 = (T)0.25; } else { K0 = K1 = k; wk1 = (T)0; }
     if (cy) { J0 = j >> 1; J1 = clampi((j & 1) ? J0 + 1 : J0 - 1, 0, C.ny - 1); wj1 = (T)0.25; } else { J0 = J1 = j; wj1 = (T)0; }
     if (cx) { I0 = i >> 1; I1 = clampi((i & 1) ? I0 + 1 : I0 - 1, 0, C.nx - 1); wi1 = (T)0.25; } else { I0 = I1 = i; wi1 = (T)0; }
     T wk0 = (T)1 - wk1, wj0 = (T)1 - wj1, wi0 = (T)1 - wi1;
-    T s = wk0 * (wj0 * (wi0 * xc[lidx(C, I0, J0, K0)] + wi1 * xc[lidx(C, I1, J0, K0)]) +
-                 wj1 * (wi0 * xc[lidx(C, I0, J1, K0)] + wi1 * xc[lidx(C, I1, J1, K0)])) +
-          wk1 * (wj0 * (wi0 * xc[lidx(C, I0, J0, K1)] + wi1 * xc[lidx(C, I1, J0, K1)]) +
-                 wj1 * (wi0 * xc[lidx(C, I0, J1, K1)] + wi1 * xc[lidx(C, I1, J1, K1)]));
+    return wk0 * (wj0 * (wi0 * xcv(I0, J0, K0) + wi1 * xcv(I1, J0, K0)) +
+                  wj1 * (wi0 * xcv(I0, J1, K0) + wi1 * xcv(I1, J1, K0))) +
+           wk1 * (wj0 * (wi0 * xcv(I0, J0, K1) + wi1 * xcv(I1, J0, K1)) +
+                  wj1 * (wi0 * xcv(I0, J1, K1) + wi1 * xcv(I1, J1, K1)));
+}
+
+// x_f = x_in + P x_c (x_in may be x_f itself)
+template <typename T>
+__device__ inline void prolong_cell(const LV<T> &F, const LV<T> &C, int cx, int cy, int cz, const T *__restrict__ xc,
+                                    const T *xin, T *xf, int i, int j, int k) {
+    const T s = prolong_sum(C, cx, cy, cz, i, j, k, [&](int I, int J, int K) { return xc[lidx(C, I, J, K)]; });
     const long id = lidx(F, i, j, k);
     xf[id] = xin[id] + s;
 }
@@ -379,6 +436,14 @@ __global__ void __launch_bounds__(256) k_restrict(LV<T> F, LV<T> C, int cx, int 
     int I, J, K;
     if (!tile_point(t, I, J, K)) return;
     restrict_cell(F, C, cx, cy, cz, rf, bc, I, J, K);
+}
+
+template <typename T, int CX, int CY, int CZ>
+__global__ void __launch_bounds__(256) k_restrict_t(LV<T> F, LV<T> C, const T *__restrict__ rf, T *__restrict__ bc,
+                                                    Tiles t) {
+    int I, J, K;
+    if (!tile_point(t, I, J, K)) return;
+    bc[lidx(C, I, J, K)] = restrict_sum<T, CX, CY, CZ>(F, rf, I, J, K);
 }
 
 // Residual and restriction in one pass for the coarsening pattern of the
@@ -921,6 +986,162 @@ __global__ void __launch_bounds__(1024) k_small_cycle(SmallCycle<T> A) {
     }
 }
 
+// Mid-level smoothing in one launch.  Each block owns an output box of the
+// level and runs NP red-black sweep pairs on it in LDS, recomputing the
+// 2*NP-cell halo its last half sweep depends on, so no block reads what
+// another writes: src and dst are distinct buffers.  Half sweep m = 1 ..
+// 2*NP has colour FIRST for odd m and 1 - FIRST for even m and covers the
+// box grown by 2*NP - m cells, which is every cell a later half sweep of the
+// box reads.  ZERO: the sweeps start from x = 0 and read only b (the
+// pre-smoother).  PROL: the starting iterate is src + P xc, the coarse
+// level's correction prolonged on the fly (the post-smoother).  Every cell
+// value is formed by the expressions of rb_value and prolong_sum, so the
+// result equals the launch-per-half-sweep sequence bit for bit.
+constexpr int BLK_THREADS = 1024;
+constexpr int BLK_LDS_PAIR = 80 * 1024;      // two blocks per CU
+constexpr int BLK_LDS_BYTES = 150 * 1024;    // one block per CU
+constexpr int BLK_MIN_BOXES = 128;
+// Above this many cells the halo recomputation (about 5x the cells of a box
+// for two pairs) makes the blocked launch slower than its half sweeps,
+// whose ~5 us launches it exists to save (measured at 512x256x128: the
+// 1M-cell level 62 us blocked against 47, the 65k-cell level 17 against 42).
+constexpr long BLK_MAX_CELLS = 1L << 18;
+
+struct BlkShape {
+    int bx, by, bz;          // output box
+    int nbx, nby, nb;        // boxes along x, along y, in total
+    int per_xcd;             // ceil(nb / 8)
+    int cnx, cny, cnz;       // extents of the LDS copy of the coarse region (PROL)
+};
+
+template <typename T, int NP, bool ZERO, bool PROL, int FIRST>
+__global__ void __launch_bounds__(BLK_THREADS) k_rbgs_blk(LV<T> L, const T *__restrict__ src, const T *__restrict__ b,
+                                                          T *__restrict__ dst, BlkShape S, LV<T> C, int cx, int cy,
+                                                          int cz, const T *__restrict__ xc) {
+    extern __shared__ double smem_blk[];
+    constexpr int H = 2 * NP;
+    const int rx = S.bx + 2 * H, ry = S.by + 2 * H, rz = S.bz + 2 * H;
+    const int rn = rx * ry * rz, rpl = rx * ry;
+    T *xs = reinterpret_cast<T *>(smem_blk);
+    T *bs = xs + rn;
+    T *cs = bs + rn;
+    // consecutive boxes (which share halos) on one XCD
+    const int lt = (int)(blockIdx.x % NXCD) * S.per_xcd + (int)(blockIdx.x / NXCD);
+    if (lt >= S.nb) return;
+    const int ox = (lt % S.nbx) * S.bx, oy = ((lt / S.nbx) % S.nby) * S.by, oz = (lt / (S.nbx * S.nby)) * S.bz;
+    const int gx = ox - H, gy = oy - H, gz = oz - H;     // level index of LDS cell (0, 0, 0)
+    const int x0 = max(gx, 0), x1 = min(gx + rx, L.nx);
+    const int y0 = max(gy, 0), y1 = min(gy + ry, L.ny);
+    const int z0 = max(gz, 0), z1 = min(gz + rz, L.nz);
+    int ci = 0, cj = 0, ck = 0;
+    if (PROL) {
+        // the coarse cells the region's prolongation reads
+        ci = cx ? max((x0 >> 1) - 1, 0) : x0;
+        cj = cy ? max((y0 >> 1) - 1, 0) : y0;
+        ck = cz ? max((z0 >> 1) - 1, 0) : z0;
+        const int ci1 = cx ? min(((x1 - 1) >> 1) + 1, C.nx - 1) : x1 - 1;
+        const int cj1 = cy ? min(((y1 - 1) >> 1) + 1, C.ny - 1) : y1 - 1;
+        const int ck1 = cz ? min(((z1 - 1) >> 1) + 1, C.nz - 1) : z1 - 1;
+        block_cells(ci1 - ci + 1, cj1 - cj + 1, ck1 - ck + 1, [&](int a, int e, int f) {
+            cs[(f * S.cny + e) * S.cnx + a] = xc[lidx(C, ci + a, cj + e, ck + f)];
+        });
+        __syncthreads();
+    }
+    block_cells(x1 - x0, y1 - y0, z1 - z0, [&](int a, int e, int f) {
+        const int i = x0 + a, j = y0 + e, k = z0 + f;
+        const int q = ((k - gz) * ry + (j - gy)) * rx + (i - gx);
+        const long g = lidx(L, i, j, k);
+        bs[q] = b[g];
+        if (!ZERO) {
+            T v = src[g];
+            if (PROL) {
+                const T p = prolong_sum(C, cx, cy, cz, i, j, k, [&](int I, int J, int K) {
+                    return cs[((K - ck) * S.cny + (J - cj)) * S.cnx + (I - ci)];
+                });
+                v = v + p;
+            }
+            xs[q] = v;
+        }
+    });
+    __syncthreads();
+#pragma unroll
+    for (int m = 1; m <= 2 * NP; m++) {
+        const int color = (m & 1) ? FIRST : 1 - FIRST;
+        const bool zero_init = ZERO && m == 1;
+        const int e = 2 * NP - m;
+        const int ax0 = max(ox - e, 0), ax1 = min(ox + S.bx + e, L.nx);
+        const int ay0 = max(oy - e, 0), ay1 = min(oy + S.by + e, L.ny);
+        const int az0 = max(oz - e, 0), az1 = min(oz + S.bz + e, L.nz);
+        block_cells((ax1 - ax0 + 1) / 2, ay1 - ay0, az1 - az0, [&](int ih, int e2, int f) {
+            const int j = ay0 + e2, k = az0 + f;
+            const int i = ax0 + 2 * ih + ((ax0 + j + k + color) & 1);
+            if (i >= ax1) return;
+            const int q = ((k - gz) * ry + (j - gy)) * rx + (i - gx);
+            const bool hw = i > 0, he = i < L.nx - 1, hs = j > 0, hn = j < L.ny - 1, hb = k > 0, hf = k < L.nz - 1;
+            T xw = (T)0, xe = (T)0, xsv = (T)0, xn = (T)0, xb = (T)0, xf = (T)0;
+            if (!zero_init) {
+                xw = xs[q - (hw ? 1 : 0)]; xe = xs[q + (he ? 1 : 0)];
+                xsv = xs[q - (hs ? rx : 0)]; xn = xs[q + (hn ? rx : 0)];
+                xb = xs[q - (hb ? rpl : 0)]; xf = xs[q + (hf ? rpl : 0)];
+            }
+            xs[q] = rb_value(L, i, j, k, bs[q], xw, xe, xsv, xn, xb, xf);
+        });
+        __syncthreads();
+    }
+    block_cells(min(S.bx, L.nx - ox), min(S.by, L.ny - oy), min(S.bz, L.nz - oz), [&](int a, int e, int f) {
+        const int i = ox + a, j = oy + e, k = oz + f;
+        dst[lidx(L, i, j, k)] = xs[((k - gz) * ry + (j - gy)) * rx + (i - gx)];
+    });
+}
+
+// LDS bytes of k_rbgs_blk for box (bx, by, bz) on level L (coarsened by its
+// cx, cy, cz): x and b over the box grown by 2*np, and the coarse region of
+// the prolongation (a grown extent r covers at most (r + 1) / 2 + 3 coarse
+// cells along a halved axis)
+static long blk_lds(const Level &L, int np, size_t tsize, const int *bx, int *cn) {
+    const int h = 2 * np;
+    const int r[3] = {bx[0] + 2 * h, bx[1] + 2 * h, bx[2] + 2 * h};
+    const int c[3] = {L.cx, L.cy, L.cz};
+    for (int a = 0; a < 3; a++) cn[a] = c[a] ? (r[a] + 1) / 2 + 3 : r[a];
+    return (long)tsize * (2L * r[0] * r[1] * r[2] + (long)cn[0] * cn[1] * cn[2]);
+}
+
+// box of the mid-level smoother: start from 32 x 16 x 8 and halve the
+// largest side (not below 8) until the LDS fits two blocks per CU (else one)
+// and there are at least BLK_MIN_BOXES boxes; false if no box fits
+static bool blk_shape_cap(const Level &L, int np, size_t tsize, long cap, BlkShape &S, int &lds);
+static bool blk_shape(const Level &L, int np, size_t tsize, BlkShape &S, int &lds) {
+    return blk_shape_cap(L, np, tsize, BLK_LDS_PAIR, S, lds) || blk_shape_cap(L, np, tsize, BLK_LDS_BYTES, S, lds);
+}
+static bool blk_shape_cap(const Level &L, int np, size_t tsize, long cap, BlkShape &S, int &lds) {
+    int bx[3] = {32, 16, 8}, cn[3];
+    const int n[3] = {L.nx, L.ny, L.nz};
+    auto boxes = [&]() {
+        return (long)((n[0] + bx[0] - 1) / bx[0]) * ((n[1] + bx[1] - 1) / bx[1]) * ((n[2] + bx[2] - 1) / bx[2]);
+    };
+    for (;;) {
+        const bool big = blk_lds(L, np, tsize, bx, cn) > cap;
+        if (!big && boxes() >= BLK_MIN_BOXES) break;
+        int a = 0;   // the largest side, the later axis on a tie (long x rows)
+        for (int q = 1; q < 3; q++)
+            if (bx[q] >= bx[a]) a = q;
+        if (bx[a] <= 8) {
+            if (big) return false;
+            break;
+        }
+        bx[a] /= 2;
+    }
+    const long bytes = blk_lds(L, np, tsize, bx, cn);
+    S.bx = bx[0]; S.by = bx[1]; S.bz = bx[2];
+    S.nbx = (n[0] + bx[0] - 1) / bx[0];
+    S.nby = (n[1] + bx[1] - 1) / bx[1];
+    S.nb = (int)boxes();
+    S.per_xcd = (S.nb + NXCD - 1) / NXCD;
+    S.cnx = cn[0]; S.cny = cn[1]; S.cnz = cn[2];
+    lds = (int)bytes;
+    return true;
+}
+
 template <typename T>
 static LV<T> lv_of(const Level &L) {
     LV<T> v;
@@ -1075,6 +1296,15 @@ int tb_mg_build(tb_ctx *c) {
                                     SMALL_LDS_BYTES));
     TB_CHECK(c, hipFuncSetAttribute((const void *)k_small_cycle<double>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     SMALL_LDS_BYTES));
+    {
+        const void *blk[8] = {
+            (const void *)k_rbgs_blk<float, 1, true, false, 0>, (const void *)k_rbgs_blk<float, 2, true, false, 0>,
+            (const void *)k_rbgs_blk<float, 1, false, true, 1>, (const void *)k_rbgs_blk<float, 2, false, true, 1>,
+            (const void *)k_rbgs_blk<double, 1, true, false, 0>, (const void *)k_rbgs_blk<double, 2, true, false, 0>,
+            (const void *)k_rbgs_blk<double, 1, false, true, 1>, (const void *)k_rbgs_blk<double, 2, false, true, 1>};
+        for (int a = 0; a < 8; a++)
+            TB_CHECK(c, hipFuncSetAttribute(blk[a], hipFuncAttributeMaxDynamicSharedMemorySize, BLK_LDS_BYTES));
+    }
     if (!c->coarse_dense) {
         TB_CHECK(c, hipStreamSynchronize(c->s));
         c->mg_built = true;
@@ -1154,11 +1384,10 @@ static bool transfer_011(const Level &L) {
 
 // x_f = x_in + P x_c from level l+1 to level l
 template <typename T>
-static void prolong(tb_ctx *c, int l, const T *xin, T *xf) {
+static void prolong(tb_ctx *c, int l, const T *xc, const T *xin, T *xf) {
     const Level &L = c->lev[l];
     LV<T> lv = lv_of<T>(L);
     const Level &Cl = c->lev[l + 1];
-    MGBuf<T> CB = bufs<T>(c, l + 1);
     LV<T> cv = lv_of<T>(Cl);
     if (transfer_011(L)) {
         KTiles t;
@@ -1168,13 +1397,12 @@ static void prolong(tb_ctx *c, int l, const T *xin, T *xf) {
         t.kc = (L.nz + PR_KC - 1) / PR_KC;
         t.n = (long)t.tx * t.ty * t.kc;
         t.per_xcd = (t.n + NXCD - 1) / NXCD;
-        hipLaunchKernelGGL(k_prolong_011<T>, ktiles_grid(t), tiles_block(), 0, c->s, lv, cv, (const T *)CB.x, xin, xf,
-                           t);
+        hipLaunchKernelGGL(k_prolong_011<T>, ktiles_grid(t), tiles_block(), 0, c->s, lv, cv, xc, xin, xf, t);
         return;
     }
     Tiles all = make_tiles(L.nx, L.ny, L.nz);
-    hipLaunchKernelGGL(k_prolong<T>, tiles_grid(all), tiles_block(), 0, c->s, lv, cv, L.cx, L.cy, L.cz,
-                       (const T *)CB.x, xin, xf, all);
+    hipLaunchKernelGGL(k_prolong<T>, tiles_grid(all), tiles_block(), 0, c->s, lv, cv, L.cx, L.cy, L.cz, xc, xin, xf,
+                       all);
 }
 
 // b of level l+1 from the residual of x on level l (fused for (0, 1, 1))
@@ -1201,8 +1429,20 @@ static void resid_restrict(tb_ctx *c, int l, const T *x) {
     Tiles all = make_tiles(L.nx, L.ny, L.nz);
     hipLaunchKernelGGL(k_mg_resid<T>, tiles_grid(all), tiles_block(), 0, c->s, lv, x, (const T *)B.b, B.r, all);
     Tiles call = make_tiles(Cl.nx, Cl.ny, Cl.nz);
-    hipLaunchKernelGGL(k_restrict<T>, tiles_grid(call), tiles_block(), 0, c->s, lv, cv, L.cx, L.cy, L.cz,
-                       (const T *)B.r, CB.b, call);
+    const T *r = B.r;
+    const dim3 g = tiles_grid(call), blk = tiles_block();
+    switch (L.cx * 4 + L.cy * 2 + L.cz) {
+    case 1: hipLaunchKernelGGL((k_restrict_t<T, 0, 0, 1>), g, blk, 0, c->s, lv, cv, r, CB.b, call); break;
+    case 2: hipLaunchKernelGGL((k_restrict_t<T, 0, 1, 0>), g, blk, 0, c->s, lv, cv, r, CB.b, call); break;
+    case 3: hipLaunchKernelGGL((k_restrict_t<T, 0, 1, 1>), g, blk, 0, c->s, lv, cv, r, CB.b, call); break;
+    case 4: hipLaunchKernelGGL((k_restrict_t<T, 1, 0, 0>), g, blk, 0, c->s, lv, cv, r, CB.b, call); break;
+    case 5: hipLaunchKernelGGL((k_restrict_t<T, 1, 0, 1>), g, blk, 0, c->s, lv, cv, r, CB.b, call); break;
+    case 6: hipLaunchKernelGGL((k_restrict_t<T, 1, 1, 0>), g, blk, 0, c->s, lv, cv, r, CB.b, call); break;
+    case 7: hipLaunchKernelGGL((k_restrict_t<T, 1, 1, 1>), g, blk, 0, c->s, lv, cv, r, CB.b, call); break;
+    default:
+        hipLaunchKernelGGL(k_restrict<T>, g, blk, 0, c->s, lv, cv, L.cx, L.cy, L.cz, r, CB.b, call);
+        break;
+    }
 }
 
 // symmetric sweep pairs from zero on a coarsest level too large for the
@@ -1210,7 +1450,47 @@ static void resid_restrict(tb_ctx *c, int l, const T *x) {
 constexpr int COARSE_SWEEPS = 50;
 
 template <typename T>
-static int vcycle_level(tb_ctx *c, int l) {
+static void launch_blk(tb_ctx *c, int l, int np, bool pre, const T *src, T *dst, const T *xc) {
+    const Level &L = c->lev[l];
+    MGBuf<T> B = bufs<T>(c, l);
+    LV<T> lv = lv_of<T>(L);
+    const Level &Cl = c->lev[l + 1];
+    LV<T> cv = lv_of<T>(Cl);
+    BlkShape S;
+    int lds = 0;
+    blk_shape(L, np, sizeof(T), S, lds);   // checked by blk_usable
+    const dim3 grid((unsigned)(S.per_xcd * NXCD)), blk(BLK_THREADS);
+    const T *b = B.b;
+    if (pre && np == 1)
+        hipLaunchKernelGGL((k_rbgs_blk<T, 1, true, false, 0>), grid, blk, lds, c->s, lv, src, b, dst, S, cv, L.cx, L.cy,
+                           L.cz, xc);
+    else if (pre)
+        hipLaunchKernelGGL((k_rbgs_blk<T, 2, true, false, 0>), grid, blk, lds, c->s, lv, src, b, dst, S, cv, L.cx, L.cy,
+                           L.cz, xc);
+    else if (np == 1)
+        hipLaunchKernelGGL((k_rbgs_blk<T, 1, false, true, 1>), grid, blk, lds, c->s, lv, src, b, dst, S, cv, L.cx, L.cy,
+                           L.cz, xc);
+    else
+        hipLaunchKernelGGL((k_rbgs_blk<T, 2, false, true, 1>), grid, blk, lds, c->s, lv, src, b, dst, S, cv, L.cx, L.cy,
+                           L.cz, xc);
+}
+
+// the blocked smoother serves a level below the finest that has no pair
+// buffers and at most BLK_MAX_CELLS cells, with 1 or 2 sweep pairs per side
+// and a box whose LDS fits
+static bool blk_usable(const tb_ctx *c, int l, int pre, int post, size_t tsize) {
+    if (!c->mg_blocked || l == 0 || pre < 1 || pre > 2 || post < 1 || post > 2) return false;
+    if ((long)c->lev[l].nx * c->lev[l].ny * c->lev[l].nz > BLK_MAX_CELLS) return false;
+    BlkShape S;
+    int lds = 0;
+    return blk_shape(c->lev[l], pre, tsize, S, lds) && blk_shape(c->lev[l], post, tsize, S, lds);
+}
+
+// One V-cycle from level l down; *res receives the buffer that holds the
+// level's result (x, or r for the blocked smoother, whose post-smoother
+// cannot overwrite the x its neighbours' halos read).
+template <typename T>
+static int vcycle_level(tb_ctx *c, int l, const T **res) {
     const Level &L = c->lev[l];
     MGBuf<T> B = bufs<T>(c, l);
     LV<T> lv = lv_of<T>(L);
@@ -1233,6 +1513,7 @@ static int vcycle_level(tb_ctx *c, int l) {
             sweep(0, 0);
         }
         TB_LAUNCH_CHECK(c);
+        *res = B.x;
         return 0;
     }
     if (l == c->small_l0[sizeof(T) == 8 ? 1 : 0]) {
@@ -1261,6 +1542,7 @@ static int vcycle_level(tb_ctx *c, int l) {
         TB_PROF(c, TB_K_MG_SMALL,
                 hipLaunchKernelGGL(k_small_cycle<T>, dim3(1), dim3(1024), small_lds_bytes<T>(c, l), c->s, A));
         TB_LAUNCH_CHECK(c);
+        *res = B.x;
         return 0;
     }
     Tiles all = make_tiles(L.nx, L.ny, L.nz);
@@ -1294,15 +1576,13 @@ static int vcycle_level(tb_ctx *c, int l) {
             hipLaunchKernelGGL(k_zero<T>, tiles_grid(all), tiles_block(), 0, c->s, lv, B.x, all);
         }
         TB_PROF(c, TB_K_MG_TRANSFER, resid_restrict<T>(c, l, (const T *)cur));
-        const Level &Cl = c->lev[l + 1];
-        MGBuf<T> CB = bufs<T>(c, l + 1);
-        LV<T> cv = lv_of<T>(Cl);
         TB_LAUNCH_CHECK(c);
-        if (vcycle_level<T>(c, l + 1)) return -1;
+        const T *xc = nullptr;
+        if (vcycle_level<T>(c, l + 1, &xc)) return -1;
         // the post pairs alternate buffers, so the prolonged iterate goes to
         // the buffer from which they end in x
         T *dst = (post % 2 == 0) ? B.x : B.a;
-        TB_PROF(c, TB_K_MG_TRANSFER, prolong<T>(c, l, (const T *)cur, dst));
+        TB_PROF(c, TB_K_MG_TRANSFER, prolong<T>(c, l, xc, (const T *)cur, dst));
         cur = dst;
         // on the finest level the last pair also forms the partial sums of
         // r . z (b . x) for the Krylov method around the cycle
@@ -1312,6 +1592,20 @@ static int vcycle_level(tb_ctx *c, int l) {
             c->vc_dot_blocks[sizeof(T) == 8 ? 1 : 0] = dot ? (int)(P.per_xcd * NXCD) : 0;
         }
         TB_LAUNCH_CHECK(c);
+        *res = B.x;
+        return 0;
+    }
+    if (blk_usable(c, l, pre, post, sizeof(T))) {
+        // pre-smoother from x = 0 into x, residual and restriction, the
+        // coarse levels, then prolongation + post-smoother from x into r
+        TB_PROF(c, kid, launch_blk<T>(c, l, pre, true, nullptr, B.x, nullptr));
+        TB_PROF(c, TB_K_MG_TRANSFER, resid_restrict<T>(c, l, (const T *)B.x));
+        TB_LAUNCH_CHECK(c);
+        const T *xc = nullptr;
+        if (vcycle_level<T>(c, l + 1, &xc)) return -1;
+        TB_PROF(c, kid, launch_blk<T>(c, l, post, false, (const T *)B.x, B.r, xc));
+        TB_LAUNCH_CHECK(c);
+        *res = B.r;
         return 0;
     }
     auto smooth = [&](int color, int zero_init) {
@@ -1333,17 +1627,16 @@ static int vcycle_level(tb_ctx *c, int l) {
         hipLaunchKernelGGL(k_zero<T>, tiles_grid(all), tiles_block(), 0, c->s, lv, B.x, all);
     }
     TB_PROF(c, TB_K_MG_TRANSFER, resid_restrict<T>(c, l, (const T *)B.x));
-    const Level &Cl = c->lev[l + 1];
-    MGBuf<T> CB = bufs<T>(c, l + 1);
-    LV<T> cv = lv_of<T>(Cl);
     TB_LAUNCH_CHECK(c);
-    if (vcycle_level<T>(c, l + 1)) return -1;
-    TB_PROF(c, TB_K_MG_TRANSFER, prolong<T>(c, l, (const T *)B.x, B.x));
+    const T *xc = nullptr;
+    if (vcycle_level<T>(c, l + 1, &xc)) return -1;
+    TB_PROF(c, TB_K_MG_TRANSFER, prolong<T>(c, l, xc, (const T *)B.x, B.x));
     for (int s = 0; s < post; s++) {
         smooth(1, 0);
         smooth(0, 0);
     }
     TB_LAUNCH_CHECK(c);
+    *res = B.x;
     return 0;
 }
 
@@ -1354,11 +1647,12 @@ static int vcycle_level(tb_ctx *c, int l) {
 // Profiling runs launch the kernels directly, so their events are recorded.
 template <typename T>
 static int vcycle_run(tb_ctx *c, int which) {
-    if (!c->use_graphs || c->prof_on) return vcycle_level<T>(c, 0);
+    const T *res = nullptr;
+    if (!c->use_graphs || c->prof_on) return vcycle_level<T>(c, 0, &res);
     if (!c->vc_exec[which]) {
         hipGraph_t g = nullptr;
         TB_CHECK(c, hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal));
-        const int rc = vcycle_level<T>(c, 0);
+        const int rc = vcycle_level<T>(c, 0, &res);
         const hipError_t e = hipStreamEndCapture(c->s, &g);
         if (rc || e != hipSuccess) {
             if (e == hipSuccess && g) hipGraphDestroy(g);
@@ -1406,6 +1700,20 @@ extern "C" int tb_set_mg_finest_sweeps(tb_ctx *c, int pre32, int post32, int pre
     c->mg_post0[0] = post32;
     c->mg_pre0[1] = pre64;
     c->mg_post0[1] = post64;
+    for (int a = 0; a < 2; a++) {
+        if (c->vc_exec[a]) hipGraphExecDestroy(c->vc_exec[a]);
+        c->vc_exec[a] = nullptr;
+    }
+    return 0;
+}
+
+extern "C" int tb_set_mg_smoother(tb_ctx *c, int blocked) {
+    if (!c) return -1;
+    if (blocked != 0 && blocked != 1) {
+        c->err = "tb_set_mg_smoother: blocked must be 0 or 1";
+        return -1;
+    }
+    c->mg_blocked = blocked != 0;
     for (int a = 0; a < 2; a++) {
         if (c->vc_exec[a]) hipGraphExecDestroy(c->vc_exec[a]);
         c->vc_exec[a] = nullptr;
