@@ -248,6 +248,7 @@ struct Level {
     float *x32, *b32, *r32;
     double *a64;          // second iterate buffer of levels smoothed by fused pairs, else null
     float *a32;
+    bool pairs;           // smoothed by fused sweep pairs
 };
 
 // ---------------------------------------------------------------------------

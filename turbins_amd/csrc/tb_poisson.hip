@@ -865,8 +865,11 @@ __global__ void k_coarse(const T *__restrict__ Minv, int n, const T *__restrict_
 // otherwise cost far more than their work.  Level l0 is read from (b) and
 // written to (x) global memory once; one residual buffer serves all levels.
 // The first level run this way is the largest one that has at most
-// SMALL_LEVEL_CELLS cells and whose levels fit SMALL_LDS_BYTES.
-constexpr long SMALL_LEVEL_CELLS = 8192;
+// SMALL_LEVEL_CELLS cells and whose levels fit SMALL_LDS_BYTES.  One CU is
+// the limit: an 8192-cell level costs ~40 us there against ~22 us as blocked
+// launches over the chip (measured at 512x256x128), so the cycle starts at
+// 1024 cells.
+constexpr long SMALL_LEVEL_CELLS = 1024;
 constexpr int SMALL_LDS_BYTES = 156 * 1024;
 constexpr int MAXLEV = 16;
 
@@ -892,17 +895,33 @@ static int small_lds_bytes(const tb_ctx *c, int l0) {
     return (int)(sizeof(double) * (1024 + 32) + sizeof(T) * e);
 }
 
+// a / d and a % d for 0 <= a < 2^22, d >= 1, from the float reciprocal inv
+// of d: the float quotient is within 1 of the true one, and one correction
+// step fixes it (a handful of instructions where an integer division takes
+// dozens, which every thread of a 1024-thread block would pay)
+__device__ inline int divmod_small(int a, int d, float inv, int &rem) {
+    int q = (int)((float)a * inv);
+    int r = a - q * d;
+    if (r < 0) { q--; r += d; }
+    else if (r >= d) { q++; r -= d; }
+    rem = r;
+    return q;
+}
+
 // Calls f(i, j, k) for this thread's cells of an nx x ny x nz box, cell q =
 // threadIdx.x + s * blockDim.x in x-fastest order; the position advances by
-// carries instead of a division per cell.
+// carries instead of a division per cell.  Boxes have fewer than 2^22 cells.
 template <typename F>
 __device__ inline void block_cells(int nx, int ny, int nz, F f) {
     const int n = nx * ny * nz;
+    const float inx = 1.0f / (float)nx, iny = 1.0f / (float)ny;
     int q = threadIdx.x;
-    int i = q % nx, r = q / nx;
-    int j = r % ny, k = r / ny;
-    const int di = blockDim.x % nx, dr = blockDim.x / nx;
-    const int dj = dr % ny, dk = dr / ny;
+    int i, j;
+    const int r = divmod_small(q, nx, inx, i);
+    int k = divmod_small(r, ny, iny, j);
+    int di, dj;
+    const int dr = divmod_small((int)blockDim.x, nx, inx, di);
+    const int dk = divmod_small(dr, ny, iny, dj);
     for (; q < n; q += blockDim.x) {
         f(i, j, k);
         i += di;
@@ -1249,7 +1268,8 @@ int tb_mg_build(tb_ctx *c) {
         // second iterate buffers of the fused sweep pairs (both precisions;
         // with two barriers per plane the FP32 pair beats two half sweeps
         // even though those find their arrays in the 256 MB Infinity Cache)
-        if ((long)L.nx * L.ny * L.nz >= PAIR_LEVEL_CELLS && L.ntot < (1L << 31)) {
+        L.pairs = (long)L.nx * L.ny * L.nz >= PAIR_LEVEL_CELLS && L.ntot < (1L << 31);
+        if (L.pairs) {
             TB_CHECK(c, hipMalloc((void **)&L.a64, sizeof(double) * L.ntot));
             TB_CHECK(c, hipMemsetAsync(L.a64, 0, sizeof(double) * L.ntot, c->s));
             TB_CHECK(c, hipMalloc((void **)&L.a32, sizeof(float) * L.ntot));
@@ -1449,8 +1469,24 @@ static void resid_restrict(tb_ctx *c, int l, const T *x) {
 // dense inverse (mg_build): forward pairs, then as many backward ones
 constexpr int COARSE_SWEEPS = 50;
 
+// the two kinds of blocked launch: pre-smoothing from zero, prolongation +
+// post-smoothing
+enum { BLK_ZERO = 0, BLK_PROL };
+
+template <typename T, int NP>
+static void launch_blk_np(tb_ctx *c, const dim3 &grid, int lds, int kind, const LV<T> &lv, const T *src,
+                          const T *b, T *dst, const BlkShape &S, const LV<T> &cv, const Level &L, const T *xc) {
+    const dim3 blk(BLK_THREADS);
+    if (kind == BLK_ZERO)
+        hipLaunchKernelGGL((k_rbgs_blk<T, NP, true, false, 0>), grid, blk, lds, c->s, lv, src, b, dst, S, cv, L.cx,
+                           L.cy, L.cz, xc);
+    else
+        hipLaunchKernelGGL((k_rbgs_blk<T, NP, false, true, 1>), grid, blk, lds, c->s, lv, src, b, dst, S, cv, L.cx,
+                           L.cy, L.cz, xc);
+}
+
 template <typename T>
-static void launch_blk(tb_ctx *c, int l, int np, bool pre, const T *src, T *dst, const T *xc) {
+static void launch_blk(tb_ctx *c, int l, int np, int kind, const T *src, T *dst, const T *xc) {
     const Level &L = c->lev[l];
     MGBuf<T> B = bufs<T>(c, l);
     LV<T> lv = lv_of<T>(L);
@@ -1459,27 +1495,16 @@ static void launch_blk(tb_ctx *c, int l, int np, bool pre, const T *src, T *dst,
     BlkShape S;
     int lds = 0;
     blk_shape(L, np, sizeof(T), S, lds);   // checked by blk_usable
-    const dim3 grid((unsigned)(S.per_xcd * NXCD)), blk(BLK_THREADS);
-    const T *b = B.b;
-    if (pre && np == 1)
-        hipLaunchKernelGGL((k_rbgs_blk<T, 1, true, false, 0>), grid, blk, lds, c->s, lv, src, b, dst, S, cv, L.cx, L.cy,
-                           L.cz, xc);
-    else if (pre)
-        hipLaunchKernelGGL((k_rbgs_blk<T, 2, true, false, 0>), grid, blk, lds, c->s, lv, src, b, dst, S, cv, L.cx, L.cy,
-                           L.cz, xc);
-    else if (np == 1)
-        hipLaunchKernelGGL((k_rbgs_blk<T, 1, false, true, 1>), grid, blk, lds, c->s, lv, src, b, dst, S, cv, L.cx, L.cy,
-                           L.cz, xc);
-    else
-        hipLaunchKernelGGL((k_rbgs_blk<T, 2, false, true, 1>), grid, blk, lds, c->s, lv, src, b, dst, S, cv, L.cx, L.cy,
-                           L.cz, xc);
+    const dim3 grid((unsigned)(S.per_xcd * NXCD));
+    if (np == 1) launch_blk_np<T, 1>(c, grid, lds, kind, lv, src, (const T *)B.b, dst, S, cv, L, xc);
+    else launch_blk_np<T, 2>(c, grid, lds, kind, lv, src, (const T *)B.b, dst, S, cv, L, xc);
 }
 
-// the blocked smoother serves a level below the finest that has no pair
-// buffers and at most BLK_MAX_CELLS cells, with 1 or 2 sweep pairs per side
-// and a box whose LDS fits
+// the blocked smoother serves a level below the finest without fused pairs
+// that has at most BLK_MAX_CELLS cells, with 1 or 2 sweep pairs per side and
+// a box whose LDS fits
 static bool blk_usable(const tb_ctx *c, int l, int pre, int post, size_t tsize) {
-    if (!c->mg_blocked || l == 0 || pre < 1 || pre > 2 || post < 1 || post > 2) return false;
+    if (!c->mg_blocked || l == 0 || pre < 1 || pre > 2 || post < 1 || post > 2 || c->lev[l].pairs) return false;
     if ((long)c->lev[l].nx * c->lev[l].ny * c->lev[l].nz > BLK_MAX_CELLS) return false;
     BlkShape S;
     int lds = 0;
@@ -1553,7 +1578,7 @@ static int vcycle_level(tb_ctx *c, int l, const T **res) {
     const int post = (l == 0 && c->mg_post0[prec] > 0) ? c->mg_post0[prec] : c->cfg.mg_post;
     // large levels: fused sweep pairs alternating between x and the level's
     // second buffer; cur is the buffer holding the current iterate
-    const bool pairs = B.a != nullptr;
+    const bool pairs = L.pairs;
     T *cur = B.x;
     if (pairs) {
         const PairTiles P = make_pair_tiles(L.nx, L.ny, L.nz, pair_chunk(L));
@@ -1598,12 +1623,12 @@ static int vcycle_level(tb_ctx *c, int l, const T **res) {
     if (blk_usable(c, l, pre, post, sizeof(T))) {
         // pre-smoother from x = 0 into x, residual and restriction, the
         // coarse levels, then prolongation + post-smoother from x into r
-        TB_PROF(c, kid, launch_blk<T>(c, l, pre, true, nullptr, B.x, nullptr));
+        TB_PROF(c, kid, launch_blk<T>(c, l, pre, BLK_ZERO, nullptr, B.x, nullptr));
         TB_PROF(c, TB_K_MG_TRANSFER, resid_restrict<T>(c, l, (const T *)B.x));
         TB_LAUNCH_CHECK(c);
         const T *xc = nullptr;
         if (vcycle_level<T>(c, l + 1, &xc)) return -1;
-        TB_PROF(c, kid, launch_blk<T>(c, l, post, false, (const T *)B.x, B.r, xc));
+        TB_PROF(c, kid, launch_blk<T>(c, l, post, BLK_PROL, (const T *)B.x, B.r, xc));
         TB_LAUNCH_CHECK(c);
         *res = B.r;
         return 0;
