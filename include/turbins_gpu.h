@@ -536,7 +536,7 @@ int tb_set_helmholtz_method(tb_ctx *ctx, int method);
  * leaves the refinement passes without FP32 work, so every mixed solve
  * hands over to FP64 GMG-PCG at once (st->fallback bit 1) */
 int tb_set_mixed_inner(tb_ctx *ctx, double reduction_floor, int maxit);
-/* learned mode: number of previous solutions (1..6, default 6) whose
+/* learned mode: number of previous solutions (1..10, default 8) whose
  * A-norm-optimal combination, with the network's direction when one is
  * attached and its gate open, is the initial guess (the reference starts
  * from the previous pressure alone, Solver.c:256) */

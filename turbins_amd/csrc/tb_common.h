@@ -358,6 +358,7 @@ struct tb_ctx {
 
     // Poisson history for the learned guess
     double *phi_hist[GUESS_HIST] = {};   // last Poisson solutions, newest first
+    float *hist32[GUESS_HIST] = {};      // the same in FP32: the Galerkin guess's directions
     int n_hist = 0;
     // Galerkin matrix of the stored solutions, gram[i][j] = phi_i . A phi_j,
     // valid for the leading gram_n entries.  A phi_j = b_j holds to the solve

@@ -125,6 +125,8 @@ extern "C" tb_ctx *tb_create(const tb_config *cfg) {
     for (int q = 0; ok && q < 4; q++) ok = dev_alloc(c, (void **)&c->st[q], (size_t)c->d.ntot) == 0;
     for (int a = 0; ok && a < 8; a++) ok = dev_alloc(c, (void **)&c->kv[a], fb) == 0;
     for (int a = 0; ok && a < GUESS_HIST; a++) ok = dev_alloc(c, (void **)&c->phi_hist[a], fb) == 0;
+    for (int a = 0; ok && a < GUESS_HIST; a++)
+        ok = dev_alloc(c, (void **)&c->hist32[a], sizeof(float) * (size_t)c->d.ntot) == 0;
     ok = ok && dev_alloc(c, (void **)&c->red, sizeof(double) * RED_SLOTS * RED_MAX_BLOCKS) == 0;
     ok = ok && dev_alloc(c, (void **)&c->scal, sizeof(double) * S_COUNT) == 0;
     ok = ok && hipHostMalloc((void **)&c->hscal, sizeof(double) * S_COUNT, hipHostMallocCoherent) == hipSuccess;
@@ -179,7 +181,7 @@ extern "C" void tb_destroy(tb_ctx *c) {
     for (int f = 0; f < TB_F_COUNT; f++) hipFree(c->f[f]);
     for (int q = 0; q < 4; q++) { hipFree(c->old[q]); hipFree(c->st[q]); free_ib(c, q); }
     for (int a = 0; a < 8; a++) hipFree(c->kv[a]);
-    for (int a = 0; a < GUESS_HIST; a++) hipFree(c->phi_hist[a]);
+    for (int a = 0; a < GUESS_HIST; a++) { hipFree(c->phi_hist[a]); hipFree(c->hist32[a]); }
     for (auto &e : c->prof_ev) hipEventDestroy(e);
     hipFree(c->red);
     hipFree(c->scal);

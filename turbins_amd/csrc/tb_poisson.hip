@@ -2136,9 +2136,24 @@ static int mixed_ir(tb_ctx *c, const PoisCtx &P, double *x, const double *b, dou
 // number of directions used in ndir (0: none usable; the caller then restores
 // phi_{n-1}).
 
+// The directions are the stored solutions rounded to FP32: the guess only
+// has to be good to the ~1e-3 relative residual it starts the solve from,
+// and the dot products and the combination, which stream every direction,
+// move half the bytes.  Products and sums stay in FP64; every direction is
+// pinned as its FP64 original was (zeros round to zeros).
 struct Dirs {
-    const double *g[GUESS_HIST];    // stored solutions, newest first
+    const float *g[GUESS_HIST];     // stored solutions, newest first
 };
+
+// phi_hist[0] = x and hist32[0] = (float) x in one pass
+__global__ void __launch_bounds__(PT) k_hist_push(long n, const double *__restrict__ x, double *__restrict__ h64,
+                                                  float *__restrict__ h32) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const double v = x[i];
+        h64[i] = v;
+        h32[i] = (float)v;
+    }
+}
 
 // per block: the H sums phi_k . b, then with the network's direction d and
 // its image q = A d the H sums phi_k . q, d . b and d . q
@@ -2162,7 +2177,7 @@ __global__ void __launch_bounds__(PT) k_guess_dots(long n, Dirs D, const double 
         }
 #pragma unroll
         for (int u = 0; u < H; u++) {
-            const double g = D.g[u][i];
+            const double g = (double)D.g[u][i];
             a[u] += g * bi;
             if (NET) a[H + u] += g * qi;
         }
@@ -2205,7 +2220,7 @@ __global__ void __launch_bounds__(PT) k_combine(long n, double *__restrict__ x, 
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
         double g[M];
 #pragma unroll
-        for (int u = 0; u < M; u++) g[u] = D.g[u][i];
+        for (int u = 0; u < M; u++) g[u] = (double)D.g[u][i];
         const double dv = d ? d[i] : 0.0;
         double s = 0.0;
 #pragma unroll
@@ -2277,7 +2292,7 @@ static int learned_guess(tb_ctx *c, const PoisCtx &P, double *x, const double *b
     if (H > c->gram_n) H = c->gram_n;
     if (H < 1) return 0;
     Dirs D;
-    for (int k = 0; k < H; k++) D.g[k] = c->phi_hist[k];
+    for (int k = 0; k < H; k++) D.g[k] = c->hist32[k];
     // the network reads the previous two solutions
     const bool use_d = c->net && c->net_open && c->n_hist >= 2;
     if (use_d) {
@@ -2547,9 +2562,16 @@ static int poisson_solve(tb_ctx *c, tb_solve_stats *st, int guess_f, bool output
         // the history and its Galerkin matrix stay as they are
     } else if (c->cfg.pois_mode == TB_POIS_LEARNED_IR || c->net || c->net_record) {
         double *oldest = c->phi_hist[GUESS_HIST - 1];
-        for (int k = GUESS_HIST - 1; k > 0; k--) c->phi_hist[k] = c->phi_hist[k - 1];
+        float *oldest32 = c->hist32[GUESS_HIST - 1];
+        for (int k = GUESS_HIST - 1; k > 0; k--) {
+            c->phi_hist[k] = c->phi_hist[k - 1];
+            c->hist32[k] = c->hist32[k - 1];
+        }
         c->phi_hist[0] = oldest;
-        TB_CHECK(c, hipMemcpyAsync(c->phi_hist[0], x, sizeof(double) * c->d.ntot, hipMemcpyDeviceToDevice, c->s));
+        c->hist32[0] = oldest32;
+        hipLaunchKernelGGL(k_hist_push, dim3(PB), dim3(PT), 0, c->s, c->d.ntot, (const double *)x, c->phi_hist[0],
+                           c->hist32[0]);
+        TB_LAUNCH_CHECK(c);
         if (c->n_hist < GUESS_HIST) c->n_hist++;
         c->phi_is_hist0 = true;
         // the last residual pass of every solve above ran on the final x
