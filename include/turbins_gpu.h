@@ -546,11 +546,14 @@ int tb_set_guess_history(tb_ctx *ctx, int h);
  * keep; 1..8 otherwise).  No reference counterpart: hypre's BoomerAMG
  * settings (Solver.c:90-265) are fixed there. */
 int tb_set_mg_finest_sweeps(tb_ctx *ctx, int pre32, int post32, int pre64, int post64);
-/* smoother of the multigrid levels between the finest and the LDS-resident
- * coarse cycle: 1 (default) one launch per V-cycle side that runs all of its
- * sweep pairs (and, after the coarse levels, the prolongation) per box in
- * LDS; 0 one launch per half sweep.  Both give the same bits.  No
- * reference counterpart (hypre's BoomerAMG, Solver.c:90-265). */
+/* multigrid launch structure: 1 (default) the levels of at most 2^18 cells
+ * between the finest and the LDS-resident coarse cycle run each V-cycle
+ * side as one launch (all sweep pairs and, after the coarse levels, the
+ * prolongation, per box in LDS), and the FP32 fused-pair levels coarsened
+ * in y and z form the prolongation inside their first post-smoothing pair;
+ * 0 one launch per half sweep and a separate prolongation.  Both give the
+ * same bits.  No reference counterpart (hypre's BoomerAMG,
+ * Solver.c:90-265). */
 int tb_set_mg_smoother(tb_ctx *ctx, int blocked);
 /* multigrid V-cycles replayed from HIP graphs (default 1) or launched
  * kernel by kernel (0); results are identical */

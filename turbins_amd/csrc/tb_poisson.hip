@@ -682,11 +682,14 @@ static PairTiles make_pair_tiles(int nx, int ny, int nz, int tk) {
 
 // DOT: the block's partial sum of b . xn over its own cells (the final
 // values of both colours) goes to part[blockIdx.x] -- r . z of a V-cycle
-// whose last sweep pair this is.
-template <typename T, int F, bool ZERO, bool DOT>
-__global__ void __launch_bounds__(PAIR_THREADS, sizeof(T) == 4 ? 4 : 2)
+// whose last sweep pair this is.  PROL (levels coarsened by (0, 1, 1)): the
+// pair starts from xo + P xc, the coarse correction prolonged as
+// k_prolong_011 does it (same expression, so the same bits), formed as each
+// plane is staged instead of in a pass of its own.
+template <typename T, int F, bool ZERO, bool DOT, bool PROL = false>
+__global__ void __launch_bounds__(PAIR_THREADS, (sizeof(T) == 4 && !PROL) ? 4 : 2)
     k_rbgs_pair(LV<T> L, const T *__restrict__ xo, T *__restrict__ xn, const T *__restrict__ b, PairTiles P,
-                double *__restrict__ part) {
+                double *__restrict__ part, LV<T> C = LV<T>(), const T *__restrict__ xc = nullptr) {
     __shared__ T so[PAIR_ORING][PAIR_OY * PAIR_OX];
     __shared__ T sr[3][PAIR_HY * PAIR_HX];
     const int nx = L.nx, ny = L.ny, nz = L.nz, px = L.px;
@@ -717,18 +720,69 @@ __global__ void __launch_bounds__(PAIR_THREADS, sizeof(T) == 4 ? 4 : 2)
         ooff[u] = (e < PAIR_OY * PAIR_OX && i >= 0 && i < nx && j >= 0 && j < ny) ? j * px + i : -1;
     }
     T ro[PAIR_OLD_PER], ro2[PAIR_OLD_PER];   // xo of the next two planes to stage
+    // PROL: the coarse values of each staged point, rows J0 (a) and J1 (b)
+    // of coarse planes K0 = m / 2 (0) and its neighbour K1 (1)
+    T ca0[PAIR_OLD_PER], cb0[PAIR_OLD_PER], ca1[PAIR_OLD_PER], cb1[PAIR_OLD_PER];
+    T ca0n[PAIR_OLD_PER], cb0n[PAIR_OLD_PER], ca1n[PAIR_OLD_PER], cb1n[PAIR_OLD_PER];
+    int coff0[PAIR_OLD_PER], coff1[PAIR_OLD_PER];
+    const int cpl = C.px * C.py;
 #pragma unroll
-    for (int u = 0; u < PAIR_OLD_PER; u++) ro2[u] = (T)0;
-    auto fetch_old = [&](int m, T *r) {
+    for (int u = 0; u < PAIR_OLD_PER; u++) {
+        ro2[u] = (T)0;
+        ca0[u] = cb0[u] = ca1[u] = cb1[u] = (T)0;
+        ca0n[u] = cb0n[u] = ca1n[u] = cb1n[u] = (T)0;
+        coff0[u] = coff1[u] = 0;
+        if (PROL && ooff[u] >= 0) {
+            const int e = t + u * PAIR_THREADS;
+            const int r = e / PAIR_OX, cc = e - r * PAIR_OX;
+            const int i = i0 - 2 + cc, j = j0 - 2 + r;
+            const int J0 = j >> 1, J1 = clampi((j & 1) ? J0 + 1 : J0 - 1, 0, C.ny - 1);
+            coff0[u] = J0 * C.px + i;
+            coff1[u] = J1 * C.px + i;
+        }
+    }
+    // staging in three steps on named register arrays (the compiler keeps
+    // them in registers only while no pointer to them is taken): fetch the
+    // plane after next into the *n set, shift it to the current set, store
+    // the current set's plane into its LDS slot
+    auto fetch_next = [&](int m) {
+        int K0 = 0, K1 = 0;
+        if (PROL) {
+            K0 = m >> 1;
+            K1 = (m & 1) ? (K0 + 1 < C.nz ? K0 + 1 : C.nz - 1) : (K0 > 0 ? K0 - 1 : 0);
+        }
 #pragma unroll
-        for (int u = 0; u < PAIR_OLD_PER; u++) r[u] = ooff[u] >= 0 ? xo[m * plane + ooff[u]] : (T)0;
+        for (int u = 0; u < PAIR_OLD_PER; u++) {
+            ro2[u] = ooff[u] >= 0 ? xo[m * plane + ooff[u]] : (T)0;
+            if (PROL) {   // unconditional (offset 0 outside the box): no branch around the loads
+                ca0n[u] = xc[K0 * cpl + coff0[u]];
+                cb0n[u] = xc[K0 * cpl + coff1[u]];
+                ca1n[u] = xc[K1 * cpl + coff0[u]];
+                cb1n[u] = xc[K1 * cpl + coff1[u]];
+            }
+        }
     };
-    auto put_old = [&](int m, const T *r) {
+    auto shift = [&]() {
+#pragma unroll
+        for (int u = 0; u < PAIR_OLD_PER; u++) {
+            ro[u] = ro2[u];
+            if (PROL) { ca0[u] = ca0n[u]; cb0[u] = cb0n[u]; ca1[u] = ca1n[u]; cb1[u] = cb1n[u]; }
+        }
+    };
+    auto put_cur = [&](int m) {
         T *S = so[m % PAIR_ORING];
+        const T wj1 = (T)0.25, wj0 = (T)1 - wj1, wk1 = (T)0.25, wk0 = (T)1 - wk1;
 #pragma unroll
         for (int u = 0; u < PAIR_OLD_PER; u++) {
             const int e = t + u * PAIR_THREADS;
-            if (e < PAIR_OY * PAIR_OX) S[e] = r[u];
+            if (e < PAIR_OY * PAIR_OX) {
+                T v = ro[u];
+                if (PROL) {
+                    const T s = wk0 * (wj0 * ca0[u] + wj1 * cb0[u]) + wk1 * (wj0 * ca1[u] + wj1 * cb1[u]);
+                    v = ooff[u] >= 0 ? v + s : (T)0;
+                }
+                S[e] = v;
+            }
         }
     };
     int rrow[PAIR_RED_PER], rc2[PAIR_RED_PER];
@@ -779,19 +833,19 @@ __global__ void __launch_bounds__(PAIR_THREADS, sizeof(T) == 4 ? 4 : 2)
     for (int u = 0; u < PAIR_RED_PER; u++) { rb_n[u] = (T)0; rv_n[u] = false; }
     T bb_n = (T)0;
     if (!ZERO) {
-        if (k0 >= 2) { fetch_old(k0 - 2, ro); put_old(k0 - 2, ro); }
-        if (k0 >= 1) { fetch_old(k0 - 1, ro); put_old(k0 - 1, ro); }
-        fetch_old(k0, ro);
-        if (k0 + 1 < nz) fetch_old(k0 + 1, ro2);
+        if (k0 >= 2) { fetch_next(k0 - 2); shift(); put_cur(k0 - 2); }
+        if (k0 >= 1) { fetch_next(k0 - 1); shift(); put_cur(k0 - 1); }
+        fetch_next(k0);
+        shift();
+        if (k0 + 1 < nz) fetch_next(k0 + 1);
     }
     if (k0 >= 1) fetch_red(k0 - 1, rb, rv);
     if (k0 < nz) fetch_red(k0, rb_n, rv_n);
     for (int m = k0 - 1; m <= k1; m++) {
         if (!ZERO) {
-            if (m + 1 < nz) put_old(m + 1, ro);
-#pragma unroll
-            for (int u = 0; u < PAIR_OLD_PER; u++) ro[u] = ro2[u];
-            if (m + 3 < nz) fetch_old(m + 3, ro2);
+            if (m + 1 < nz) put_cur(m + 1);
+            shift();
+            if (m + 3 < nz) fetch_next(m + 3);
         }
         T rb_nn[PAIR_RED_PER];
         bool rv_nn[PAIR_RED_PER];
@@ -1583,20 +1637,34 @@ static int vcycle_level(tb_ctx *c, int l, const T **res) {
     if (pairs) {
         const PairTiles P = make_pair_tiles(L.nx, L.ny, L.nz, pair_chunk(L));
         const dim3 grid(P.per_xcd * NXCD), blk(PAIR_THREADS);
-        auto pair = [&](int first, bool zero, bool dot) {
-            T *dst = zero ? B.x : (cur == B.x ? B.a : B.x);
+        // the first post pair forms the prolongation itself on (0, 1, 1)
+        // levels; the pairs then write pre + post times, so the first one
+        // goes to the buffer that makes the last one land in x
+        const bool fuse = sizeof(T) == 4 && c->mg_blocked && transfer_011(L) && pre > 0 && post > 0;
+        T *zdst = (fuse && (pre + post) % 2 == 0) ? B.a : B.x;
+        const Level &Cl = c->lev[l + 1];
+        const LV<T> cv = lv_of<T>(Cl);
+        auto pair = [&](int first, bool zero, bool dot, const T *pxc) {
+            T *dst = zero ? zdst : (cur == B.x ? B.a : B.x);
             const T *src = cur;
+            const T *bb = B.b;
             if (c->prof_on && kid >= 0) tb_prof_begin(c, kid);
-            if (zero) hipLaunchKernelGGL((k_rbgs_pair<T, 0, true, false>), grid, blk, 0, c->s, lv, src, dst, (const T *)B.b, P, nullptr);
-            else if (first == 0) hipLaunchKernelGGL((k_rbgs_pair<T, 0, false, false>), grid, blk, 0, c->s, lv, src, dst, (const T *)B.b, P, nullptr);
-            else if (!dot) hipLaunchKernelGGL((k_rbgs_pair<T, 1, false, false>), grid, blk, 0, c->s, lv, src, dst, (const T *)B.b, P, nullptr);
-            else hipLaunchKernelGGL((k_rbgs_pair<T, 1, false, true>), grid, blk, 0, c->s, lv, src, dst, (const T *)B.b, P, c->red);
+            if (zero) hipLaunchKernelGGL((k_rbgs_pair<T, 0, true, false>), grid, blk, 0, c->s, lv, src, dst, bb, P, nullptr, lv, nullptr);
+            else if (first == 0) hipLaunchKernelGGL((k_rbgs_pair<T, 0, false, false>), grid, blk, 0, c->s, lv, src, dst, bb, P, nullptr, lv, nullptr);
+            else if (pxc) {
+                // FP32 only: the FP64 form needs more registers than two blocks per CU leave
+                if constexpr (sizeof(T) == 4) {
+                    if (!dot) hipLaunchKernelGGL((k_rbgs_pair<T, 1, false, false, true>), grid, blk, 0, c->s, lv, src, dst, bb, P, nullptr, cv, pxc);
+                    else hipLaunchKernelGGL((k_rbgs_pair<T, 1, false, true, true>), grid, blk, 0, c->s, lv, src, dst, bb, P, c->red, cv, pxc);
+                }
+            } else if (!dot) hipLaunchKernelGGL((k_rbgs_pair<T, 1, false, false>), grid, blk, 0, c->s, lv, src, dst, bb, P, nullptr, lv, nullptr);
+            else hipLaunchKernelGGL((k_rbgs_pair<T, 1, false, true>), grid, blk, 0, c->s, lv, src, dst, bb, P, c->red, lv, nullptr);
             if (c->prof_on && kid >= 0) tb_prof_end(c, kid);
             cur = dst;
         };
         if (pre > 0) {
-            pair(0, true, false);
-            for (int s = 1; s < pre; s++) pair(0, false, false);
+            pair(0, true, false, nullptr);
+            for (int s = 1; s < pre; s++) pair(0, false, false, nullptr);
         } else {
             hipLaunchKernelGGL(k_zero<T>, tiles_grid(all), tiles_block(), 0, c->s, lv, B.x, all);
         }
@@ -1604,15 +1672,17 @@ static int vcycle_level(tb_ctx *c, int l, const T **res) {
         TB_LAUNCH_CHECK(c);
         const T *xc = nullptr;
         if (vcycle_level<T>(c, l + 1, &xc)) return -1;
-        // the post pairs alternate buffers, so the prolonged iterate goes to
-        // the buffer from which they end in x
-        T *dst = (post % 2 == 0) ? B.x : B.a;
-        TB_PROF(c, TB_K_MG_TRANSFER, prolong<T>(c, l, xc, (const T *)cur, dst));
-        cur = dst;
+        if (!fuse) {
+            // the post pairs alternate buffers, so the prolonged iterate goes
+            // to the buffer from which they end in x
+            T *dst = (post % 2 == 0) ? B.x : B.a;
+            TB_PROF(c, TB_K_MG_TRANSFER, prolong<T>(c, l, xc, (const T *)cur, dst));
+            cur = dst;
+        }
         // on the finest level the last pair also forms the partial sums of
         // r . z (b . x) for the Krylov method around the cycle
         const bool dot = l == 0 && post > 0;
-        for (int s = 0; s < post; s++) pair(1, false, dot && s == post - 1);
+        for (int s = 0; s < post; s++) pair(1, false, dot && s == post - 1, (fuse && s == 0) ? xc : nullptr);
         if (l == 0) {
             c->vc_dot_blocks[sizeof(T) == 8 ? 1 : 0] = dot ? (int)(P.per_xcd * NXCD) : 0;
         }
