@@ -188,13 +188,23 @@ __global__ void k_ib_apply(const long *__restrict__ row, const int *__restrict__
     int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= n) return;
     double s = 0.0, dg = 1.0;
-    long r = row[a];
-    int m = cnt[a];
-    for (int e = 0; e < m; e++) {
-        long cidx = col[(long)a * 9 + e];
-        double v = val[(long)a * 9 + e];
-        if (cidx == r) dg = v;
-        s += v * x[cidx];
+    const long r = row[a];
+    const int m = cnt[a];
+    // loads first, then the sum in column order (as in k_ib_gs)
+    long ci[9];
+    double cv[9], xv[9];
+#pragma unroll
+    for (int e = 0; e < 9; e++) {
+        ci[e] = e < m ? col[(long)a * 9 + e] : r;
+        cv[e] = e < m ? val[(long)a * 9 + e] : 0.0;
+    }
+#pragma unroll
+    for (int e = 0; e < 9; e++) xv[e] = e < m ? x[ci[e]] : 0.0;
+#pragma unroll
+    for (int e = 0; e < 9; e++) {
+        if (e >= m) continue;
+        if (ci[e] == r) dg = cv[e];
+        s += cv[e] * xv[e];
     }
     double f = (target > 0.0) ? target / dg : 1.0;
     y[r] = (f > 1.0) ? f * s : s;
@@ -621,12 +631,24 @@ __global__ void k_ib_gs(const long *__restrict__ row, const int *__restrict__ cn
     int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= n) return;
     const long r = row[a];
+    const int m = cnt[a];
+    // every column and value, then every gather, issued before the first
+    // is used; the sum runs in column order as a loop over them would
+    long ci[9];
+    double cv[9], xv[9];
+#pragma unroll
+    for (int e = 0; e < 9; e++) {
+        ci[e] = e < m ? col[(long)a * 9 + e] : r;
+        cv[e] = e < m ? val[(long)a * 9 + e] : 0.0;
+    }
+#pragma unroll
+    for (int e = 0; e < 9; e++) xv[e] = (e < m && ci[e] != r) ? x[ci[e]] : 0.0;
     double s = 0.0, dg = 1.0;
-    for (int e = 0; e < cnt[a]; e++) {
-        const long cidx = col[(long)a * 9 + e];
-        const double v = val[(long)a * 9 + e];
-        if (cidx == r) dg = v;
-        else s += v * x[cidx];
+#pragma unroll
+    for (int e = 0; e < 9; e++) {
+        if (e >= m) continue;
+        if (ci[e] == r) dg = cv[e];
+        else s += cv[e] * xv[e];
     }
     tmp[a] = (b[r] - s) / dg;
 }
