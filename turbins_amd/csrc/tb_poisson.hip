@@ -687,7 +687,7 @@ static PairTiles make_pair_tiles(int nx, int ny, int nz, int tk) {
 // k_prolong_011 does it (same expression, so the same bits), formed as each
 // plane is staged instead of in a pass of its own.
 template <typename T, int F, bool ZERO, bool DOT, bool PROL = false>
-__global__ void __launch_bounds__(PAIR_THREADS, (sizeof(T) == 4 && !PROL) ? 4 : 2)
+__global__ void __launch_bounds__(PAIR_THREADS, sizeof(T) == 4 ? 4 : 2)
     k_rbgs_pair(LV<T> L, const T *__restrict__ xo, T *__restrict__ xn, const T *__restrict__ b, PairTiles P,
                 double *__restrict__ part, LV<T> C = LV<T>(), const T *__restrict__ xc = nullptr) {
     __shared__ T so[PAIR_ORING][PAIR_OY * PAIR_OX];
@@ -721,16 +721,15 @@ __global__ void __launch_bounds__(PAIR_THREADS, (sizeof(T) == 4 && !PROL) ? 4 : 
     }
     T ro[PAIR_OLD_PER], ro2[PAIR_OLD_PER];   // xo of the next two planes to stage
     // PROL: the coarse values of each staged point, rows J0 (a) and J1 (b)
-    // of coarse planes K0 = m / 2 (0) and its neighbour K1 (1)
+    // of coarse planes K0 = m / 2 (0) and its neighbour K1 (1), fetched one
+    // plane ahead of their use (the xo values two)
     T ca0[PAIR_OLD_PER], cb0[PAIR_OLD_PER], ca1[PAIR_OLD_PER], cb1[PAIR_OLD_PER];
-    T ca0n[PAIR_OLD_PER], cb0n[PAIR_OLD_PER], ca1n[PAIR_OLD_PER], cb1n[PAIR_OLD_PER];
     int coff0[PAIR_OLD_PER], coff1[PAIR_OLD_PER];
     const int cpl = C.px * C.py;
 #pragma unroll
     for (int u = 0; u < PAIR_OLD_PER; u++) {
         ro2[u] = (T)0;
         ca0[u] = cb0[u] = ca1[u] = cb1[u] = (T)0;
-        ca0n[u] = cb0n[u] = ca1n[u] = cb1n[u] = (T)0;
         coff0[u] = coff1[u] = 0;
         if (PROL && ooff[u] >= 0) {
             const int e = t + u * PAIR_THREADS;
@@ -741,32 +740,28 @@ __global__ void __launch_bounds__(PAIR_THREADS, (sizeof(T) == 4 && !PROL) ? 4 : 
             coff1[u] = J1 * C.px + i;
         }
     }
-    // staging in three steps on named register arrays (the compiler keeps
-    // them in registers only while no pointer to them is taken): fetch the
-    // plane after next into the *n set, shift it to the current set, store
-    // the current set's plane into its LDS slot
+    // staging on named register arrays (the compiler keeps them in
+    // registers only while no pointer to them is taken): fetch the xo plane
+    // after next into ro2, shift it to ro, fetch its coarse values, store
+    // ro's plane into its LDS slot
     auto fetch_next = [&](int m) {
-        int K0 = 0, K1 = 0;
-        if (PROL) {
-            K0 = m >> 1;
-            K1 = (m & 1) ? (K0 + 1 < C.nz ? K0 + 1 : C.nz - 1) : (K0 > 0 ? K0 - 1 : 0);
-        }
 #pragma unroll
-        for (int u = 0; u < PAIR_OLD_PER; u++) {
-            ro2[u] = ooff[u] >= 0 ? xo[m * plane + ooff[u]] : (T)0;
-            if (PROL) {   // unconditional (offset 0 outside the box): no branch around the loads
-                ca0n[u] = xc[K0 * cpl + coff0[u]];
-                cb0n[u] = xc[K0 * cpl + coff1[u]];
-                ca1n[u] = xc[K1 * cpl + coff0[u]];
-                cb1n[u] = xc[K1 * cpl + coff1[u]];
-            }
-        }
+        for (int u = 0; u < PAIR_OLD_PER; u++) ro2[u] = ooff[u] >= 0 ? xo[m * plane + ooff[u]] : (T)0;
     };
     auto shift = [&]() {
 #pragma unroll
-        for (int u = 0; u < PAIR_OLD_PER; u++) {
-            ro[u] = ro2[u];
-            if (PROL) { ca0[u] = ca0n[u]; cb0[u] = cb0n[u]; ca1[u] = ca1n[u]; cb1[u] = cb1n[u]; }
+        for (int u = 0; u < PAIR_OLD_PER; u++) ro[u] = ro2[u];
+    };
+    auto fetch_coarse = [&](int m) {
+        if (!PROL) return;
+        const int K0 = m >> 1;
+        const int K1 = (m & 1) ? (K0 + 1 < C.nz ? K0 + 1 : C.nz - 1) : (K0 > 0 ? K0 - 1 : 0);
+#pragma unroll
+        for (int u = 0; u < PAIR_OLD_PER; u++) {   // offset 0 outside the box: no branch around the loads
+            ca0[u] = xc[K0 * cpl + coff0[u]];
+            cb0[u] = xc[K0 * cpl + coff1[u]];
+            ca1[u] = xc[K1 * cpl + coff0[u]];
+            cb1[u] = xc[K1 * cpl + coff1[u]];
         }
     };
     auto put_cur = [&](int m) {
@@ -833,10 +828,11 @@ __global__ void __launch_bounds__(PAIR_THREADS, (sizeof(T) == 4 && !PROL) ? 4 : 
     for (int u = 0; u < PAIR_RED_PER; u++) { rb_n[u] = (T)0; rv_n[u] = false; }
     T bb_n = (T)0;
     if (!ZERO) {
-        if (k0 >= 2) { fetch_next(k0 - 2); shift(); put_cur(k0 - 2); }
-        if (k0 >= 1) { fetch_next(k0 - 1); shift(); put_cur(k0 - 1); }
+        if (k0 >= 2) { fetch_next(k0 - 2); shift(); fetch_coarse(k0 - 2); put_cur(k0 - 2); }
+        if (k0 >= 1) { fetch_next(k0 - 1); shift(); fetch_coarse(k0 - 1); put_cur(k0 - 1); }
         fetch_next(k0);
         shift();
+        fetch_coarse(k0);
         if (k0 + 1 < nz) fetch_next(k0 + 1);
     }
     if (k0 >= 1) fetch_red(k0 - 1, rb, rv);
@@ -845,6 +841,7 @@ __global__ void __launch_bounds__(PAIR_THREADS, (sizeof(T) == 4 && !PROL) ? 4 : 
         if (!ZERO) {
             if (m + 1 < nz) put_cur(m + 1);
             shift();
+            if (m + 2 < nz) fetch_coarse(m + 2);
             if (m + 3 < nz) fetch_next(m + 3);
         }
         T rb_nn[PAIR_RED_PER];
