@@ -687,7 +687,7 @@ static PairTiles make_pair_tiles(int nx, int ny, int nz, int tk) {
 // k_prolong_011 does it (same expression, so the same bits), formed as each
 // plane is staged instead of in a pass of its own.
 template <typename T, int F, bool ZERO, bool DOT, bool PROL = false>
-__global__ void __launch_bounds__(PAIR_THREADS, sizeof(T) == 4 ? 4 : 2)
+__global__ void __launch_bounds__(PAIR_THREADS, sizeof(T) == 4 ? ((PROL && !DOT) ? 8 : 4) : 2)
     k_rbgs_pair(LV<T> L, const T *__restrict__ xo, T *__restrict__ xn, const T *__restrict__ b, PairTiles P,
                 double *__restrict__ part, LV<T> C = LV<T>(), const T *__restrict__ xc = nullptr) {
     __shared__ T so[PAIR_ORING][PAIR_OY * PAIR_OX];
