@@ -47,7 +47,7 @@ METRIC = "timesteps/s at 512x256x128, 1e-8 Poisson residual: plain vs learned+mi
 HBM_PEAK_GBS = 8000.0
 
 # algorithmic bytes per point (each array read or written once per point at
-# its storage width); see DESIGN.md for the table
+# its storage width)
 BYTES_PER_POINT = {
     # ENO as tb_step runs it, with the right-hand sides in the same pass:
     # momentum u, v, w, three flags, three pressure gradients and c in, three
@@ -254,8 +254,8 @@ def main():
     ap.add_argument("--cpu-reps", type=int, default=3)
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
-              "decomposition is a design (DESIGN.md), not a measured path", file=sys.stderr)
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the x-slab "
+              "decomposition over several GPUs is designed, not built or measured", file=sys.stderr)
         sys.exit(2)
 
     from turbins_amd import Solver, abi, workload as W

@@ -1,5 +1,5 @@
-"""The reference-side C adaptor of INTEGRATION.md compiles: its ```c block is
-compiled with gcc -std=c99 against include/turbins_gpu.h and stub
+"""The reference-side C adaptor of the integration guide compiles: the ```c
+block of its section 2 is compiled with gcc -std=c99 against include/turbins_gpu.h and stub
 declarations of the reference structs that carry their field names and types
 (DataTypes.h: struct parameters :177-235, struct mac_grid :560-590, struct
 velocity :250-300, struct concentration :8-101, struct pressure :143-174)
@@ -56,7 +56,7 @@ def _snippet():
     text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
     sec = text[text.index("## 2. C binding in the reference driver"):]
     m = re.search(r"```c\n(.*?)```", sec, re.S)
-    assert m, "no C block in INTEGRATION.md section 2"
+    assert m, "no C block in section 2 of the integration guide"
     return m.group(1)
 
 
