@@ -61,7 +61,7 @@ WEIGHTS = os.path.join(ROOT, "turbins_amd", "weights", "poisson_guess.f32")
 # (profiles/hbm_traffic_bump512x256.md); only valid for the kernels as built.
 # HELM_GS: the sweep pair, and the first pair of a solve, which also reads
 # the last solve's increment
-MEASURED_TRAFFIC = {"HELM_GS": 463.8e6, "HELM_GS_FIRST": 640.3e6}
+MEASURED_TRAFFIC = {"HELM_GS": 463.8e6, "HELM_GS_FIRST": 540.2e6}
 
 
 def log(*a):
@@ -152,8 +152,8 @@ def roofline(prof, stats, c):
         elif name == "HELM_GS":
             # one fused red+black sweep pair: x, b and the status byte read
             # once, the new x written once; the first pair of each solve also
-            # reads the last solve's increment (8 B/point)
-            total = 25 * NT * n + 8 * NT * n_first
+            # reads the last solve's increment (FP32, 4 B/point)
+            total = 25 * NT * n + 4 * NT * n_first
         elif name == "RBGS0":
             total = 24 * ncell * n      # FP64 fused pair: x, b read, x written
         elif name == "RBGS0_F32":

@@ -262,7 +262,8 @@ struct HelmHistory {
     int sweeps[2] = {0, 0};
     double rate[2] = {0.0, 0.0};
     unsigned nsolve = 0;
-    double *inc = nullptr;       // the last solve's increment x_new - x_start (allocated on first use)
+    double *buf = nullptr;       // second iterate buffer of the solves (allocated on first use, with inc)
+    float *inc = nullptr;        // the last solve's increment x_new - x_start, in FP32
     bool inc_ok = false;         // inc belongs to this field's current trajectory
 };
 

@@ -67,9 +67,12 @@ static int dev_alloc(tb_ctx *c, void **p, size_t bytes) {
 }
 
 int tb_helm_inc_alloc(tb_ctx *c, int q) {
-    if (c->helm[q].inc) return 0;
-    c->helm[q].inc_ok = false;
-    return dev_alloc(c, (void **)&c->helm[q].inc, sizeof(double) * (size_t)c->d.ntot);
+    HelmHistory &h = c->helm[q];
+    if (h.buf && h.inc) return 0;
+    h.inc_ok = false;
+    if (!h.buf && dev_alloc(c, (void **)&h.buf, sizeof(double) * (size_t)c->d.ntot)) return -1;
+    if (!h.inc && dev_alloc(c, (void **)&h.inc, sizeof(float) * (size_t)c->d.ntot)) return -1;
+    return 0;
 }
 
 double tb_elapsed_ms(tb_ctx *c) {
@@ -174,8 +177,9 @@ extern "C" void tb_destroy(tb_ctx *c) {
         hipFree(c->cs[n].c); hipFree(c->cs[n].conv); hipFree(c->cs[n].rhs); hipFree(c->cs[n].old);
     }
     tb_open_free(c);
-    for (int q = 0; q < 3; q++) hipFree(c->helm[q].inc);
-    for (int n = 0; n < TB_NCONC_MAX; n++) hipFree(c->cs[n].h.inc);   // slot conc_sel's is helm[3].inc (synced)
+    for (int q = 0; q < 3; q++) { hipFree(c->helm[q].buf); hipFree(c->helm[q].inc); }
+    // slot conc_sel's buffers are helm[3]'s (synced)
+    for (int n = 0; n < TB_NCONC_MAX; n++) { hipFree(c->cs[n].h.buf); hipFree(c->cs[n].h.inc); }
     hipFree(c->ctot);
     tb_free_deposits(c, 0);
     for (int f = 0; f < TB_F_COUNT; f++) hipFree(c->f[f]);
@@ -492,11 +496,13 @@ int tb_conc_use(tb_ctx *c, int n) {
     return 0;
 }
 
-static void dev_release(tb_ctx *c, double *&p) {
+// frees a field-sized buffer of T and clears its pointer
+template <typename T>
+static void dev_release(tb_ctx *c, T *&p) {
     if (!p) return;
     hipFree(p);
     p = nullptr;
-    c->bytes -= (int64_t)(sizeof(double) * (size_t)c->d.ntot);
+    c->bytes -= (int64_t)(sizeof(T) * (size_t)c->d.ntot);
 }
 
 extern "C" int tb_set_conc_fields(tb_ctx *c, int nconc, const tb_conc_field *fields, int hindered) {
@@ -525,6 +531,7 @@ extern "C" int tb_set_conc_fields(tb_ctx *c, int nconc, const tb_conc_field *fie
             }
         } else {
             dev_release(c, s.c); dev_release(c, s.conv); dev_release(c, s.rhs); dev_release(c, s.old);
+            dev_release(c, s.h.buf);
             dev_release(c, s.h.inc);
         }
     }
@@ -536,8 +543,10 @@ extern "C" int tb_set_conc_fields(tb_ctx *c, int nconc, const tb_conc_field *fie
     }
     for (int n = 0; n < nconc; n++) {
         c->cs[n].p = fields[n];
-        double *inc = c->cs[n].h.inc;   // kept, its content no longer a guess
+        double *buf = c->cs[n].h.buf;   // kept, the increment's content no longer a guess
+        float *inc = c->cs[n].h.inc;
         c->cs[n].h = HelmHistory();
+        c->cs[n].h.buf = buf;
         c->cs[n].h.inc = inc;
     }
     c->helm[TB_Q_C] = c->cs[0].h;

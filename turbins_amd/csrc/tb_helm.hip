@@ -51,7 +51,7 @@ __device__ inline double wallmod(double ap, double a, int rule) {
 template <int MODE>
 __global__ void __launch_bounds__(256) k_helm_km(HelmOp H, const double *__restrict__ x, double *__restrict__ y,
                                                  const double *__restrict__ w, double *part, KTiles t,
-                                                 const double *__restrict__ x0, double *__restrict__ dn) {
+                                                 const double *__restrict__ x0, float *__restrict__ dn) {
     __shared__ double sh[32];
     double acc[2] = {0.0, 0.0};
     int i, j, k0, k1;
@@ -159,7 +159,7 @@ __global__ void __launch_bounds__(256) k_helm_km(HelmOp H, const double *__restr
             } else if (MODE < 3) {
                 st_stream(&y[id], yv);
             }
-            if (MODE == 4) st_stream(&dn[id], xc - x0v);
+            if (MODE == 4) st_stream(&dn[id], (float)(xc - x0v));
             if (MODE == 1) acc[0] += wv * yv;
             if (MODE >= 3) { acc[0] += (wv - yv) * (wv - yv); acc[1] += wv * wv; }
             if (MODE == 2) { acc[0] += xc * yv; acc[1] += yv * yv; }
@@ -231,7 +231,7 @@ static HelmOp make_op(tb_ctx *c, int q, double dt) {
 // y = A x (scaled: IB rows rescaled for the Krylov solve); mode 1/2/3 leave
 // partial sums in c->red.  Returns the number of partials, or -1.
 static int helm_apply(tb_ctx *c, const HelmOp &H, const double *x, double *y, bool scaled, int mode,
-                      const double *w = nullptr, const double *x0 = nullptr, double *dn = nullptr) {
+                      const double *w = nullptr, const double *x0 = nullptr, float *dn = nullptr) {
     int q = H.q;
     if (c->nib[q] > 0)
         hipLaunchKernelGGL(k_ib_apply, dim3((c->nib[q] + 255) / 256), dim3(256), 0, c->s, c->ib_row[q], c->ib_cnt[q],
@@ -419,7 +419,7 @@ static GsTiles make_gs_tiles(const Dims &d) {
 template <bool ADD>
 __global__ void __launch_bounds__(GS_THREADS) k_helm_gs_pair(HelmOp H, const double *__restrict__ xo,
                                                              double *__restrict__ xn, const double *__restrict__ b,
-                                                             GsTiles T, const double *__restrict__ dd) {
+                                                             GsTiles T, const float *__restrict__ dd) {
     __shared__ double so[GS_ORING][GS_OY * GS_OX];
     __shared__ double sr[3][GS_HY * GS_HX];
     __shared__ AxisGS cx[GS_OX], cy[GS_OY], cz[GS_OZ];
@@ -469,7 +469,7 @@ __global__ void __launch_bounds__(GS_THREADS) k_helm_gs_pair(HelmOp H, const dou
 #pragma unroll
         for (int u = 0; u < GS_OLD_PER; u++) {
             ro[u] = ooff[u] >= 0 ? xo[m * plane + ooff[u]] : 0.0;
-            if (ADD) rd[u] = ooff[u] >= 0 ? dd[m * plane + ooff[u]] : 0.0;
+            if (ADD) rd[u] = ooff[u] >= 0 ? (double)dd[m * plane + ooff[u]] : 0.0;
         }
     };
     auto put_old = [&](int m) {
@@ -662,7 +662,7 @@ __global__ void k_ib_scatter(const long *__restrict__ row, int n, const double *
 // ||b - A x||^2 and ||b||^2 into S_TMP1, S_TMP2 (host mirror)
 // |b - A x|^2 and |b|^2; with x0 given also the increment x - x0 into dn
 static int helm_residual(tb_ctx *c, const HelmOp &H, const double *x, const double *b, double *ibuf, double &rr,
-                         double &bb, const double *x0 = nullptr, double *dn = nullptr) {
+                         double &bb, const double *x0 = nullptr, float *dn = nullptr) {
     int nb = helm_apply(c, H, x, ibuf, false, x0 ? 4 : 3, b, x0, dn);
     if (nb < 0 || tb_reduce_finish(c, nb, 2, S_TMP1)) return -1;
     if (tb_read_scalars(c, S_TMP1, 2)) return -1;
@@ -678,7 +678,7 @@ static int helm_residual(tb_ctx *c, const HelmOp &H, const double *x, const doub
 // be cur itself: the last planes written into cur first are read there only
 // with zero coefficients (no row couples to them).
 static int gs_pairs(tb_ctx *c, const HelmOp &H, double *&cur, double *&alt, const double *b, int n,
-                    const double *src = nullptr, const double *inc = nullptr) {
+                    const double *src = nullptr, const float *inc = nullptr) {
     const int q = H.q;
     double *tmp = c->kv[4];
     const GsTiles T = make_gs_tiles(c->d);
@@ -701,7 +701,7 @@ static int gs_pairs(tb_ctx *c, const HelmOp &H, double *&cur, double *&alt, cons
         } else {
             TB_PROF(c, TB_K_HELM_GS,
                     hipLaunchKernelGGL(k_helm_gs_pair<false>, dim3(T.per_xcd * NXCD), dim3(GS_THREADS), 0, c->s, H,
-                                       from, alt, b, T, (const double *)nullptr));
+                                       from, alt, b, T, (const float *)nullptr));
         }
         double *w = cur;
         cur = alt;
@@ -733,13 +733,16 @@ static int copy_box(tb_ctx *c, double *dst, const double *src) {
 // than 0.5) or run out of sweeps, x then holds the last iterate.  The initial
 // guess is the field plus the increment of the quantity's last solve, when
 // that solve converged on the same trajectory (the reference starts from the
-// field alone; both stop at the same residual test).  Three buffers take
-// part: the field x, read by the first pair and kept so that each residual
-// check also writes the increment, and the two iterate buffers, kv[6] and the
-// increment buffer, which the first pair may read from as it writes kv[6].
-// On convergence the field takes the result's buffer, the increment buffer
-// the new increment's and kv[6] the old field's (no kernel or graph keeps
-// any of these pointers between calls).
+// field alone; both stop at the same residual test).  Three FP64 buffers
+// take part: the field x, read by the first pair and kept so that each
+// residual check also writes the increment, and the two iterate buffers,
+// kv[6] and the quantity's own second buffer.  On convergence the field
+// takes the result's buffer, the second buffer the other iterate's and
+// kv[6] the old field's (no kernel or graph keeps any of these pointers
+// between calls).  The increment is kept in FP32: it only moves the start
+// of the next solve, which converges to the same residual test, and the
+// check that writes it and the first pair that reads it stream half the
+// bytes for it.
 // The true residual costs about two thirds of a sweep pair, so it is
 // evaluated only where the solve may have converged: first after as many
 // pairs as the last solve of the same quantity and RK stage parity needed
@@ -760,9 +763,9 @@ static int helm_gs_solve(tb_ctx *c, const HelmOp &H, double *x, const double *b,
     if (tb_helm_inc_alloc(c, q)) return -1;
     HelmHistory &h = c->helm[q];
     const bool field = x == c->f[TB_F_U + q];
-    const double *inc = (field && h.inc_ok) ? h.inc : nullptr;
-    h.inc_ok = false;   // the increment buffer is overwritten from here on
-    double *cur = h.inc, *alt = c->kv[6];
+    const float *inc = (field && h.inc_ok) ? h.inc : nullptr;
+    h.inc_ok = false;   // the increment is overwritten by the first residual check
+    double *cur = h.buf, *alt = c->kv[6];
     const int par = (int)(h.nsolve++ & 1u);
     int chunk = h.sweeps[par] > 0 ? h.sweeps[par] : 2;
     double prev = NAN, rate = h.rate[par];
@@ -770,9 +773,10 @@ static int helm_gs_solve(tb_ctx *c, const HelmOp &H, double *x, const double *b,
     while (!ok && its < maxit) {
         const int m = chunk < maxit - its ? chunk : maxit - its;
         if (gs_pairs(c, H, cur, alt, b, m, src, inc)) return -1;
-        src = inc = nullptr;
+        src = nullptr;
+        inc = nullptr;
         its += m;
-        if (helm_residual(c, H, cur, b, ibuf, rr, bb, field ? x : nullptr, alt)) return -1;
+        if (helm_residual(c, H, cur, b, ibuf, rr, bb, field ? x : nullptr, field ? h.inc : nullptr)) return -1;
         bn = sqrt(bb);
         if (bn == 0.0) bn = 1.0;
         res = sqrt(rr) / bn;
@@ -794,10 +798,10 @@ static int helm_gs_solve(tb_ctx *c, const HelmOp &H, double *x, const double *b,
         if (prev == prev) h.rate[par] = rate;
     }
     if (ok && field) {
-        // cur: the solution, alt: its increment over x (the iterates never
+        // cur: the solution, h.inc: its increment over x (the iterates never
         // write x, which is now free)
         c->f[TB_F_U + q] = cur;
-        h.inc = alt;
+        h.buf = alt;
         c->kv[6] = x;
         h.inc_ok = true;
     } else if (copy_box(c, x, cur)) {
