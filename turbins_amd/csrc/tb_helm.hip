@@ -464,12 +464,13 @@ __global__ void __launch_bounds__(GS_THREADS) k_helm_gs_pair(HelmOp H, const dou
     }
     // the loads of a plane stay in flight until put_old (with ADD the sum is
     // formed there too, not between the loads)
-    double ro[GS_OLD_PER], rd[GS_OLD_PER];
+    double ro[GS_OLD_PER];
+    float rd[GS_OLD_PER];   // widened where the sum is formed, so the load is not waited for here
     auto fetch_old = [&](int m) {
 #pragma unroll
         for (int u = 0; u < GS_OLD_PER; u++) {
             ro[u] = ooff[u] >= 0 ? xo[m * plane + ooff[u]] : 0.0;
-            if (ADD) rd[u] = ooff[u] >= 0 ? (double)dd[m * plane + ooff[u]] : 0.0;
+            if (ADD) rd[u] = ooff[u] >= 0 ? dd[m * plane + ooff[u]] : 0.0f;
         }
     };
     auto put_old = [&](int m) {
@@ -477,7 +478,7 @@ __global__ void __launch_bounds__(GS_THREADS) k_helm_gs_pair(HelmOp H, const dou
 #pragma unroll
         for (int u = 0; u < GS_OLD_PER; u++) {
             const int e = t + u * GS_THREADS;
-            if (e < GS_OY * GS_OX) S[e] = ADD ? ro[u] + rd[u] : ro[u];
+            if (e < GS_OY * GS_OX) S[e] = ADD ? ro[u] + (double)rd[u] : ro[u];
         }
     };
     // this thread's red points of the halo-1 plane: row, column pair, y
